@@ -1,0 +1,306 @@
+// Direction-optimising BFS for gfx950 (default path of gg_bfs).
+//
+// Same result contract as the reference's BFS program (Algorithms/BFS/bfs.h,
+// checked against CPUBFS bfs.cu:57-111): depth[src] = 0, unreached = -1.
+//
+//   top-down  (queue -> queue): LBS expand over the frontier's out-edges
+//             (k_lbs_count / k_lbs_scan / k_lbs_expand with BfsPushOp);
+//             first touch wins an atomic test-and-set in the visited bitmap
+//             and stamps the depth.
+//   bottom-up (bitmap -> bitmap): one wave per 64 consecutive vertices;
+//             each unvisited vertex scans its in-edges until it finds a
+//             parent in the frontier bitmap (n/8 bytes, L2-resident), the
+//             wave writes its two bitmap words with plain stores.
+//   switch    top-down -> bottom-up when m_f > m_u / alpha; back when the
+//             frontier shrinks below n / beta (Beamer et al., SC'12).
+//
+// Byte model (per level, F frontier, E edges examined, N newly visited,
+// U unvisited vertices scanned): top-down 12F + 8E + 8N; bottom-up
+// 8U + 4E + n/8 + 8N.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "internal.h"
+
+namespace gg {
+namespace {
+
+using namespace GASengine;
+
+// counters: [0] next frontier size, [1] out-degree sum of the new frontier
+// (m_f), [2] edges examined, [3] unvisited vertices scanned.
+struct BfsPushOp {
+  const int32_t *ro;
+  const int32_t *col;
+  int32_t *depth;
+  uint32_t *visited;
+  unsigned long long *counters;
+  int32_t next_depth;
+  unsigned long long mf;
+
+  __device__ __forceinline__ void prologue(int32_t) {}
+  __device__ __forceinline__ int32_t degree(int32_t v) { return ro[v + 1] - ro[v]; }
+  __device__ __forceinline__ int32_t row_begin(int32_t v) { return ro[v]; }
+  __device__ __forceinline__ bool edge(int32_t, int32_t slot, int32_t &out) {
+    const int32_t nb = col[slot];
+    if (!bit_test_and_set(visited, nb)) return false;
+    depth[nb] = next_depth;
+    mf += (unsigned long long)(ro[nb + 1] - ro[nb]);
+    out = nb;
+    return true;
+  }
+  __device__ __forceinline__ void finish() {
+    unsigned long long t = wave_sum(mf);
+    if (lane_id() == 0 && t) atomicAdd(&counters[1], t);
+  }
+};
+
+__global__ void k_bfs_init(int32_t *depth, uint32_t *visited, int32_t *queue, int32_t *qsize, const int32_t *ro,
+                           int32_t src, unsigned long long *counters) {
+  depth[src] = 0;
+  visited[src >> 5] = 1u << (src & 31);
+  queue[0] = src;
+  *qsize = 1;
+  counters[1] = (unsigned long long)(ro[src + 1] - ro[src]);
+}
+
+__global__ void __launch_bounds__(256)
+k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, const int32_t *__restrict__ ro,
+                int32_t *__restrict__ depth, uint32_t *__restrict__ visited, const uint32_t *__restrict__ front,
+                uint32_t *__restrict__ next, int32_t n, int32_t next_depth, unsigned long long *__restrict__ counters) {
+  const int32_t words = (n + 31) >> 5;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int lane = lane_id();
+  unsigned long long found_cnt = 0, mf = 0, examined = 0, scanned = 0;
+  for (int64_t p = (((int64_t)blockIdx.x * blockDim.x) + threadIdx.x) >> 6; p * 64 < n; p += nwaves) {
+    const int32_t w = (int32_t)(2 * p + (lane >> 5));
+    const int32_t v = (int32_t)(p * 64 + lane);
+    bool todo = false;
+    if (v < n) todo = !((visited[w] >> (lane & 31)) & 1u);
+    bool found = false;
+    if (todo) {
+      scanned++;
+      const int32_t b = co[v], e = co[v + 1];
+      for (int32_t k = b; k < e; k++) {
+        examined++;
+        const int32_t u = ri[k];
+        if ((front[u >> 5] >> (u & 31)) & 1u) {
+          found = true;
+          break;
+        }
+      }
+    }
+    const uint64_t mask = ballot(found);
+    if (found) {
+      depth[v] = next_depth;
+      mf += (unsigned long long)(ro[v + 1] - ro[v]);
+    }
+    if (lane < 2) {
+      const int32_t ww = (int32_t)(2 * p + lane);
+      const uint32_t bits = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
+      if (ww < words) {
+        next[ww] = bits;
+        if (bits) visited[ww] |= bits;
+      }
+    }
+    if (lane == 0) found_cnt += (unsigned long long)__popcll(mask);
+  }
+  found_cnt = wave_sum(found_cnt);
+  mf = wave_sum(mf);
+  examined = wave_sum(examined);
+  scanned = wave_sum(scanned);
+  if (lane == 0) {
+    if (found_cnt) atomicAdd(&counters[0], found_cnt);
+    if (mf) atomicAdd(&counters[1], mf);
+    if (examined) atomicAdd(&counters[2], examined);
+    if (scanned) atomicAdd(&counters[3], scanned);
+  }
+}
+
+__global__ void k_queue_to_bits(const int32_t *__restrict__ queue, const int32_t *__restrict__ qsize,
+                                uint32_t *__restrict__ bits) {
+  const int32_t nq = *qsize;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+    const int32_t v = queue[i];
+    atomicOr(&bits[v >> 5], 1u << (v & 31));
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_bits_to_queue(const uint32_t *__restrict__ bits, int32_t words, int32_t *__restrict__ queue,
+                int32_t *__restrict__ qsize) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < words; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t w = base + threadIdx.x;
+    uint32_t word = w < words ? bits[w] : 0u;
+    const int32_t cnt = __popc(word);
+    const int32_t incl = wave_inclusive_scan(cnt);
+    const int32_t total = __shfl(incl, kWave - 1);
+    int32_t start = 0;
+    if (lane_id() == 0 && total) start = atomicAdd(qsize, total);
+    start = __shfl(start, 0) + incl - cnt;
+    while (word) {
+      const int b = __builtin_ctz(word);
+      word &= word - 1;
+      queue[start++] = (int32_t)(w * 32 + b);
+    }
+  }
+}
+
+}  // namespace
+
+int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint32_t flags, gg_stats *stats) {
+  GG_TRY(activate(g));
+  const int32_t n = g->n;
+  const int32_t words = (n + 31) / 32;
+  hipStream_t s = g->stream;
+  GG_TRY(g->frontier[0].ensure(4 * (size_t)n));
+  GG_TRY(g->frontier[1].ensure(4 * (size_t)n));
+  GG_TRY(g->bits[0].ensure(4 * (size_t)words));  // visited
+  GG_TRY(g->bits[1].ensure(4 * (size_t)words));  // frontier bitmaps (ping-pong)
+  GG_TRY(g->bits[2].ensure(4 * (size_t)words));
+  GG_TRY(g->scan_local.ensure(4 * (size_t)n));
+  GG_TRY(g->partials.ensure(4 * ((size_t)n / kCountTile + 2)));
+  GG_TRY(g->info.ensure(sizeof(LbsInfo)));
+  GG_TRY(g->counters.ensure(64 * sizeof(unsigned long long)));
+  int32_t *depth;
+  if (flags & GG_OUT_DEVICE) {
+    GG_CHECK_ARG(depth_out, "depth is NULL");
+    depth = depth_out;
+  } else {
+    GG_TRY(g->vtx[0].ensure(4 * (size_t)n));
+    depth = g->vtx[0].as<int32_t>();
+  }
+  uint32_t *visited = g->bits[0].as<uint32_t>();
+  uint32_t *fbits[2] = {g->bits[1].as<uint32_t>(), g->bits[2].as<uint32_t>()};
+  int32_t *queue[2] = {g->frontier[0].as<int32_t>(), g->frontier[1].as<int32_t>()};
+  // counters: [0..3] per level (see BfsPushOp), [8],[9]: queue sizes
+  unsigned long long *ctr = g->counters.as<unsigned long long>();
+  int32_t *qsize = reinterpret_cast<int32_t *>(ctr + 8);
+  g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
+
+  hipEvent_t e0, e1;
+  GG_HIP(hipEventCreate(&e0));
+  GG_HIP(hipEventCreate(&e1));
+  GG_HIP(hipEventRecord(e0, s));
+  {
+    TimedRegion t(g->prof, "bfs.init");
+    GG_HIP(fill_async<int32_t>(depth, -1, n, s));
+    GG_HIP(fill_async<uint32_t>(visited, 0u, words, s));
+    GG_HIP(fill_async<unsigned long long>(ctr, 0ull, 16, s));
+    hipLaunchKernelGGL(k_bfs_init, dim3(1), dim3(1), 0, s, depth, visited, queue[0], qsize, g->dg.row_offsets, src,
+                       ctr);
+    GG_HIP(hipGetLastError());
+  }
+  g->prof.add_bytes("bfs.init", 4.0 * n + words * 4.0);
+  GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  GG_HIP(hipStreamSynchronize(s));
+  int64_t mf = g->h64()[1];
+  int64_t mu = g->m - mf;
+  int64_t nf = 1, prev_nf = 0;
+  int64_t reached = 1, reached_edges = mf, examined_total = 0;
+  double bytes = 4.0 * n + words * 4.0;
+  const int alpha = g->opt.bfs_alpha, beta = g->opt.bfs_beta;
+  bool bottom_up = false;
+  int cur = 0;  // index of the current queue / frontier bitmap
+  int levels = 0, pull_levels = 0;
+
+  while (nf > 0 && levels < g->opt.iter_num) {
+    // Direction choice.
+    if (!bottom_up && mf > mu / alpha && nf > prev_nf) {
+      TimedRegion t(g->prof, "bfs.queue_to_bits");
+      GG_HIP(fill_async<uint32_t>(fbits[cur], 0u, words, s));
+      hipLaunchKernelGGL(k_queue_to_bits, dim3(grid_for(nf, 256, 4096)), dim3(256), 0, s, queue[cur], qsize + cur,
+                         fbits[cur]);
+      GG_HIP(hipGetLastError());
+      bottom_up = true;
+    } else if (bottom_up && nf < n / beta && nf < prev_nf) {
+      TimedRegion t(g->prof, "bfs.bits_to_queue");
+      GG_HIP(fill_async<int32_t>(qsize + cur, 0, 1, s));
+      hipLaunchKernelGGL(k_bits_to_queue, dim3(grid_for(words, 256, 4096)), dim3(256), 0, s, fbits[cur], words,
+                         queue[cur], qsize + cur);
+      GG_HIP(hipGetLastError());
+      bottom_up = false;
+    }
+    const int nxt = cur ^ 1;
+    GG_HIP(fill_async<unsigned long long>(ctr, 0ull, 4, s));
+    GG_HIP(fill_async<int32_t>(qsize + nxt, 0, 1, s));
+    double lvl_bytes;
+    if (bottom_up) {
+      {
+        TimedRegion t(g->prof, "bfs.bottom_up");
+        hipLaunchKernelGGL(k_bfs_bottom_up, dim3(grid_for((int64_t)words * 32, 256, 8192)), dim3(256), 0, s,
+                           g->dg.column_offsets, g->dg.row_indices, g->dg.row_offsets, depth, visited, fbits[cur],
+                           fbits[nxt], n, levels + 1, ctr);
+        GG_HIP(hipGetLastError());
+      }
+      GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+      GG_HIP(hipStreamSynchronize(s));
+      const int64_t N = g->h64()[0], E = g->h64()[2], U = g->h64()[3];
+      lvl_bytes = 8.0 * U + 4.0 * E + n / 8.0 + 8.0 * N;
+      g->prof.add_bytes("bfs.bottom_up", lvl_bytes);
+      examined_total += E;
+      prev_nf = nf;
+      nf = N;
+      pull_levels++;
+    } else {
+      BfsPushOp op{g->dg.row_offsets, g->dg.column_indices, depth, visited, ctr, levels + 1, 0ull};
+      {
+        TimedRegion t(g->prof, "bfs.lbs_count");
+        hipLaunchKernelGGL((k_lbs_count<BfsPushOp, false>), dim3(grid_for(nf, kCountTile)), dim3(kCountBlock), 0, s,
+                           op, queue[cur], qsize + cur, g->scan_local.as<int32_t>(), g->partials.as<int32_t>());
+      }
+      {
+        TimedRegion t(g->prof, "bfs.lbs_scan");
+        hipLaunchKernelGGL(k_lbs_scan, dim3(1), dim3(kScanBlock), 0, s, qsize + cur, g->partials.as<int32_t>(),
+                           g->info.as<LbsInfo>(), ctr + 2);
+      }
+      {
+        TimedRegion t(g->prof, "bfs.top_down");
+        hipLaunchKernelGGL((k_lbs_expand<BfsPushOp, 8>), dim3(2048), dim3(kExpandBlock), 0, s, op, queue[cur],
+                           qsize + cur, g->scan_local.as<int32_t>(), g->partials.as<int32_t>(),
+                           g->info.as<LbsInfo>(), queue[nxt], qsize + nxt);
+      }
+      GG_HIP(hipGetLastError());
+      GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 10 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+      GG_HIP(hipStreamSynchronize(s));
+      const int64_t E = g->h64()[2];
+      const int64_t N = reinterpret_cast<int32_t *>(g->h_pinned + 16)[nxt];
+      lvl_bytes = 12.0 * nf + 8.0 * E + 8.0 * N;
+      g->prof.add_bytes("bfs.lbs_count", 12.0 * nf);
+      g->prof.add_bytes("bfs.top_down", 8.0 * E + 8.0 * N);
+      examined_total += E;
+      prev_nf = nf;
+      nf = N;
+    }
+    bytes += lvl_bytes;
+    mf = g->h64()[1];
+    mu -= mf;
+    reached += nf;
+    reached_edges += mf;
+    levels++;
+    cur = nxt;
+  }
+  GG_HIP(hipEventRecord(e1, s));
+  GG_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  finish_profile(g);
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->iterations = levels;
+    stats->pull_iterations = pull_levels;
+    stats->reached_vertices = reached;
+    stats->reached_edges = reached_edges;
+    stats->edges_inspected = examined_total;
+    stats->alg_bytes = bytes;
+    stats->gpu_ms = ms;
+  }
+  if (flags & GG_OUT_DEVICE) return GG_OK;
+  return copy_out(g, depth_out, depth, 4 * (size_t)n, flags);
+}
+
+}  // namespace gg

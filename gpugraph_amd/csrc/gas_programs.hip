@@ -1,0 +1,193 @@
+// C ABI runs of the four vertex programs through the generic GAS engine
+// (EnactorVertexCentric<CsrProblem<Program>, Program>):
+//   gg_sssp (default path), gg_cc (label propagation), gg_bfs with
+//   GG_BFS_PUSH_ONLY, gg_pagerank_reference.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstring>
+#include <string>
+
+#include "internal.h"
+#include "programs/bfs.h"
+#include "programs/cc.h"
+#include "programs/pagerank.h"
+#include "programs/sssp.h"
+
+namespace gg {
+
+// Times the enactor's kernel classes under "<program>.<kernel>".
+struct PrefixTimer : GASengine::KernelTimer {
+  EventProfiler &p;
+  std::string prefix;
+  std::string cur;
+  PrefixTimer(EventProfiler &p_, const char *pre) : p(p_), prefix(pre) {}
+  void begin(const char *name) override {
+    cur = prefix + "." + name;
+    p.begin(cur.c_str());
+  }
+  void end(const char *name) override {
+    cur = prefix + "." + name;
+    p.end(cur.c_str());
+  }
+};
+
+template <class P>
+struct Slot;
+template <>
+struct Slot<bfs> {
+  static constexpr int index = 0;
+  static constexpr const char *name = "bfs_gas";
+  static constexpr int default_threshold = 0;  // Algorithms/BFS/register.h:41
+};
+template <>
+struct Slot<sssp> {
+  static constexpr int index = 1;
+  static constexpr const char *name = "sssp";
+  static constexpr int default_threshold = 10000;  // Algorithms/SSSP/register.h:39
+};
+template <>
+struct Slot<cc> {
+  static constexpr int index = 2;
+  static constexpr const char *name = "cc";
+  static constexpr int default_threshold = 10000;  // Algorithms/CC/register.h:39
+};
+template <>
+struct Slot<pagerank> {
+  static constexpr int index = 3;
+  static constexpr const char *name = "pagerank_ref";
+  static constexpr int default_threshold = 1000;  // Algorithms/PageRank/register.h:40
+};
+
+template <class P>
+using Problem = GASengine::CsrProblem<P>;
+
+template <class P>
+int get_problem(gg_graph *g, Problem<P> **out) {
+  auto &slot = g->prog_state[Slot<P>::index];
+  if (!slot) {
+    auto *p = new (std::nothrow) Problem<P>();
+    if (!p) return set_error(GG_ERR_OOM, "host allocation failed");
+    hipError_t e = p->Attach(g->dg, g->stream);
+    if (e != hipSuccess) {
+      delete p;
+      return set_error(e == hipErrorOutOfMemory ? GG_ERR_OOM : GG_ERR_HIP, "workspace allocation: %s",
+                       hipGetErrorString(e));
+    }
+    slot = std::shared_ptr<void>(p, [](void *q) {
+      auto *pp = static_cast<Problem<P> *>(q);
+      P::Release(pp->vertex_list, pp->edge_list);
+      delete pp;
+    });
+  }
+  auto *p = static_cast<Problem<P> *>(slot.get());
+  p->stream = g->stream;
+  *out = p;
+  return GG_OK;
+}
+
+// Algorithmic bytes per iteration (frontier F, edges E, next frontier N).
+template <class P>
+double iteration_bytes(const GASengine::IterationRecord &r) {
+  const double F = r.frontier, E = (double)r.edges, N = r.next;
+  if (std::is_same<P, sssp>::value) return 16 * F + 12 * E + 8 * N;
+  if (std::is_same<P, cc>::value) return 16 * F + 8 * E + 8 * N;
+  if (std::is_same<P, bfs>::value) return 12 * F + 8 * E + 8 * N;
+  return 12 * F + 12 * E + 8 * N;  // pagerank: gather reads ri + rank + outdeg per in-edge
+}
+
+template <class P>
+int run_program(gg_graph *g, int num_srcs, int *srcs, typename P::DataType *out, uint32_t flags, gg_stats *stats,
+                typename P::DataType sentinel, bool count_reach) {
+  GG_TRY(activate(g));
+  Problem<P> *prob = nullptr;
+  GG_TRY(get_problem<P>(g, &prob));
+  GASengine::EnactorVertexCentric<Problem<P>, P> enactor;
+  enactor.config.wave_degree = g->opt.wave_degree;
+  enactor.config.block_degree = g->opt.block_degree;
+  g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
+  PrefixTimer timer(g->prof, Slot<P>::name);
+  if (g->prof.enabled) enactor.timer = &timer;
+  const int threshold = g->opt.threshold >= 0 ? g->opt.threshold : Slot<P>::default_threshold;
+
+  hipEvent_t e0, e1;
+  GG_HIP(hipEventCreate(&e0));
+  GG_HIP(hipEventCreate(&e1));
+  GG_HIP(hipEventRecord(e0, g->stream));
+  hipError_t err = enactor.EnactIterativeSearch(*prob, nullptr, g->directed, num_srcs, srcs, g->opt.iter_num,
+                                                threshold);
+  (void)hipEventRecord(e1, g->stream);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (err != hipSuccess)
+    return set_error(err == hipErrorOutOfMemory ? GG_ERR_OOM : GG_ERR_HIP, "%s: %s", Slot<P>::name,
+                     hipGetErrorString(err));
+  typename P::DataType *res = P::resultPointer(prob->vertex_list);
+  if (g->prof.enabled) {
+    for (auto &r : enactor.history) {
+      const double b = iteration_bytes<P>(r);
+      const std::string base = std::string(Slot<P>::name) + ".";
+      // Edge traffic to the expand kernel, per-vertex traffic to the count pass.
+      const double fb = b - (std::is_same<P, cc>::value ? 8 : 12) * (double)r.edges - 8.0 * r.next;
+      g->prof.add_bytes((base + (r.lbs ? "lbs_expand" : "tiered_expand")).c_str(), b - fb);
+      g->prof.add_bytes((base + (r.lbs ? "lbs_count" : "prologue")).c_str(), fb);
+    }
+    finish_profile(g);
+  }
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->iterations = (int32_t)enactor.history.size();
+    stats->edges_inspected = enactor.total_edges;
+    stats->gpu_ms = ms;
+    for (auto &r : enactor.history) stats->alg_bytes += iteration_bytes<P>(r);
+    if (count_reach) GG_TRY(reach_stats(g, (const int32_t *)res, *(int32_t *)&sentinel, stats));
+  }
+  return copy_out(g, out, res, sizeof(typename P::DataType) * (size_t)g->n, flags);
+}
+
+}  // namespace gg
+
+// Declared in bfs_do.hip / cc_hook.hip.
+namespace gg {
+int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth, uint32_t flags, gg_stats *stats);
+int cc_hooking(gg_graph *g, int32_t *label, uint32_t flags, gg_stats *stats);
+}  // namespace gg
+
+extern "C" {
+
+int gg_bfs(gg_graph *g, int32_t src, int32_t *depth, uint32_t flags, gg_stats *stats) {
+  gg::clear_error();
+  GG_CHECK_ARG(g, "graph is NULL");
+  GG_CHECK_ARG(src >= 0 && src < g->n, "src %d out of range [0, %d)", src, g->n);
+  if (!(flags & GG_BFS_PUSH_ONLY)) return gg::bfs_direction_optimizing(g, src, depth, flags, stats);
+  int s = src;
+  return gg::run_program<bfs>(g, 1, &s, depth, flags, stats, -1, true);
+}
+
+int gg_sssp(gg_graph *g, int32_t src, int32_t *dist, uint32_t flags, gg_stats *stats) {
+  gg::clear_error();
+  GG_CHECK_ARG(g, "graph is NULL");
+  GG_CHECK_ARG(src >= 0 && src < g->n, "src %d out of range [0, %d)", src, g->n);
+  GG_CHECK_ARG(g->value_dtype == GG_VALUES_I32, "SSSP needs int32 edge weights");
+  int s = src;
+  return gg::run_program<sssp>(g, 1, &s, dist, flags, stats, sssp::INIT_VALUE, true);
+}
+
+int gg_cc(gg_graph *g, int32_t *label, uint32_t flags, gg_stats *stats) {
+  gg::clear_error();
+  GG_CHECK_ARG(g, "graph is NULL");
+  if (flags & GG_CC_HOOK) return gg::cc_hooking(g, label, flags, stats);
+  // Every vertex is labelled: reach_stats counts all of them.
+  return gg::run_program<cc>(g, 0, nullptr, label, flags, stats, INT_MIN, true);
+}
+
+int gg_pagerank_reference(gg_graph *g, float *rank, uint32_t flags, gg_stats *stats) {
+  gg::clear_error();
+  GG_CHECK_ARG(g, "graph is NULL");
+  return gg::run_program<pagerank>(g, 0, nullptr, rank, flags, stats, 0.0f, false);
+}
+
+}  // extern "C"

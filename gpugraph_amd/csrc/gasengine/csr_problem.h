@@ -1,0 +1,227 @@
+// CsrProblem: device graph + per-iteration state of a GAS run, the role of
+// the reference's CsrProblem/GraphSlice (GASengine/csr_problem.cuh:117-1046).
+//
+// Layout in HBM (n vertices, m edges, all int32 unless noted):
+//   CSR  row_offsets[n+1], column_indices[m], edge_values[m]
+//   CSC  column_offsets[n+1], row_indices[m], csc_edge_ids[m]   (directed only;
+//        undirected graphs use the CSR as their own CSC)
+//   frontier[2][n]        ping-pong vertex queues (no O(m) edge frontier: the
+//                         reference's EDGE_FRONTIERS queues are 1.5*m each)
+//   dedup[2][ceil(n/32)]  bitmaps deduplicating the next frontier, alternated
+//                         per iteration and cleared word-wise by the consumer
+//   changed[n] (char), gather_tmp[n] (GatherType)
+//   scan_local[n], partials[ceil(n/1024)], LbsInfo, sizes[2], counters[8]
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "Enums.h"
+#include "lbs.h"
+#include "program_ops.h"
+
+namespace GASengine {
+
+// Stream on which programs' Initialize/extractResult issue their work; set by
+// the enactor around those calls (the reference uses the default stream).
+inline hipStream_t &engine_stream() {
+  static thread_local hipStream_t s = nullptr;
+  return s;
+}
+
+#define GAS_RETURN_IF(expr)            \
+  do {                                 \
+    hipError_t _e = (expr);            \
+    if (_e != hipSuccess) return _e;   \
+  } while (0)
+
+// Engine buffers, sized for n vertices.  Reused across runs.
+struct Workspace {
+  int32_t capacity = 0;
+  size_t gather_bytes = 0;
+  int32_t *frontier[2] = {nullptr, nullptr};
+  uint32_t *dedup[2] = {nullptr, nullptr};
+  char *changed = nullptr;
+  void *gather_tmp = nullptr;
+  int32_t *scan_local = nullptr;
+  int32_t *partials = nullptr;
+  LbsInfo *info = nullptr;
+  int32_t *sizes = nullptr;                   // [0],[1]: frontier sizes
+  unsigned long long *counters = nullptr;     // [0]: edges inspected
+  int32_t *frontier_values[3] = {nullptr, nullptr, nullptr};
+  int32_t *h_sizes = nullptr;                 // pinned readback: [0] size, [2..3] edge counter
+
+  static int32_t words(int32_t n) { return (n + 31) / 32; }
+
+  hipError_t Allocate(int32_t n, size_t gather_elem_bytes) {
+    if (capacity >= n && gather_bytes >= gather_elem_bytes * (size_t)n && frontier[0]) return hipSuccess;
+    Release();
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    const size_t nw = (size_t)words(n > 0 ? n : 1);
+    const size_t np = (nn + kCountTile - 1) / kCountTile + 1;
+    GAS_RETURN_IF(hipMalloc(&frontier[0], nn * 4));
+    GAS_RETURN_IF(hipMalloc(&frontier[1], nn * 4));
+    GAS_RETURN_IF(hipMalloc(&dedup[0], nw * 4));
+    GAS_RETURN_IF(hipMalloc(&dedup[1], nw * 4));
+    GAS_RETURN_IF(hipMalloc(&changed, nn));
+    GAS_RETURN_IF(hipMalloc(&gather_tmp, nn * (gather_elem_bytes > 4 ? gather_elem_bytes : 4)));
+    GAS_RETURN_IF(hipMalloc(&scan_local, nn * 4));
+    GAS_RETURN_IF(hipMalloc(&partials, np * 4));
+    GAS_RETURN_IF(hipMalloc(&info, sizeof(LbsInfo)));
+    GAS_RETURN_IF(hipMalloc(&sizes, 4 * sizeof(int32_t)));
+    GAS_RETURN_IF(hipMalloc(&counters, 8 * sizeof(unsigned long long)));
+    GAS_RETURN_IF(hipHostMalloc(&h_sizes, 8 * sizeof(int32_t)));
+    capacity = n;
+    gather_bytes = gather_elem_bytes * nn;
+    return hipSuccess;
+  }
+
+  // Misc-value queues handed to Program::Initialize (d_frontier_values[3]).
+  hipError_t AllocateValues() {
+    for (int i = 0; i < 3; i++)
+      if (!frontier_values[i]) GAS_RETURN_IF(hipMalloc(&frontier_values[i], (size_t)(capacity > 0 ? capacity : 1) * 8));
+    return hipSuccess;
+  }
+
+  void Release() {
+    for (int i = 0; i < 2; i++) {
+      if (frontier[i]) (void)hipFree(frontier[i]);
+      if (dedup[i]) (void)hipFree(dedup[i]);
+      frontier[i] = nullptr;
+      dedup[i] = nullptr;
+    }
+    for (int i = 0; i < 3; i++) {
+      if (frontier_values[i]) (void)hipFree(frontier_values[i]);
+      frontier_values[i] = nullptr;
+    }
+    void *ps[] = {changed, gather_tmp, scan_local, partials, info, sizes, counters};
+    for (void *p : ps)
+      if (p) (void)hipFree(p);
+    if (h_sizes) (void)hipHostFree(h_sizes);
+    changed = nullptr;
+    gather_tmp = nullptr;
+    scan_local = nullptr;
+    partials = nullptr;
+    info = nullptr;
+    sizes = nullptr;
+    counters = nullptr;
+    h_sizes = nullptr;
+    capacity = 0;
+    gather_bytes = 0;
+  }
+};
+
+// Host CSC of a CSR by a stable counting sort on the column: within each
+// column the rows stay ascending (the reference's stable sort by column of
+// the (row, col)-sorted COO, b40c/graph/csr_graph.cuh:173-192), plus the
+// CSC->CSR edge-id permutation (util.cuh:55-126 builds it with thrust).
+inline void host_csr_to_csc(int32_t n, int64_t m, const int32_t *ro, const int32_t *col, int32_t *co, int32_t *ri,
+                            int32_t *eid) {
+  std::vector<int64_t> cnt((size_t)n + 1, 0);
+  for (int64_t e = 0; e < m; e++) cnt[(size_t)col[e] + 1]++;
+  for (int32_t v = 0; v < n; v++) cnt[(size_t)v + 1] += cnt[(size_t)v];
+  for (int32_t v = 0; v <= n; v++) co[v] = (int32_t)cnt[(size_t)v];
+  for (int32_t u = 0; u < n; u++)
+    for (int32_t e = ro[u]; e < ro[u + 1]; e++) {
+      int64_t pos = cnt[(size_t)col[e]]++;
+      ri[pos] = u;
+      if (eid) eid[pos] = e;
+    }
+}
+
+template <class Program, typename VertexId = int, typename SizeT = int,
+          typename Value = typename Program::DataType, bool MARK_PREDECESSORS = false, bool WITH_VALUE = true>
+struct CsrProblem {
+  typedef Program ProgramType;
+  typedef typename Program::DataType DataType;
+
+  DeviceGraph graph;
+  Workspace ws;
+  typename Program::VertexType vertex_list;
+  typename Program::EdgeType edge_list;
+  hipStream_t stream = nullptr;
+  std::vector<void *> owned;  // device copies made by FromHostProblem
+
+  CsrProblem() = default;
+  CsrProblem(const CsrProblem &) = delete;
+  CsrProblem &operator=(const CsrProblem &) = delete;
+  ~CsrProblem() {
+    ws.Release();
+    for (void *p : owned) (void)hipFree(p);
+  }
+
+  // Share a graph already resident on the device (no copy).
+  hipError_t Attach(const DeviceGraph &g, hipStream_t s) {
+    graph = g;
+    stream = s;
+    return ws.Allocate(g.nodes, sizeof(typename Program::GatherType));
+  }
+
+  // Reference form (csr_problem.cuh:512-524, upstream 10-argument overload):
+  // copy a host CSR (and, for directed graphs, its CSC when given; built on
+  // the host otherwise) to the current device.
+  hipError_t FromHostProblem(bool /*stream_from_host*/, SizeT nodes, SizeT edges, VertexId *h_column_indices,
+                             SizeT *h_row_offsets, Value *h_edge_values, VertexId *h_row_indices,
+                             SizeT *h_column_offsets, int /*num_gpus*/, int directed) {
+    DeviceGraph g;
+    g.nodes = nodes;
+    g.edges = edges;
+    g.directed = directed;
+    auto up = [&](const void *src, size_t bytes, void **dst) -> hipError_t {
+      GAS_RETURN_IF(hipMalloc(dst, bytes > 0 ? bytes : 4));
+      owned.push_back(*dst);
+      if (bytes) GAS_RETURN_IF(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+      return hipSuccess;
+    };
+    void *p = nullptr;
+    GAS_RETURN_IF(up(h_row_offsets, sizeof(SizeT) * ((size_t)nodes + 1), &p));
+    g.row_offsets = (const int32_t *)p;
+    GAS_RETURN_IF(up(h_column_indices, sizeof(VertexId) * (size_t)edges, &p));
+    g.column_indices = (const int32_t *)p;
+    if (WITH_VALUE && h_edge_values) {
+      GAS_RETURN_IF(up(h_edge_values, sizeof(Value) * (size_t)edges, &p));
+      g.edge_values = p;
+    }
+    if (directed) {
+      std::vector<int32_t> co((size_t)nodes + 1), ri((size_t)edges), eid((size_t)edges);
+      host_csr_to_csc(nodes, edges, h_row_offsets, h_column_indices, co.data(), ri.data(), eid.data());
+      (void)h_row_indices;
+      (void)h_column_offsets;
+      GAS_RETURN_IF(up(co.data(), 4 * co.size(), &p));
+      g.column_offsets = (const int32_t *)p;
+      GAS_RETURN_IF(up(ri.data(), 4 * ri.size(), &p));
+      g.row_indices = (const int32_t *)p;
+      GAS_RETURN_IF(up(eid.data(), 4 * eid.size(), &p));
+      g.csc_edge_ids = (const int32_t *)p;
+    } else {
+      g.column_offsets = g.row_offsets;
+      g.row_indices = g.column_indices;
+    }
+    return Attach(g, stream);
+  }
+
+  // Clear per-run engine state (csr_problem.cuh:845-1045).  The frontier
+  // contents come from Program::Initialize, called by the enactor.
+  hipError_t Reset(FrontierType /*frontier_type*/ = VERTEX_FRONTIERS, double /*queue_sizing*/ = 1.0) {
+    const int32_t n = graph.nodes;
+    using G = typename Program::GatherType;
+    GAS_RETURN_IF(fill_async<G>((G *)ws.gather_tmp, (G)Program::INIT_VALUE, n, stream));
+    GAS_RETURN_IF(fill_async<char>(ws.changed, (char)0, n, stream));
+    GAS_RETURN_IF(fill_async<uint32_t>(ws.dedup[0], 0u, Workspace::words(n), stream));
+    GAS_RETURN_IF(fill_async<uint32_t>(ws.dedup[1], 0u, Workspace::words(n), stream));
+    GAS_RETURN_IF(fill_async<unsigned long long>(ws.counters, 0ull, 8, stream));
+    return hipSuccess;
+  }
+
+  // Copy the program's result to host memory (csr_problem.cuh:433-458).
+  hipError_t ExtractResults(DataType *h_out) {
+    GAS_RETURN_IF(hipStreamSynchronize(stream));
+    engine_stream() = stream;
+    Program::extractResult(vertex_list, h_out);
+    return hipStreamSynchronize(stream);
+  }
+};
+
+}  // namespace GASengine

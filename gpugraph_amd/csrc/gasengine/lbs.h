@@ -1,0 +1,312 @@
+// Frontier expansion kernels for gfx950, generic over an edge operation.
+//
+// Two strategies, as in the reference (enactor_vertex_centric.cuh:3637-3667
+// picks by frontier size against `threshold`):
+//
+//  * Load-balance search (LBS), the role of the reference's two-phase
+//    "mgpu" expand (mgpukernel/kernel.cuh:63-180 + moderngpu Scan /
+//    MergePathPartitions / CTALoadBalance):
+//      k_lbs_count   per frontier item: prologue + degree, workgroup-local
+//                    exclusive scan, one partial per workgroup
+//      k_lbs_scan    one workgroup scans the partials -> global offsets
+//      k_lbs_expand  the merge path of (items + edges) is cut into equal
+//                    tiles; each workgroup stages its items in LDS, builds an
+//                    edge->item map with a per-thread serial merge, then walks
+//                    its edges with consecutive lanes on consecutive edges
+//                    (coalesced column reads), whatever the degree skew.
+//  * Tiered dynamic expand, the role of expand_atomic (expand_atomic/cta.cuh:
+//    264-417, 701-754), re-tiled for wave64: degree >= block_degree -> whole
+//    workgroup per vertex, >= wave_degree -> one wave per vertex, rest -> an
+//    LDS-staged workgroup gather.  One launch, no scan: for small frontiers.
+//
+// Output: every edge for which op.edge() returns true appends its value to
+// the next queue with one atomic per wave (wave_append).
+//
+// Op concept (device, passed by value):
+//   void    prologue(int32_t v)          once per frontier item before degree
+//   int32_t degree(int32_t v)            edges to expand (0 = none)
+//   int32_t row_begin(int32_t v)         first edge slot of v
+//   bool    edge(int32_t v, int32_t slot, int32_t &out)
+//   void    finish()                     once per thread at kernel end (all
+//                                        threads; per-thread accumulators)
+#pragma once
+
+#include "util.h"
+
+namespace GASengine {
+
+constexpr int kCountBlock = 256;
+constexpr int kCountIpt = 4;
+constexpr int kCountTile = kCountBlock * kCountIpt;  // items per scan partial
+constexpr int kScanBlock = 1024;
+constexpr int kExpandBlock = 256;
+
+// Device-side counters of one expansion (written by the kernels).
+struct LbsInfo {
+  int32_t edges;       // total edges of the frontier (sum of degrees)
+  int32_t pad;
+  int64_t path;        // merge path length = items + edges
+};
+
+template <class Op, bool PROLOGUE>
+__global__ void __launch_bounds__(kCountBlock)
+k_lbs_count(Op op, const int32_t *__restrict__ frontier, const int32_t *__restrict__ fsize,
+            int32_t *__restrict__ scan_local, int32_t *__restrict__ partials) {
+  __shared__ int32_t smem[kCountBlock / kWave + 1];
+  const int32_t nf = *fsize;
+  for (int64_t b = blockIdx.x; b * kCountTile < nf; b += gridDim.x) {
+    const int64_t first = b * kCountTile + (int64_t)threadIdx.x * kCountIpt;
+    int32_t deg[kCountIpt];
+    int32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kCountIpt; k++) {
+      deg[k] = 0;
+      if (first + k < nf) {
+        int32_t v = frontier[first + k];
+        if (PROLOGUE) op.prologue(v);
+        deg[k] = op.degree(v);
+      }
+      sum += deg[k];
+    }
+    int32_t total;
+    int32_t ex = block_exclusive_scan<kCountBlock>(sum, smem, &total);
+#pragma unroll
+    for (int k = 0; k < kCountIpt; k++) {
+      if (first + k < nf) scan_local[first + k] = ex;
+      ex += deg[k];
+    }
+    if (threadIdx.x == 0) partials[b] = total;
+  }
+}
+
+// One workgroup: exclusive scan of the per-tile partials in place, then the
+// totals into *info and the running edge counter.
+static __global__ void __launch_bounds__(kScanBlock)
+k_lbs_scan(const int32_t *__restrict__ fsize, int32_t *__restrict__ partials, LbsInfo *__restrict__ info,
+           unsigned long long *__restrict__ edge_counter) {
+  __shared__ int32_t smem[kScanBlock / kWave + 1];
+  const int32_t nf = *fsize;
+  const int32_t nb = (int32_t)(((int64_t)nf + kCountTile - 1) / kCountTile);
+  int32_t carry = 0;
+  for (int32_t base = 0; base < nb; base += kScanBlock) {
+    int32_t i = base + threadIdx.x;
+    int32_t v = i < nb ? partials[i] : 0;
+    int32_t tot;
+    int32_t ex = block_exclusive_scan<kScanBlock>(v, smem, &tot);
+    if (i < nb) partials[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    info->edges = carry;
+    info->path = (int64_t)nf + carry;
+    if (edge_counter) atomicAdd(edge_counter, (unsigned long long)carry);
+  }
+}
+
+__device__ __forceinline__ int32_t lbs_offset(const int32_t *scan_local, const int32_t *partials, int32_t i) {
+  return scan_local[i] + partials[i / kCountTile];
+}
+
+// Items consumed at merge-path diagonal d (items precede edges on ties).
+__device__ __forceinline__ int32_t lbs_path_search(const int32_t *scan_local, const int32_t *partials, int64_t d,
+                                                   int32_t nf, int32_t ne) {
+  int64_t lo = d - ne > 0 ? d - ne : 0;
+  int64_t hi = d < nf ? d : nf;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)lbs_offset(scan_local, partials, (int32_t)mid) <= d - mid - 1)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return (int32_t)lo;
+}
+
+template <class Op, int VT>
+__global__ void __launch_bounds__(kExpandBlock)
+k_lbs_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__restrict__ fsize,
+             const int32_t *__restrict__ scan_local, const int32_t *__restrict__ partials,
+             const LbsInfo *__restrict__ info, int32_t *__restrict__ out_queue, int32_t *__restrict__ out_size) {
+  constexpr int TILE = kExpandBlock * VT;
+  // slot k <-> frontier item i0 - 1 + k (slot 0: the item owning the tile's first edges)
+  __shared__ int32_t s_offs[TILE + 1];
+  __shared__ int32_t s_vert[TILE + 1];
+  __shared__ int32_t s_begin[TILE + 1];
+  __shared__ int32_t s_slot[TILE];
+  __shared__ int32_t s_bounds[2];
+  const int32_t nf = *fsize;
+  const int32_t ne_total = info->edges;
+  const int64_t path = info->path;
+  for (int64_t t = blockIdx.x; t * TILE < path; t += gridDim.x) {
+    const int64_t d0 = t * TILE;
+    const int64_t d1 = d0 + TILE < path ? d0 + TILE : path;
+    if (threadIdx.x < 2)
+      s_bounds[threadIdx.x] = lbs_path_search(scan_local, partials, threadIdx.x ? d1 : d0, nf, ne_total);
+    __syncthreads();
+    const int32_t i0 = s_bounds[0], i1 = s_bounds[1];
+    const int32_t j0 = (int32_t)(d0 - i0), j1 = (int32_t)(d1 - i1);
+    const int32_t ni = i1 - i0, ne = j1 - j0;
+    for (int k = threadIdx.x; k <= ni; k += kExpandBlock) {
+      const int32_t i = i0 - 1 + k;
+      if (i >= 0) {
+        const int32_t v = frontier[i];
+        s_vert[k] = v;
+        s_offs[k] = lbs_offset(scan_local, partials, i);
+        s_begin[k] = op.row_begin(v);
+      } else {
+        s_vert[k] = -1;
+        s_offs[k] = 0;
+        s_begin[k] = 0;
+      }
+    }
+    __syncthreads();
+    {
+      const int32_t dl = (int32_t)threadIdx.x * VT;
+      if (dl < ni + ne) {
+        int32_t lo = dl - ne > 0 ? dl - ne : 0, hi = dl < ni ? dl : ni;
+        while (lo < hi) {
+          int32_t mid = (lo + hi) >> 1;
+          if (s_offs[mid + 1] <= j0 + (dl - mid) - 1)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        int32_t li = lo, lj = dl - lo;
+#pragma unroll
+        for (int k = 0; k < VT; k++) {
+          if (li + lj >= ni + ne) break;
+          if (li < ni && s_offs[li + 1] <= j0 + lj) {
+            li++;
+          } else {
+            s_slot[lj] = li;
+            lj++;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int32_t base = 0; base < ne; base += kExpandBlock) {
+      const int32_t k = base + threadIdx.x;
+      bool keep = false;
+      int32_t out = 0;
+      if (k < ne) {
+        const int32_t s = s_slot[k];
+        keep = op.edge(s_vert[s], s_begin[s] + (j0 + k - s_offs[s]), out);
+      }
+      wave_append(keep, out, out_queue, out_size);
+    }
+    __syncthreads();
+  }
+  op.finish();
+}
+
+template <class Op, bool PROLOGUE>
+__global__ void __launch_bounds__(kExpandBlock)
+k_tiered_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__restrict__ fsize,
+                int32_t *__restrict__ out_queue, int32_t *__restrict__ out_size, int32_t wave_degree,
+                int32_t block_degree, unsigned long long *__restrict__ edge_counter) {
+  constexpr int B = kExpandBlock;
+  __shared__ int32_t s_offs[B];
+  __shared__ int32_t s_vert[B];
+  __shared__ int32_t s_begin[B];
+  __shared__ int32_t s_scan[B / kWave + 1];
+  __shared__ int32_t s_owner;
+  __shared__ int32_t s_big[3];
+  __shared__ unsigned long long s_red[B / kWave];
+  const int32_t nf = *fsize;
+  unsigned long long inspected = 0;
+  for (int64_t base = (int64_t)blockIdx.x * B; base < nf; base += (int64_t)gridDim.x * B) {
+    const int64_t i = base + threadIdx.x;
+    int32_t v = -1, deg = 0, begin = 0;
+    if (i < nf) {
+      v = frontier[i];
+      if (PROLOGUE) op.prologue(v);
+      deg = op.degree(v);
+      if (deg > 0) begin = op.row_begin(v);
+    }
+    inspected += (unsigned long long)deg;
+    // Tier 1: whole workgroup per high-degree vertex.
+    for (;;) {
+      if (threadIdx.x == 0) s_owner = -1;
+      __syncthreads();
+      if (deg >= block_degree) s_owner = threadIdx.x;
+      __syncthreads();
+      const int owner = s_owner;
+      if (owner < 0) break;
+      if ((int)threadIdx.x == owner) {
+        s_big[0] = v;
+        s_big[1] = begin;
+        s_big[2] = deg;
+        deg = 0;
+      }
+      __syncthreads();
+      const int32_t bv = s_big[0], bb = s_big[1], bd = s_big[2];
+      for (int32_t k0 = 0; k0 < bd; k0 += B) {
+        const int32_t k = k0 + threadIdx.x;
+        bool keep = false;
+        int32_t out = 0;
+        if (k < bd) keep = op.edge(bv, bb + k, out);
+        wave_append(keep, out, out_queue, out_size);
+      }
+      __syncthreads();
+    }
+    // Tier 2: one wave per mid-degree vertex.
+    uint64_t mask = ballot(deg >= wave_degree);
+    while (mask) {
+      const int l = __builtin_ctzll(mask);
+      mask &= mask - 1;
+      const int32_t wv = __shfl(v, l), wb = __shfl(begin, l), wd = __shfl(deg, l);
+      for (int32_t k0 = 0; k0 < wd; k0 += kWave) {
+        const int32_t k = k0 + lane_id();
+        bool keep = false;
+        int32_t out = 0;
+        if (k < wd) keep = op.edge(wv, wb + k, out);
+        wave_append(keep, out, out_queue, out_size);
+      }
+    }
+    if (deg >= wave_degree) deg = 0;
+    // Tier 3: LDS-staged gather of the remaining low-degree edges.
+    int32_t total;
+    const int32_t ex = block_exclusive_scan<B>(deg, s_scan, &total);
+    s_offs[threadIdx.x] = ex;
+    s_vert[threadIdx.x] = v;
+    s_begin[threadIdx.x] = begin;
+    __syncthreads();
+    for (int32_t k0 = 0; k0 < total; k0 += B) {
+      const int32_t k = k0 + threadIdx.x;
+      bool keep = false;
+      int32_t out = 0;
+      if (k < total) {
+        int32_t lo = 0, hi = B;
+        while (lo < hi) {
+          int32_t mid = (lo + hi) >> 1;
+          if (s_offs[mid] <= k)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        const int32_t j = lo - 1;
+        keep = op.edge(s_vert[j], s_begin[j] + (k - s_offs[j]), out);
+      }
+      wave_append(keep, out, out_queue, out_size);
+    }
+    __syncthreads();
+  }
+  if (edge_counter) {
+    unsigned long long t = block_sum_u64<B>(inspected, s_red);
+    if (threadIdx.x == 0 && t) atomicAdd(edge_counter, t);
+  }
+  op.finish();
+}
+
+// Per-frontier-item prologue only (apply/post_apply) for the tiered path,
+// which must finish before any contract of the same iteration runs.
+template <class Op>
+__global__ void __launch_bounds__(256)
+k_frontier_prologue(Op op, const int32_t *__restrict__ frontier, const int32_t *__restrict__ fsize) {
+  const int32_t nf = *fsize;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += (int64_t)gridDim.x * blockDim.x)
+    op.prologue(frontier[i]);
+}
+
+}  // namespace GASengine

@@ -1,0 +1,223 @@
+// Adapters from a vertex Program (the reference's functor concept,
+// doc/mapgraph-overview.h:246-255; e.g. Algorithms/SSSP/sssp.h:34-393) to the
+// engine's kernels: the expand/contract edge op used by the LBS and tiered
+// expand kernels, and the pull-gather kernel.
+//
+// Functor arities: the canonical forms are those of sssp.h / cc.h /
+// PageRank.h (apply: 6 args, contract: 6 args).  The fork's BFS contract
+// (Algorithms/BFS/bfs.h:252, 8 args with rank_id and the visited bitmap) and
+// the 7-arg apply (Algorithms/Swarm_similarity/bfs.h:198, rank_id first) are
+// accepted too, detected at compile time.
+#pragma once
+
+#include <type_traits>
+
+#include "Enums.h"
+#include "util.h"
+
+namespace GASengine {
+
+// Non-owning view of a resident graph (CSR + CSC).  For undirected graphs the
+// CSC aliases the CSR and csc_edge_ids is NULL (slot k of v's in-list is then
+// the CSR entry v->u, whose value equals u->v for the symmetric builders).
+struct DeviceGraph {
+  int32_t nodes = 0;
+  int64_t edges = 0;
+  int directed = 0;
+  const int32_t *row_offsets = nullptr;
+  const int32_t *column_indices = nullptr;
+  const int32_t *column_offsets = nullptr;
+  const int32_t *row_indices = nullptr;
+  const int32_t *csc_edge_ids = nullptr;
+  void *edge_values = nullptr;
+};
+
+namespace detail {
+
+template <class P>
+using GT = typename P::GatherType;
+template <class P>
+using MT = typename P::MiscType;
+template <class P>
+using VT = typename P::VertexType;
+template <class P>
+using ET = typename P::EdgeType;
+
+// Parameter count of a functor's operator() (read from its type only, so it
+// works for __device__ members in host and device passes alike).
+template <class M>
+struct arity_of;
+template <class R, class C, class... A>
+struct arity_of<R (C::*)(A...)> {
+  static constexpr int value = sizeof...(A);
+};
+template <class R, class C, class... A>
+struct arity_of<R (C::*)(A...) const> {
+  static constexpr int value = sizeof...(A);
+};
+template <class F>
+constexpr int arity() {
+  return arity_of<decltype(&F::operator())>::value;
+}
+
+template <class P>
+__device__ __forceinline__ void call_apply(int v, int iteration, GT<P> g, VT<P> &vl, ET<P> &el, char &changed) {
+  typename P::apply f;
+  if constexpr (arity<typename P::apply>() == 6)
+    f(v, iteration, g, vl, el, changed);
+  else
+    f(0, v, iteration, g, vl, el, changed);  // 7-arg form: (rank_id, vertex, iteration, ...)
+}
+
+template <class P>
+__device__ __forceinline__ void call_contract(int iteration, int &v, VT<P> &vl, ET<P> &el, GT<P> *tmp, MT<P> &misc,
+                                              unsigned char *bitmap) {
+  typename P::contract f;
+  if constexpr (arity<typename P::contract>() == 6)
+    f(iteration, v, vl, el, tmp, misc);
+  else
+    f(0, iteration, bitmap, v, vl, el, tmp, misc);  // 8-arg form: (rank_id, iteration, bitmap, ...)
+}
+
+template <class P, class = void>
+struct has_gather_vertex : std::false_type {};
+template <class P>
+struct has_gather_vertex<P, std::void_t<typename P::gather_vertex>> : std::true_type {};
+
+}  // namespace detail
+
+// Iteration state shared by the kernels of one GAS iteration (by value).
+template <class Program>
+struct GasState {
+  DeviceGraph g;
+  typename Program::VertexType vl;
+  typename Program::EdgeType el;
+  typename Program::GatherType *gather_tmp;
+  char *changed;
+  uint32_t *dedup_next;  // bitmap deduplicating the next frontier
+  uint32_t *dedup_prev;  // bitmap that deduplicated the current frontier (cleared here)
+  int iteration;
+  int in_edges;  // 0: expand over CSR out-edges, 1: over CSC in-edges
+  int do_apply;
+  int do_post;
+};
+
+// Expand + contract fused per edge: expand_edge -> contract -> bitmap dedup
+// -> enqueue.  The reference materialises the edge frontier (keys + misc,
+// 8 B/edge) between its expand and contract kernels; here the contract of an
+// edge runs right after its expand, which keeps the reference's per-element
+// order (contract on every emitted element, then the visited-bitmap cull,
+// contract_atomic/cta.cuh:529-629).
+template <class Program>
+struct ProgramOp {
+  GasState<Program> s;
+
+  __device__ __forceinline__ void prologue(int32_t v) {
+    s.dedup_prev[v >> 5] = 0u;  // every set bit there belongs to a current-frontier vertex
+    if (s.do_apply) {
+      char ch = s.changed[v];
+      detail::call_apply<Program>(v, s.iteration, s.gather_tmp[v], s.vl, s.el, ch);
+      s.changed[v] = ch;
+    }
+    if (s.do_post) {
+      typename Program::post_apply f;
+      f(v, s.vl, s.el, s.gather_tmp);
+    }
+  }
+
+  __device__ __forceinline__ int32_t degree(int32_t v) {
+    typename Program::expand_vertex f;
+    if (!f(v, s.changed[v], s.vl, s.el)) return 0;
+    const int32_t *off = s.in_edges ? s.g.column_offsets : s.g.row_offsets;
+    return off[v + 1] - off[v];
+  }
+
+  __device__ __forceinline__ int32_t row_begin(int32_t v) {
+    return (s.in_edges ? s.g.column_offsets : s.g.row_offsets)[v];
+  }
+
+  __device__ __forceinline__ bool edge(int32_t v, int32_t slot, int32_t &out) {
+    int32_t nb, eid;
+    if (s.in_edges) {
+      nb = s.g.row_indices[slot];
+      eid = s.g.csc_edge_ids ? s.g.csc_edge_ids[slot] : slot;
+    } else {
+      nb = s.g.column_indices[slot];
+      eid = slot;
+    }
+    int frontier = -1;
+    typename Program::MiscType misc = typename Program::MiscType();
+    typename Program::expand_edge fe;
+    fe(true, s.iteration, v, nb, eid, s.vl, s.el, frontier, misc);
+    if (frontier == -1) return false;
+    detail::call_contract<Program>(s.iteration, frontier, s.vl, s.el, s.gather_tmp, misc,
+                                   reinterpret_cast<unsigned char *>(s.dedup_next));
+    if (frontier == -1) return false;
+    if (!bit_test_and_set(s.dedup_next, frontier)) return false;
+    out = frontier;
+    return true;
+  }
+
+  __device__ __forceinline__ void finish() {}
+};
+
+// Pull gather over in-edges (GATHER_IN_EDGES: CSC) or out-edges
+// (GATHER_OUT_EDGES: CSR) of each frontier vertex, reduced with gather_sum
+// starting from INIT_VALUE, stored to gather_tmp[v] (the reference
+// materialises (dst, value) pairs and runs ReduceByKey, enactor_vertex_centric
+// .cuh:2104-2424).  One wave per vertex; lanes stride the edge list.
+// ACCUM: add to gather_tmp[v] instead of overwriting (second pass of
+// GATHER_ALL_EDGES).
+template <class Program, bool ACCUM>
+__global__ void __launch_bounds__(256)
+k_gather(GasState<Program> s, const int32_t *__restrict__ frontier, const int32_t *__restrict__ fsize, int in_edges) {
+  using G = typename Program::GatherType;
+  const int32_t nf = *fsize;
+  const int64_t waves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  typename Program::gather_edge ge;
+  typename Program::gather_sum gs;
+  for (int64_t i = wid; i < nf; i += waves) {
+    const int32_t v = frontier[i];
+    const int32_t *off = in_edges ? s.g.column_offsets : s.g.row_offsets;
+    const int32_t b = off[v], e = off[v + 1];
+    G acc = (G)Program::INIT_VALUE;
+    bool any = false;
+    for (int32_t k = b + lane_id(); k < e; k += kWave) {
+      int32_t nb, eid;
+      if (in_edges) {
+        nb = s.g.row_indices[k];
+        eid = s.g.csc_edge_ids ? s.g.csc_edge_ids[k] : k;
+      } else {
+        nb = s.g.column_indices[k];
+        eid = k;
+      }
+      G val;
+      ge(v, eid, nb, s.vl, s.el, val);
+      acc = any ? gs(acc, val) : val;
+      any = true;
+    }
+    // Lanes without edges hold INIT_VALUE and are excluded from the reduction.
+#pragma unroll
+    for (int off2 = 32; off2 > 0; off2 >>= 1) {
+      G o = __shfl_xor(acc, off2);
+      bool oany = __shfl_xor((int)any, off2);
+      if (any && oany)
+        acc = gs(acc, o);
+      else if (oany)
+        acc = o;
+      any = any || oany;
+    }
+    if (lane_id() == 0) {
+      G r = any ? acc : (G)Program::INIT_VALUE;
+      if (ACCUM) r = gs(s.gather_tmp[v], r);
+      s.gather_tmp[v] = r;
+      if constexpr (detail::has_gather_vertex<Program>::value) {
+        typename Program::gather_vertex gv;
+        gv(v, r, s.vl, s.el);
+      }
+    }
+  }
+}
+
+}  // namespace GASengine

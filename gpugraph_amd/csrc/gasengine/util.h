@@ -1,0 +1,146 @@
+// Device primitives for the GAS engine on CDNA (wave64): lane masks, wave
+// aggregated queue append, workgroup scans, bitmap helpers, fill kernels.
+// Replaces the roles of b40c/util (scan, tile I/O, memset/sequence kernels)
+// and moderngpu's CTA scan for this engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace GASengine {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// Number of set bits of `mask` strictly below this lane.
+__device__ __forceinline__ int rank_below(uint64_t mask) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Append `value` to queue[*size] for every lane with keep set: one atomic per
+// wave (leader reserves popcount slots), order within the wave by lane.
+// Must be reached by all lanes of the wave.
+__device__ __forceinline__ void wave_append(bool keep, int32_t value, int32_t *queue, int32_t *size) {
+  uint64_t mask = ballot(keep);
+  if (mask == 0) return;
+  int leader = __builtin_ctzll(mask);
+  int base = 0;
+  if (lane_id() == leader) base = atomicAdd(size, (int32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  if (keep) queue[base + rank_below(mask)] = value;
+}
+
+// Wave-wide reductions (all lanes participate).
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Inclusive scan across the wave.
+__device__ __forceinline__ int32_t wave_inclusive_scan(int32_t v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    int32_t t = __shfl_up(v, off);
+    if (l >= off) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan over a workgroup of BLOCK threads.  `smem` needs
+// BLOCK/64 + 1 ints.  Returns the exclusive prefix; *total gets the sum.
+template <int BLOCK>
+__device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t *smem, int32_t *total) {
+  constexpr int W = BLOCK / kWave;
+  const int wid = threadIdx.x / kWave, l = lane_id();
+  int32_t inc = wave_inclusive_scan(v);
+  if (l == kWave - 1) smem[wid] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      int32_t t = smem[w];
+      smem[w] = run;
+      run += t;
+    }
+    smem[W] = run;
+  }
+  __syncthreads();
+  int32_t excl = smem[wid] + inc - v;
+  *total = smem[W];
+  __syncthreads();
+  return excl;
+}
+
+// Workgroup sum of 64-bit values; result valid in every thread.
+template <int BLOCK>
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v, unsigned long long *smem) {
+  constexpr int W = BLOCK / kWave;
+  v = wave_sum(v);
+  const int wid = threadIdx.x / kWave;
+  if (lane_id() == 0) smem[wid] = v;
+  __syncthreads();
+  unsigned long long t = 0;
+#pragma unroll
+  for (int w = 0; w < W; w++) t += smem[w];
+  __syncthreads();
+  return t;
+}
+
+// ---- bitmaps (bit v of word v>>5) --------------------------------------
+
+__device__ __forceinline__ bool bit_test(const uint32_t *bits, int32_t v) {
+  return (bits[v >> 5] >> (v & 31)) & 1u;
+}
+
+// Atomic test-and-set; true iff this call flipped the bit from 0 to 1.  A
+// plain read first skips the atomic for bits already set.
+__device__ __forceinline__ bool bit_test_and_set(uint32_t *bits, int32_t v) {
+  const uint32_t m = 1u << (v & 31);
+  if (__atomic_load_n(&bits[v >> 5], __ATOMIC_RELAXED) & m) return false;
+  return (atomicOr(&bits[v >> 5], m) & m) == 0;
+}
+
+// ---- fills --------------------------------------------------------------
+
+template <class T>
+__global__ void k_fill(T *p, T value, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+template <class T>
+__global__ void k_sequence(T *p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (T)i;
+}
+
+inline int grid_for(int64_t n, int block, int max_grid = 65536) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > max_grid) g = max_grid;
+  return (int)g;
+}
+
+template <class T>
+inline hipError_t fill_async(T *p, T value, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, p, value, n);
+  return hipGetLastError();
+}
+
+template <class T>
+inline hipError_t sequence_async(T *p, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sequence<T>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace GASengine

@@ -1,0 +1,363 @@
+// Library core: errors, graph handles (device CSR + CSC), options, the
+// per-kernel event profiler and result helpers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "internal.h"
+
+// ---------------------------------------------------------------- errors --
+namespace {
+thread_local char g_err[512] = "";
+}
+
+namespace gg {
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+void clear_error() { g_err[0] = '\0'; }
+
+int activate(gg_graph *g) {
+  int cur = -1;
+  GG_HIP(hipGetDevice(&cur));
+  if (cur != g->device) GG_HIP(hipSetDevice(g->device));
+  return GG_OK;
+}
+
+int copy_out(gg_graph *g, void *dst, const void *src, size_t bytes, uint32_t flags) {
+  if (!dst || bytes == 0) return GG_OK;
+  if (flags & GG_OUT_DEVICE) {
+    if (dst != src) GG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g->stream));
+    GG_HIP(hipStreamSynchronize(g->stream));
+  } else {
+    GG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g->stream));
+    GG_HIP(hipStreamSynchronize(g->stream));
+  }
+  return GG_OK;
+}
+
+__global__ void k_reach(const int32_t *__restrict__ res, int32_t sentinel, const int32_t *__restrict__ ro, int32_t n,
+                        unsigned long long *__restrict__ out) {
+  __shared__ unsigned long long red[256 / 64];
+  unsigned long long cnt = 0, deg = 0;
+  for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    if (res[v] != sentinel) {
+      cnt++;
+      deg += (unsigned long long)(ro[v + 1] - ro[v]);
+    }
+  cnt = GASengine::block_sum_u64<256>(cnt, red);
+  deg = GASengine::block_sum_u64<256>(deg, red);
+  if (threadIdx.x == 0) {
+    if (cnt) atomicAdd(&out[0], cnt);
+    if (deg) atomicAdd(&out[1], deg);
+  }
+}
+
+int reach_stats(gg_graph *g, const int32_t *result, int32_t sentinel, gg_stats *st) {
+  if (!st) return GG_OK;
+  GG_TRY(g->counters.ensure(64 * sizeof(unsigned long long)));
+  auto *c = g->counters.as<unsigned long long>() + 32;
+  GG_HIP(GASengine::fill_async<unsigned long long>(c, 0ull, 2, g->stream));
+  hipLaunchKernelGGL(k_reach, dim3(GASengine::grid_for(g->n, 256, 2048)), dim3(256), 0, g->stream, result, sentinel,
+                     g->dg.row_offsets, g->n, c);
+  GG_HIP(hipGetLastError());
+  GG_HIP(hipMemcpyAsync(g->h_pinned, c, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g->stream));
+  GG_HIP(hipStreamSynchronize(g->stream));
+  st->reached_vertices = g->h64()[0];
+  st->reached_edges = g->h64()[1];
+  return GG_OK;
+}
+
+void finish_profile(gg_graph *g) {
+  if (g->prof.enabled) g->prof.flush();
+}
+
+// ------------------------------------------------------------- profiler --
+EventProfiler::~EventProfiler() { reset(); }
+
+hipEvent_t EventProfiler::take() {
+  if (!pool_.empty()) {
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+void EventProfiler::begin(const char *) {
+  if (!enabled) return;
+  open_ = take();
+  (void)hipEventRecord(open_, stream);
+}
+
+void EventProfiler::end(const char *name) {
+  if (!enabled || !open_) return;
+  hipEvent_t b = take();
+  (void)hipEventRecord(b, stream);
+  pending_.push_back({name, open_, b});
+  open_ = nullptr;
+}
+
+void EventProfiler::flush() {
+  for (auto &p : pending_) {
+    float ms = 0.f;
+    (void)hipEventSynchronize(p.b);
+    (void)hipEventElapsedTime(&ms, p.a, p.b);
+    Entry &e = table[p.name];
+    e.launches++;
+    e.ms += ms;
+    pool_.push_back(p.a);
+    pool_.push_back(p.b);
+  }
+  pending_.clear();
+}
+
+void EventProfiler::reset() {
+  flush();
+  table.clear();
+  for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+  pool_.clear();
+}
+
+}  // namespace gg
+
+// ------------------------------------------------------------------ ABI --
+extern "C" {
+
+const char *gg_last_error(void) { return g_err; }
+
+int gg_version(void) { return 10000; }
+
+int gg_device_count(int32_t *count) {
+  gg::clear_error();
+  GG_CHECK_ARG(count, "count is NULL");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *count = c;
+  return GG_OK;
+}
+
+int gg_default_options(gg_options *o) {
+  GG_CHECK_ARG(o, "out is NULL");
+  o->threshold = -1;  // per-algorithm default: BFS 0, SSSP/CC 10000, PR 1000
+  o->wave_degree = 32;
+  o->block_degree = 1024;
+  o->iter_num = INT_MAX;
+  o->bfs_alpha = 14;
+  o->bfs_beta = 24;
+  o->lbs_tile = 2048;
+  o->sssp_delta = 0;
+  return GG_OK;
+}
+
+int gg_graph_create(int32_t n, int64_t m, const int32_t *row_offsets, const int32_t *col_indices, const void *values,
+                    int32_t value_dtype, uint32_t flags, gg_graph **out) {
+  gg::clear_error();
+  GG_CHECK_ARG(out, "out is NULL");
+  *out = nullptr;
+  GG_CHECK_ARG(n >= 1, "n must be >= 1 (got %d)", n);
+  GG_CHECK_ARG(m >= 0 && m < INT32_MAX, "m must be in [0, 2^31) (got %lld)", (long long)m);
+  GG_CHECK_ARG(row_offsets && (m == 0 || col_indices), "row_offsets/col_indices is NULL");
+  GG_CHECK_ARG(value_dtype == GG_VALUES_NONE || value_dtype == GG_VALUES_I32 || value_dtype == GG_VALUES_F32,
+               "bad value_dtype %d", value_dtype);
+  GG_CHECK_ARG(value_dtype == GG_VALUES_NONE || values, "values is NULL but value_dtype is set");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return gg::set_error(GG_ERR_NO_DEVICE, "no HIP device");
+
+  const bool dev_in = flags & GG_GRAPH_DEVICE_PTRS;
+  std::vector<int32_t> h_ro, h_col;
+  const int32_t *hro = row_offsets, *hcol = col_indices;
+  if (dev_in) {
+    h_ro.resize((size_t)n + 1);
+    h_col.resize((size_t)std::max<int64_t>(m, 1));
+    GG_HIP(hipMemcpy(h_ro.data(), row_offsets, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
+    if (m) GG_HIP(hipMemcpy(h_col.data(), col_indices, 4 * (size_t)m, hipMemcpyDeviceToHost));
+    hro = h_ro.data();
+    hcol = h_col.data();
+  }
+  // Validate the CSR on the host: the kernels index with it unchecked.
+  GG_CHECK_ARG(hro[0] == 0 && hro[n] == m, "row_offsets must start at 0 and end at m");
+  for (int32_t v = 0; v < n; v++)
+    GG_CHECK_ARG(hro[v] <= hro[v + 1], "row_offsets not monotone at %d", v);
+  for (int64_t e = 0; e < m; e++)
+    GG_CHECK_ARG(hcol[e] >= 0 && hcol[e] < n, "column index %d at %lld out of range", hcol[e], (long long)e);
+
+  gg_graph *g = new (std::nothrow) gg_graph();
+  if (!g) return gg::set_error(GG_ERR_OOM, "host allocation failed");
+  std::unique_ptr<gg_graph, int (*)(gg_graph *)> guard(g, gg_graph_destroy);
+  GG_HIP(hipGetDevice(&g->device));
+  GG_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+  g->own_stream = true;
+  g->prof.stream = g->stream;
+  g->n = n;
+  g->m = m;
+  g->directed = (flags & GG_GRAPH_DIRECTED) ? 1 : 0;
+  g->value_dtype = value_dtype;
+  gg_default_options(&g->opt);
+  GG_HIP(hipHostMalloc((void **)&g->h_pinned, 64 * sizeof(int64_t)));
+
+  const hipMemcpyKind kind = dev_in ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  GG_TRY(g->ro.ensure(4 * ((size_t)n + 1)));
+  GG_TRY(g->col.ensure(4 * (size_t)std::max<int64_t>(m, 1)));
+  GG_HIP(hipMemcpy(g->ro.p, row_offsets, 4 * ((size_t)n + 1), kind));
+  if (m) GG_HIP(hipMemcpy(g->col.p, col_indices, 4 * (size_t)m, kind));
+  if (value_dtype != GG_VALUES_NONE) {
+    GG_TRY(g->val.ensure(4 * (size_t)std::max<int64_t>(m, 1)));
+    if (m) GG_HIP(hipMemcpy(g->val.p, values, 4 * (size_t)m, kind));
+  }
+  GASengine::DeviceGraph &d = g->dg;
+  d.nodes = n;
+  d.edges = m;
+  d.directed = g->directed;
+  d.row_offsets = g->ro.as<int32_t>();
+  d.column_indices = g->col.as<int32_t>();
+  d.edge_values = g->val.p;
+  if (g->directed) {
+    std::vector<int32_t> co((size_t)n + 1), ri((size_t)std::max<int64_t>(m, 1)), eid((size_t)std::max<int64_t>(m, 1));
+    GASengine::host_csr_to_csc(n, m, hro, hcol, co.data(), ri.data(), eid.data());
+    GG_TRY(g->co.ensure(4 * co.size()));
+    GG_TRY(g->ri.ensure(4 * ri.size()));
+    GG_TRY(g->eid.ensure(4 * eid.size()));
+    GG_HIP(hipMemcpy(g->co.p, co.data(), 4 * co.size(), hipMemcpyHostToDevice));
+    GG_HIP(hipMemcpy(g->ri.p, ri.data(), 4 * ri.size(), hipMemcpyHostToDevice));
+    GG_HIP(hipMemcpy(g->eid.p, eid.data(), 4 * eid.size(), hipMemcpyHostToDevice));
+    d.column_offsets = g->co.as<int32_t>();
+    d.row_indices = g->ri.as<int32_t>();
+    d.csc_edge_ids = g->eid.as<int32_t>();
+  } else {
+    d.column_offsets = d.row_offsets;
+    d.row_indices = d.column_indices;
+    d.csc_edge_ids = nullptr;
+  }
+  *out = guard.release();
+  return GG_OK;
+}
+
+int gg_graph_destroy(gg_graph *g) {
+  if (!g) return GG_OK;
+  int cur = -1;
+  const int dev = g->device;
+  (void)hipGetDevice(&cur);
+  if (cur != dev) (void)hipSetDevice(dev);
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  for (auto &p : g->prog_state) p.reset();
+  g->prof.reset();
+  if (g->h_pinned) (void)hipHostFree(g->h_pinned);
+  if (g->own_stream && g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+  if (cur >= 0 && cur != dev) (void)hipSetDevice(cur);
+  return GG_OK;
+}
+
+int gg_graph_set_stream(gg_graph *g, void *stream) {
+  gg::clear_error();
+  GG_CHECK_ARG(g, "graph is NULL");
+  GG_TRY(gg::activate(g));
+  GG_HIP(hipStreamSynchronize(g->stream));
+  if (stream) {
+    if (g->own_stream) GG_HIP(hipStreamDestroy(g->stream));
+    g->stream = (hipStream_t)stream;
+    g->own_stream = false;
+  } else if (!g->own_stream) {
+    GG_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    g->own_stream = true;
+  }
+  g->prof.stream = g->stream;
+  return GG_OK;
+}
+
+int gg_graph_set_options(gg_graph *g, const gg_options *o) {
+  gg::clear_error();
+  GG_CHECK_ARG(g && o, "NULL argument");
+  GG_CHECK_ARG(o->wave_degree >= 1 && o->block_degree >= o->wave_degree, "need 1 <= wave_degree <= block_degree");
+  GG_CHECK_ARG(o->iter_num >= 1, "iter_num must be >= 1");
+  GG_CHECK_ARG(o->bfs_alpha >= 1 && o->bfs_beta >= 1, "bfs_alpha/bfs_beta must be >= 1");
+  GG_CHECK_ARG(o->lbs_tile == 2048, "lbs_tile: only 2048 is compiled in");
+  GG_CHECK_ARG(o->sssp_delta >= 0, "sssp_delta must be >= 0");
+  g->opt = *o;
+  return GG_OK;
+}
+
+int gg_graph_get_options(const gg_graph *g, gg_options *o) {
+  GG_CHECK_ARG(g && o, "NULL argument");
+  *o = g->opt;
+  return GG_OK;
+}
+
+int gg_graph_update_values(gg_graph *g, const void *values, uint32_t flags) {
+  gg::clear_error();
+  GG_CHECK_ARG(g && values, "NULL argument");
+  GG_CHECK_ARG(g->value_dtype != GG_VALUES_NONE, "graph was created without values");
+  GG_TRY(gg::activate(g));
+  if (g->m)
+    GG_HIP(hipMemcpyAsync(g->val.p, values, 4 * (size_t)g->m,
+                          (flags & GG_OUT_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, g->stream));
+  GG_HIP(hipStreamSynchronize(g->stream));
+  g->pr_cache_valid = false;
+  return GG_OK;
+}
+
+int gg_graph_info(const gg_graph *g, int32_t *n, int64_t *m, int32_t *directed, int32_t *value_dtype) {
+  GG_CHECK_ARG(g, "graph is NULL");
+  if (n) *n = g->n;
+  if (m) *m = g->m;
+  if (directed) *directed = g->directed;
+  if (value_dtype) *value_dtype = g->value_dtype;
+  return GG_OK;
+}
+
+int gg_graph_device_csr(const gg_graph *g, const int32_t **row_offsets, const int32_t **col_indices,
+                        const void **values) {
+  GG_CHECK_ARG(g, "graph is NULL");
+  if (row_offsets) *row_offsets = g->dg.row_offsets;
+  if (col_indices) *col_indices = g->dg.column_indices;
+  if (values) *values = g->dg.edge_values;
+  return GG_OK;
+}
+
+int gg_graph_synchronize(gg_graph *g) {
+  gg::clear_error();
+  GG_CHECK_ARG(g, "graph is NULL");
+  GG_TRY(gg::activate(g));
+  GG_HIP(hipStreamSynchronize(g->stream));
+  return GG_OK;
+}
+
+int gg_profile_reset(gg_graph *g) {
+  GG_CHECK_ARG(g, "graph is NULL");
+  g->prof.reset();
+  return GG_OK;
+}
+
+int gg_profile_get(gg_graph *g, gg_kernel_stat *out, int32_t max_entries, int32_t *count) {
+  GG_CHECK_ARG(g && count, "NULL argument");
+  g->prof.flush();
+  int32_t i = 0;
+  for (auto &kv : g->prof.table) {
+    if (out && i < max_entries) {
+      std::memset(&out[i], 0, sizeof(gg_kernel_stat));
+      std::strncpy(out[i].name, kv.first.c_str(), sizeof(out[i].name) - 1);
+      out[i].launches = kv.second.launches;
+      out[i].total_ms = kv.second.ms;
+      out[i].alg_bytes = kv.second.bytes;
+    }
+    i++;
+  }
+  *count = i;
+  return GG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,146 @@
+// Internal definition of the gg_graph handle and helpers shared by the
+// library's HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "gasengine/enactor_vertex_centric.h"
+
+namespace gg {
+
+#define GG_HIP(expr)                                                                              \
+  do {                                                                                            \
+    hipError_t _e = (expr);                                                                       \
+    if (_e != hipSuccess)                                                                         \
+      return ::gg::set_error(_e == hipErrorOutOfMemory ? GG_ERR_OOM : GG_ERR_HIP, "%s failed: %s (%s:%d)", \
+                             #expr, hipGetErrorString(_e), __FILE__, __LINE__);                   \
+  } while (0)
+
+#define GG_TRY(expr)               \
+  do {                             \
+    int _rc = (expr);              \
+    if (_rc != GG_OK) return _rc;  \
+  } while (0)
+
+// Device buffer owned by a graph handle.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // Grow to at least `want` bytes (contents not preserved).
+  int ensure(size_t want) {
+    if (want == 0) want = 4;
+    if (bytes >= want) return GG_OK;
+    reset();
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return set_error(GG_ERR_OOM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    bytes = want;
+    return GG_OK;
+  }
+  template <class T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+// HIP-event timer per kernel class (GG_INSTRUMENT).
+class EventProfiler : public GASengine::KernelTimer {
+ public:
+  struct Entry {
+    int64_t launches = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+  };
+  hipStream_t stream = nullptr;
+  bool enabled = false;
+  std::map<std::string, Entry> table;
+
+  ~EventProfiler() override;
+  void begin(const char *name) override;
+  void end(const char *name) override;
+  void add_bytes(const char *name, double bytes) { if (enabled) table[name].bytes += bytes; }
+  // Resolve recorded event pairs (stream must be synchronised).
+  void flush();
+  void reset();
+
+ private:
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> pool_;
+  hipEvent_t open_ = nullptr;
+  hipEvent_t take();
+};
+
+// Scoped helper for kernels launched outside the generic enactor.
+struct TimedRegion {
+  EventProfiler &p;
+  const char *name;
+  TimedRegion(EventProfiler &p_, const char *n) : p(p_), name(n) { p.begin(n); }
+  ~TimedRegion() { p.end(name); }
+};
+
+}  // namespace gg
+
+struct gg_graph {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int32_t n = 0;
+  int64_t m = 0;
+  int directed = 0;
+  int32_t value_dtype = GG_VALUES_NONE;
+  gg_options opt{};
+
+  // Resident topology.
+  gg::DevBuf ro, col, val, co, ri, eid;
+  GASengine::DeviceGraph dg;  // view of the buffers above
+
+  // Shared scratch for the specialised paths (BFS, CC hooking, PageRank).
+  gg::DevBuf frontier[2], bits[4], vtx[4], scan_local, partials, info, counters;
+  gg::DevBuf pr[8];    // PageRank vectors / CSC-ordered transition weights
+  gg::DevBuf batch[8]; // batched-instance state
+  int32_t *h_pinned = nullptr;  // pinned readback area (64 x int64)
+  bool pr_cache_valid = false;
+
+  // Generic-program state kept across runs (one per program type).
+  std::shared_ptr<void> prog_state[4];
+
+  gg::EventProfiler prof;
+
+  int64_t *h64() { return reinterpret_cast<int64_t *>(h_pinned); }
+};
+
+namespace gg {
+
+// Make the graph's device current; returns status.
+int activate(gg_graph *g);
+// Copy n elements of a result from device array `src` to `dst` (host or
+// device per flags).
+int copy_out(gg_graph *g, void *dst, const void *src, size_t bytes, uint32_t flags);
+// reached vertices / reached-edge (degree) sums for a result array: counts
+// entries != sentinel.  Synchronises the stream.
+int reach_stats(gg_graph *g, const int32_t *result, int32_t sentinel, gg_stats *st);
+// Finish a run: resolve profiler events.
+void finish_profile(gg_graph *g);
+
+}  // namespace gg

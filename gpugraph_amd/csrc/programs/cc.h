@@ -1,0 +1,121 @@
+// Connected components by label propagation (reference: Algorithms/CC/cc.h:
+// 32-349).  label = vertex id, every vertex starts in the frontier; a changed
+// vertex pushes its label to neighbours holding a larger one (misc = label),
+// contract folds it with atomicMin into gather_tmp, apply takes the min.  The
+// fixed point is the minimum vertex id of each component.
+#pragma once
+
+#include "../gasengine/csr_problem.h"
+
+struct cc {
+  typedef int DataType;
+  typedef DataType MiscType;
+  typedef DataType GatherType;
+  typedef int VertexId;
+  typedef int SizeT;
+
+  static const int INIT_VALUE = 100000000;
+  static const bool allow_duplicates = true;
+
+  struct VertexType {
+    int nodes;
+    int edges;
+    DataType *d_dists_out;
+    DataType *d_dists;
+    VertexType() : nodes(0), edges(0), d_dists_out(nullptr), d_dists(nullptr) {}
+  };
+
+  struct EdgeType {
+    int nodes;
+    int edges;
+    EdgeType() : nodes(0), edges(0) {}
+  };
+
+  static SrcVertex srcVertex() { return ALL; }
+  static GatherEdges gatherOverEdges() { return NO_GATHER_EDGES; }
+  static ApplyVertices applyOverEdges() { return APPLY_FRONTIER; }
+  static PostApplyVertices postApplyOverEdges() { return POST_APPLY_FRONTIER; }
+  static ExpandEdges expandOverEdges() { return EXPAND_OUT_EDGES; }
+
+  static void Initialize(const int directed, const int nodes, const int edges, int num_srcs, int *srcs,
+                         int *d_row_offsets, int *d_column_indices, int *d_column_offsets, int *d_row_indices,
+                         DataType *d_edge_values, VertexType &vertex_list, EdgeType &edge_list,
+                         int *d_frontier_keys[3], MiscType *d_frontier_values[3]) {
+    (void)directed; (void)num_srcs; (void)srcs; (void)d_row_offsets; (void)d_column_indices;
+    (void)d_column_offsets; (void)d_row_indices; (void)d_edge_values; (void)edge_list;
+    hipStream_t s = GASengine::engine_stream();
+    vertex_list.nodes = nodes;
+    vertex_list.edges = edges;
+    const size_t nn = (size_t)(nodes > 0 ? nodes : 1);
+    if (!vertex_list.d_dists) (void)hipMalloc(&vertex_list.d_dists, sizeof(DataType) * nn);
+    if (!vertex_list.d_dists_out) (void)hipMalloc(&vertex_list.d_dists_out, sizeof(DataType) * nn);
+    (void)GASengine::sequence_async<int>(vertex_list.d_dists, nodes, s);
+    (void)GASengine::sequence_async<int>(vertex_list.d_dists_out, nodes, s);
+    (void)GASengine::sequence_async<int>(d_frontier_keys[0], nodes, s);
+    (void)GASengine::fill_async<int>(d_frontier_values[0], INIT_VALUE, nodes, s);
+  }
+
+  static void Release(VertexType &vertex_list, EdgeType &) {
+    if (vertex_list.d_dists) (void)hipFree(vertex_list.d_dists);
+    if (vertex_list.d_dists_out) (void)hipFree(vertex_list.d_dists_out);
+    vertex_list.d_dists = vertex_list.d_dists_out = nullptr;
+  }
+
+  struct gather_vertex {
+    __device__ void operator()(const int, const GatherType, VertexType &, EdgeType &) {}
+  };
+
+  struct gather_edge {
+    __device__ void operator()(const int, const int, const int, VertexType &, EdgeType &, GatherType &) {}
+  };
+
+  struct gather_sum {
+    __device__ GatherType operator()(const GatherType &, const GatherType &) { return 0; }
+  };
+
+  struct apply {
+    __device__ void operator()(const int vertex_id, const int iteration, GatherType gathervalue,
+                               VertexType &vertex_list, EdgeType &, char &changed) {
+      const int oldvalue = vertex_list.d_dists[vertex_id];
+      const int newvalue = min(oldvalue, gathervalue);
+      changed = (iteration == 0 || oldvalue != newvalue) ? 1 : 0;
+      vertex_list.d_dists_out[vertex_id] = newvalue;
+    }
+  };
+
+  struct post_apply {
+    __device__ void operator()(const int vertex_id, VertexType &vertex_list, EdgeType &, GatherType *gather_tmp) {
+      vertex_list.d_dists[vertex_id] = vertex_list.d_dists_out[vertex_id];
+      gather_tmp[vertex_id] = INIT_VALUE;
+    }
+  };
+
+  struct expand_vertex {
+    __device__ bool operator()(const int, const char changed, VertexType &, EdgeType &) { return changed == 1; }
+  };
+
+  struct expand_edge {
+    __device__ void operator()(const bool changed, const int iteration, const int vertex_id,
+                               const int neighbor_id_in, const int, VertexType &vertex_list, EdgeType &,
+                               int &frontier, int &misc_value) {
+      const int src_dist = vertex_list.d_dists[vertex_id];
+      const int dst_dist = vertex_list.d_dists[neighbor_id_in];
+      frontier = ((changed || iteration == 0) && dst_dist > src_dist) ? neighbor_id_in : -1;
+      misc_value = src_dist;
+    }
+  };
+
+  struct contract {
+    __device__ void operator()(const int, int &vertex_id, VertexType &, EdgeType &, GatherType *gather_tmp,
+                               int &misc_value) {
+      atomicMin(&gather_tmp[vertex_id], misc_value);
+    }
+  };
+
+  static void extractResult(VertexType &vertex_list, DataType *h_output) {
+    (void)hipMemcpy(h_output, vertex_list.d_dists, sizeof(DataType) * (size_t)vertex_list.nodes,
+                    hipMemcpyDeviceToHost);
+  }
+
+  static DataType *resultPointer(VertexType &vertex_list) { return vertex_list.d_dists; }
+};

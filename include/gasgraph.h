@@ -1,0 +1,242 @@
+/*
+ * gasgraph.h -- C ABI of libgasgraph.so, the MI355X (gfx950) Gather-Apply-
+ * Scatter frontier engine.
+ *
+ * The reference (MapGraph, a header-only CUDA C++ engine) has no FFI: callers
+ * instantiate EnactorVertexCentric<CsrProblem, Program> in their own
+ * translation unit.  Each entry point below replaces one step of that
+ * sequence for the four shipped vertex programs; the citation on each names
+ * the reference interface it stands in for.  The same engine is available as
+ * header-only HIP templates (gpugraph_amd/csrc/gasengine/) for user programs.
+ *
+ * Conventions
+ *  - Every call returns int status: GG_OK (0) or a GG_ERR_* code; the message
+ *    of the last failure on the calling thread is gg_last_error().  The
+ *    library never exits or aborts (the reference calls exit()/abort(),
+ *    GASengine/enactor_vertex_centric.cuh:107-116,622-623,768-772).
+ *  - Vertex ids, row offsets and column indices are int32 (m < 2^31).
+ *  - Output pointers are host memory unless GG_OUT_DEVICE is set, in which
+ *    case they are device pointers on the graph's device and no copy is made.
+ *  - All work of one graph handle is issued on its stream (gg_graph_set_stream);
+ *    calls on distinct handles are independent.  One handle must not be used
+ *    from two threads at once.
+ */
+#ifndef GASGRAPH_H
+#define GASGRAPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define GG_OK 0
+#define GG_ERR_INVALID 1        /* bad argument (null pointer, id out of range, ...) */
+#define GG_ERR_OOM 2            /* device or host allocation failed */
+#define GG_ERR_OVERFLOW 3       /* a queue or capacity limit was exceeded */
+#define GG_ERR_HIP 4            /* a HIP runtime call failed */
+#define GG_ERR_NOT_CONVERGED 5  /* iterative solver hit max_iter above tol */
+#define GG_ERR_NO_DEVICE 6      /* no HIP device visible */
+#define GG_ERR_IO 7             /* file open / parse failure */
+#define GG_ERR_UNSUPPORTED 8    /* option not supported for this graph */
+
+/* ---- edge value dtypes ------------------------------------------------- */
+#define GG_VALUES_NONE 0
+#define GG_VALUES_I32 1
+#define GG_VALUES_F32 2
+
+/* ---- graph flags ------------------------------------------------------- */
+#define GG_GRAPH_DIRECTED 1u    /* build the CSC (in-edges) on the device;
+                                   otherwise the CSR must be symmetric and is
+                                   used as its own CSC */
+#define GG_GRAPH_DEVICE_PTRS 2u /* inputs are device pointers on the current device */
+
+/* ---- run flags --------------------------------------------------------- */
+#define GG_OUT_DEVICE 1u        /* output pointers are device memory */
+#define GG_INSTRUMENT 2u        /* record per-kernel HIP-event timings (gg_profile_*) */
+#define GG_BFS_PUSH_ONLY 4u     /* BFS through the generic GAS program path (no bottom-up) */
+#define GG_CC_HOOK 8u           /* CC by min-id hooking + pointer jumping instead of label propagation */
+#define GG_SSSP_GAS 16u         /* SSSP through the generic GAS program path (reference semantics) */
+
+typedef struct gg_graph gg_graph;
+
+/* Engine tuning knobs.  The reference hard-codes the degree tiers (32/512,
+ * enactor_vertex_centric.cuh:3949-3950) and takes the LBS/dynamic switch from
+ * a per-algorithm "threshold" parameter (Algorithms/SSSP/register.h:39). */
+typedef struct gg_options {
+  int32_t threshold;     /* |frontier| above which the scan-based LBS expand is used */
+  int32_t wave_degree;   /* degree >= this: one wave64 per vertex (tiered expand) */
+  int32_t block_degree;  /* degree >= this: whole workgroup per vertex (tiered expand) */
+  int32_t iter_num;      /* maximum GAS iterations (reference "iter_num") */
+  int32_t bfs_alpha;     /* top-down -> bottom-up when m_f > m_u / alpha */
+  int32_t bfs_beta;      /* bottom-up -> top-down when n_f < n / beta */
+  int32_t lbs_tile;      /* merge-path positions per LBS workgroup tile (multiple of 256) */
+  int32_t sssp_delta;    /* near/far split width for SSSP (0 = plain frontier Bellman-Ford) */
+} gg_options;
+
+/* Per-run statistics (the reference prints these with INSTRUMENT/verbose). */
+typedef struct gg_stats {
+  int32_t iterations;       /* GAS iterations (BFS levels) */
+  int32_t pull_iterations;  /* of which bottom-up / pull steps */
+  int64_t reached_vertices; /* vertices with a final value != sentinel */
+  int64_t reached_edges;    /* sum of CSR out-degree over reached vertices (GTEPS numerator) */
+  int64_t edges_inspected;  /* edges actually examined, summed over iterations */
+  double alg_bytes;         /* algorithmic HBM bytes of the run (byte model of the kernel set) */
+  double gpu_ms;            /* HIP-event time of the device section */
+  double residual;          /* PageRank: last L1 residual (FP64); else 0 */
+} gg_stats;
+
+/* ---- library ------------------------------------------------------------ */
+const char *gg_last_error(void);
+int gg_version(void);                     /* major*10000 + minor*100 + patch */
+int gg_device_count(int32_t *count);
+int gg_default_options(gg_options *out);
+
+/* ---- graph -------------------------------------------------------------- */
+/* Replaces CsrProblem::FromHostProblem (GASengine/csr_problem.cuh:512-839):
+ * copies the CSR to the current device and, for GG_GRAPH_DIRECTED, builds the
+ * CSC and the CSC->CSR edge-id permutation (reference: thrust stable sort in
+ * util.cuh:55-126, called csr_problem.cuh:754-765). values may be NULL. */
+int gg_graph_create(int32_t n, int64_t m, const int32_t *row_offsets,
+                    const int32_t *col_indices, const void *values,
+                    int32_t value_dtype, uint32_t flags, gg_graph **out);
+/* Replaces GraphSlice::~GraphSlice (csr_problem.cuh:234-353). */
+int gg_graph_destroy(gg_graph *g);
+/* stream: a hipStream_t on the graph's device, or NULL for the library's own. */
+int gg_graph_set_stream(gg_graph *g, void *stream);
+int gg_graph_set_options(gg_graph *g, const gg_options *opt);
+int gg_graph_get_options(const gg_graph *g, gg_options *opt);
+/* Replace the edge values (same dtype, m entries) without rebuilding the
+ * topology; host pointer unless GG_OUT_DEVICE is set in flags. */
+int gg_graph_update_values(gg_graph *g, const void *values, uint32_t flags);
+int gg_graph_info(const gg_graph *g, int32_t *n, int64_t *m, int32_t *directed,
+                  int32_t *value_dtype);
+/* Device pointers of the resident CSR (for zero-copy callers). */
+int gg_graph_device_csr(const gg_graph *g, const int32_t **row_offsets,
+                        const int32_t **col_indices, const void **values);
+/* Wait for all work issued on the graph's stream. */
+int gg_graph_synchronize(gg_graph *g);
+
+/* ---- vertex programs ---------------------------------------------------- */
+/* BFS depths (src = 0, unreached = -1).  Replaces EnactIterativeSearch with
+ * Program = bfs (Algorithms/BFS/bfs.h:28-298; driver bfs.cu:873-881) plus
+ * ExtractResults (csr_problem.cuh:433-458).  Default: direction-optimising
+ * (top-down LBS push / bottom-up bitmap pull); GG_BFS_PUSH_ONLY runs the
+ * generic GAS program path. */
+int gg_bfs(gg_graph *g, int32_t src, int32_t *depth, uint32_t flags,
+           gg_stats *stats);
+
+/* SSSP distances with int32 edge weights (unreached = 100000000).  Replaces
+ * EnactIterativeSearch with Program = sssp (Algorithms/SSSP/sssp.h:34-393;
+ * driver sssp.cu:389-404). */
+int gg_sssp(gg_graph *g, int32_t src, int32_t *dist, uint32_t flags,
+            gg_stats *stats);
+
+/* Connected-component labels = min vertex id of the component.  Replaces
+ * EnactIterativeSearch with Program = cc (Algorithms/CC/cc.h:32-349; driver
+ * cc.cu:291-292).  Default: GAS label propagation; GG_CC_HOOK: hooking. */
+int gg_cc(gg_graph *g, int32_t *label, uint32_t flags, gg_stats *stats);
+
+/* The reference's PageRank program (Algorithms/PageRank/PageRank.h:33-409):
+ * r = 0.15 + 0.85 * sum r_u/outdeg_u over in-edges, a vertex stays active
+ * while |delta| >= 0.01; FP32 through the generic GAS gather/apply path. */
+int gg_pagerank_reference(gg_graph *g, float *rank, uint32_t flags,
+                          gg_stats *stats);
+
+/* PageRank r = d (P^T r + dangling/n) + (1-d) t, P_uv = w_uv / sum_v' w_uv'
+ * (F32 values) or 1/outdeg(u) (no F32 values).  FP32 gather with Kahan
+ * compensation, FP64 dangling sum and L1 residual; stops when the L1
+ * residual <= tol_l1 (GG_ERR_NOT_CONVERGED after max_iter sweeps).
+ * teleport: n floats or NULL for uniform.  No reference counterpart beyond
+ * the pagerank program's gather/apply (PageRank.h:197-281). */
+int gg_pagerank(gg_graph *g, float damping, const float *teleport,
+                double tol_l1, int32_t max_iter, float *rank, uint32_t flags,
+                gg_stats *stats);
+
+/* Adjoint of gg_pagerank for L = sum_v c_v r_v, given the stored forward
+ * ranks: grad_w (m, CSR order) = dL/dw for the raw F32 values (or dL/dP_uv
+ * when the graph has no F32 values); grad_t (n) = dL/dt.  Either output may
+ * be NULL.  No reference counterpart. */
+int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank,
+                        const float *c, double tol_l1, int32_t max_iter,
+                        float *grad_w, float *grad_t, uint32_t flags,
+                        gg_stats *stats);
+
+/* ---- batched instances -------------------------------------------------- */
+/* num_sssp SSSP sources and num_ppr personalized-PageRank seeds (one-hot
+ * teleports) solved together: per-instance frontiers in shared kernels,
+ * iteration loop captured as a HIP graph.  Outputs are instance-major:
+ * sssp_dist[k*n + v], ppr_rank[k*n + v].  Replaces the drivers' sequential
+ * per-source loop (sssp.cu:389-404, bfs.cu:865-881). */
+typedef struct gg_batch_spec {
+  int32_t num_sssp;
+  const int32_t *sssp_sources;
+  int32_t *sssp_dist;
+  int32_t num_ppr;
+  const int32_t *ppr_seeds;
+  float *ppr_rank;
+  float damping;
+  double tol_l1;
+  int32_t max_iter;
+} gg_batch_spec;
+
+int gg_batch_run(gg_graph *g, const gg_batch_spec *spec, uint32_t flags,
+                 gg_stats *stats);
+
+/* ---- instrumentation ---------------------------------------------------- */
+/* Per-kernel-class event timings collected by runs with GG_INSTRUMENT
+ * (the reference's INSTRUMENT template flag, enactor_vertex_centric.cuh:96). */
+typedef struct gg_kernel_stat {
+  char name[48];
+  int64_t launches;
+  double total_ms;
+  double alg_bytes;  /* algorithmic bytes attributed to this kernel class */
+} gg_kernel_stat;
+
+int gg_profile_reset(gg_graph *g);
+int gg_profile_get(gg_graph *g, gg_kernel_stat *out, int32_t max_entries,
+                   int32_t *count);
+
+/* ---- graph construction (host) ----------------------------------------- */
+/* Host CSR produced by the builders below; free with gg_csr_free. */
+typedef struct gg_csr {
+  int32_t n;
+  int64_t m;
+  int32_t *row_offsets;  /* n+1 */
+  int32_t *col_indices;  /* m */
+  void *values;          /* m entries of value_dtype, or NULL */
+  int32_t value_dtype;
+} gg_csr;
+
+void gg_csr_free(gg_csr *csr);
+
+/* Matrix Market reader (b40c/graph/builder/market.cuh:42-209 +
+ * csr_graph.cuh FromCoo:121-150): 1-based ids, optional mirror when
+ * undirected, rows sorted by (row, col) with a stable sort, duplicates and
+ * self-loops kept.  "pattern" files get value 1. */
+int gg_read_mtx(const char *path, int32_t undirected, int32_t value_dtype,
+                gg_csr *out);
+
+/* Seeded generators.  Same (parameters, seed) -> same CSR on every host. */
+/* G(n, M): M distinct undirected pairs u != v, symmetric CSR. */
+int gg_gen_erdos_renyi(int32_t n, int64_t num_pairs, uint64_t seed, gg_csr *out);
+/* R-MAT(a,b,c) with 2^scale vertices and edge_factor*2^scale draws; vertex
+ * ids scrambled by a seeded bijection; self-loops and duplicates removed;
+ * symmetric when undirected. */
+int gg_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
+                double c, int32_t undirected, uint64_t seed, gg_csr *out);
+/* side x side 4-neighbour lattice, each bond kept with probability keep_p,
+ * plus floor(long_range_frac * kept) random long-range pairs; symmetric. */
+int gg_gen_lattice(int32_t side, double keep_p, double long_range_frac,
+                   uint64_t seed, gg_csr *out);
+/* Attach values to a CSR: int32 uniform in [lo, hi] (symmetric in (u,v) when
+ * symmetric != 0), or F32 (k+1)/2^24 with k 24 random bits (lo/hi ignored). */
+int gg_csr_weights(gg_csr *csr, int32_t value_dtype, int32_t lo, int32_t hi,
+                   int32_t symmetric, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GASGRAPH_H */

@@ -1,0 +1,119 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so, built by oracle/Makefile).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.  The
+product package (gpugraph_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_I = C.POINTER(C.c_int32)
+_F = C.POINTER(C.c_float)
+_D = C.POINTER(C.c_double)
+
+
+def build():
+    subprocess.run(["make", "-B", "-C", _HERE], check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.oracle_bfs.argtypes = [C.c_int32, _I, _I, C.c_int32, _I]
+        L.oracle_sssp_selection.argtypes = [C.c_int32, _I, _I, _I, C.c_int32, _I]
+        L.oracle_sssp_dijkstra.argtypes = [C.c_int32, _I, _I, _I, C.c_int32, _I]
+        L.oracle_cc.argtypes = [C.c_int32, _I, _I, _I]
+        L.oracle_cc.restype = C.c_int32
+        L.oracle_pr_reference.argtypes = [C.c_int32, _I, _I, _I, _F]
+        L.oracle_pr_reference.restype = C.c_int32
+        L.oracle_pagerank_f64.argtypes = [C.c_int32, _I, _I, _D, _D, C.c_double, C.c_double, C.c_int32, _D, _D]
+        L.oracle_pagerank_f64.restype = C.c_int32
+        L.oracle_pagerank_adjoint_f64.argtypes = [C.c_int32, _I, _I, _D, C.c_double, _D, _D, C.c_double, C.c_int32,
+                                                  _D, _D, _D]
+        L.oracle_pagerank_adjoint_f64.restype = C.c_int32
+        _lib = L
+    return _lib
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32).ctypes.data_as(_I)
+
+
+def _d(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(_D)
+
+
+def bfs(ro, col, src):
+    ro, col = np.ascontiguousarray(ro, np.int32), np.ascontiguousarray(col, np.int32)
+    out = np.empty(len(ro) - 1, np.int32)
+    lib().oracle_bfs(len(ro) - 1, _i(ro), _i(col), int(src), out.ctypes.data_as(_I))
+    return out
+
+
+def sssp(ro, col, w, src, selection=False):
+    ro, col, w = (np.ascontiguousarray(x, np.int32) for x in (ro, col, w))
+    out = np.empty(len(ro) - 1, np.int32)
+    fn = lib().oracle_sssp_selection if selection else lib().oracle_sssp_dijkstra
+    fn(len(ro) - 1, _i(ro), _i(col), _i(w), int(src), out.ctypes.data_as(_I))
+    return out
+
+
+def cc(ro, col):
+    ro, col = np.ascontiguousarray(ro, np.int32), np.ascontiguousarray(col, np.int32)
+    out = np.empty(len(ro) - 1, np.int32)
+    sweeps = lib().oracle_cc(len(ro) - 1, _i(ro), _i(col), out.ctypes.data_as(_I))
+    return out, sweeps
+
+
+def csc(ro, col):
+    """Stable CSC (rows ascending within a column) of a CSR, in numpy."""
+    n = len(ro) - 1
+    rows = np.repeat(np.arange(n, dtype=np.int32), np.diff(ro))
+    order = np.argsort(col, kind="stable")
+    co = np.zeros(n + 1, np.int64)
+    np.add.at(co, np.asarray(col, np.int64) + 1, 1)
+    return np.cumsum(co).astype(np.int32), rows[order].astype(np.int32), order.astype(np.int32)
+
+
+def pr_reference(ro, col):
+    co, ri, _ = csc(ro, col)
+    out = np.empty(len(ro) - 1, np.float32)
+    iters = lib().oracle_pr_reference(len(ro) - 1, _i(ro), _i(co), _i(ri), out.ctypes.data_as(_F))
+    return out, iters
+
+
+def pagerank_f64(ro, col, p=None, t=None, d=0.85, tol=1e-12, max_iter=10000):
+    n = len(ro) - 1
+    r = np.empty(n, np.float64)
+    res = C.c_double()
+    p = None if p is None else np.ascontiguousarray(p, np.float64)
+    t = None if t is None else np.ascontiguousarray(t, np.float64)
+    it = lib().oracle_pagerank_f64(n, _i(ro), _i(col), _d(p), _d(t), d, tol, max_iter, r.ctypes.data_as(_D),
+                                   C.byref(res))
+    return r, it, res.value
+
+
+def pagerank_adjoint_f64(ro, col, w, r, c, d=0.85, tol=1e-14, max_iter=10000):
+    n = len(ro) - 1
+    m = len(col)
+    lam = np.empty(n, np.float64)
+    gw = np.zeros(m, np.float64)
+    gt = np.empty(n, np.float64)
+    w = None if w is None else np.ascontiguousarray(w, np.float64)
+    r = np.ascontiguousarray(r, np.float64)
+    c = np.ascontiguousarray(c, np.float64)
+    it = lib().oracle_pagerank_adjoint_f64(n, _i(ro), _i(col), _d(w), d, _d(r), _d(c), tol, max_iter,
+                                           lam.ctypes.data_as(_D), gw.ctypes.data_as(_D), gt.ctypes.data_as(_D))
+    return lam, gw, gt, it

@@ -1,0 +1,281 @@
+/*
+ * oracle.c -- CPU restatement of the reference's CPU checkers (see oracle.h).
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py.  Never linked into or called by the product
+ * library.  Plain C99, single-threaded, so its timings are a 1-core baseline.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SSSP_INF 100000000 /* Algorithms/SSSP/sssp.h:43 INIT_VALUE */
+
+/* Algorithms/BFS/bfs.cu:57-111 -- FIFO queue BFS, depth = parent depth + 1. */
+void oracle_bfs(int32_t n, const int32_t *ro, const int32_t *col, int32_t src,
+                int32_t *depth) {
+  for (int32_t i = 0; i < n; i++) depth[i] = -1;
+  if (src < 0 || src >= n) return;
+  int32_t *queue = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  int32_t head = 0, tail = 0;
+  depth[src] = 0;
+  queue[tail++] = src;
+  while (head < tail) {
+    int32_t u = queue[head++];
+    int32_t nd = depth[u] + 1;
+    for (int32_t e = ro[u]; e < ro[u + 1]; e++) {
+      int32_t v = col[e];
+      if (depth[v] == -1) {
+        depth[v] = nd;
+        queue[tail++] = v;
+      }
+    }
+  }
+  free(queue);
+}
+
+/* Algorithms/SSSP/sssp.cu:126-183 -- selection Dijkstra exactly as written:
+ * pick the unvisited vertex with the smallest dist < 100000000 (lowest id on
+ * ties), relax its out-edges with 64-bit sums. */
+void oracle_sssp_selection(int32_t n, const int32_t *ro, const int32_t *col,
+                           const int32_t *w, int32_t src, int32_t *dist) {
+  for (int32_t i = 0; i < n; i++) dist[i] = SSSP_INF;
+  if (src < 0 || src >= n) return;
+  unsigned char *visited = (unsigned char *)calloc((size_t)(n > 0 ? n : 1), 1);
+  dist[src] = 0;
+  for (;;) {
+    int32_t u = -1, sd = SSSP_INF;
+    for (int32_t i = 0; i < n; i++)
+      if (!visited[i] && dist[i] < sd) {
+        sd = dist[i];
+        u = i;
+      }
+    if (u == -1) break;
+    visited[u] = 1;
+    for (int32_t e = ro[u]; e < ro[u + 1]; e++) {
+      int32_t v = col[e];
+      int64_t len = (int64_t)dist[u] + (int64_t)w[e];
+      if (len < dist[v]) dist[v] = (int32_t)len;
+    }
+  }
+  free(visited);
+}
+
+/* Binary min-heap of (dist, vertex) with lazy deletion. */
+typedef struct {
+  int64_t key;
+  int32_t v;
+} heap_item;
+
+static void heap_push(heap_item *h, int64_t *size, heap_item it) {
+  int64_t i = (*size)++;
+  while (i > 0) {
+    int64_t p = (i - 1) >> 1;
+    if (h[p].key <= it.key) break;
+    h[i] = h[p];
+    i = p;
+  }
+  h[i] = it;
+}
+
+static heap_item heap_pop(heap_item *h, int64_t *size) {
+  heap_item top = h[0];
+  heap_item last = h[--(*size)];
+  int64_t i = 0;
+  for (;;) {
+    int64_t c = 2 * i + 1;
+    if (c >= *size) break;
+    if (c + 1 < *size && h[c + 1].key < h[c].key) c++;
+    if (last.key <= h[c].key) break;
+    h[i] = h[c];
+    i = c;
+  }
+  if (*size > 0) h[i] = last;
+  return top;
+}
+
+/* Same fixed point as oracle_sssp_selection (unique shortest-path distances,
+ * 100000000 for unreached, strict-< relaxation so nothing reaches the
+ * sentinel), with a heap instead of the O(n) selection scan. */
+void oracle_sssp_dijkstra(int32_t n, const int32_t *ro, const int32_t *col,
+                          const int32_t *w, int32_t src, int32_t *dist) {
+  for (int32_t i = 0; i < n; i++) dist[i] = SSSP_INF;
+  if (src < 0 || src >= n) return;
+  int64_t m = ro[n];
+  heap_item *h = (heap_item *)malloc(sizeof(heap_item) * (size_t)(m + 2));
+  unsigned char *done = (unsigned char *)calloc((size_t)n, 1);
+  int64_t size = 0;
+  dist[src] = 0;
+  heap_item s = {0, src};
+  heap_push(h, &size, s);
+  while (size > 0) {
+    heap_item it = heap_pop(h, &size);
+    int32_t u = it.v;
+    if (done[u] || it.key != dist[u]) continue;
+    done[u] = 1;
+    for (int32_t e = ro[u]; e < ro[u + 1]; e++) {
+      int32_t v = col[e];
+      int64_t len = (int64_t)dist[u] + (int64_t)w[e];
+      if (len < dist[v]) {
+        dist[v] = (int32_t)len;
+        heap_item ni = {len, v};
+        heap_push(h, &size, ni);
+      }
+    }
+  }
+  free(done);
+  free(h);
+}
+
+/* Algorithms/CC/cc.cu:127-169 -- label = id, then in-place sweeps
+ * label[v] = min(label[v], min_{nb} label[nb]) until no change. */
+int32_t oracle_cc(int32_t n, const int32_t *ro, const int32_t *col,
+                  int32_t *label) {
+  for (int32_t i = 0; i < n; i++) label[i] = i;
+  int32_t sweeps = 0;
+  int changed = 1;
+  while (changed) {
+    changed = 0;
+    sweeps++;
+    for (int32_t v = 0; v < n; v++) {
+      int32_t minnb = SSSP_INF;
+      for (int32_t e = ro[v]; e < ro[v + 1]; e++) {
+        int32_t l = label[col[e]];
+        if (l < minnb) minnb = l;
+      }
+      if (minnb < label[v]) {
+        label[v] = minnb;
+        changed = 1;
+      }
+    }
+  }
+  return sweeps;
+}
+
+/* Algorithms/PageRank/PageRank.cu:108-161 -- in-place (Gauss-Seidel) float
+ * sweeps over the CSC; 0.15 + 0.85*sum is formed in double then stored to
+ * float, as the reference's mixed float/double expression does. */
+int32_t oracle_pr_reference(int32_t n, const int32_t *ro, const int32_t *co,
+                            const int32_t *ri, float *rank) {
+  for (int32_t i = 0; i < n; i++) rank[i] = 0.15f;
+  int32_t iters = 0;
+  int changed = 1;
+  while (changed) {
+    changed = 0;
+    for (int32_t v = 0; v < n; v++) {
+      float sum = 0.0f;
+      for (int32_t e = co[v]; e < co[v + 1]; e++) {
+        int32_t nb = ri[e];
+        sum += rank[nb] / (float)(ro[nb + 1] - ro[nb]);
+      }
+      sum = (float)(0.15 + 0.85 * (double)sum);
+      if (fabsf(sum - rank[v]) >= 0.01f) changed = 1;
+      rank[v] = sum;
+    }
+    iters++;
+  }
+  return iters;
+}
+
+int32_t oracle_pagerank_f64(int32_t n, const int32_t *ro, const int32_t *col,
+                            const double *p, const double *t, double d,
+                            double tol, int32_t max_iter, double *r,
+                            double *resid) {
+  double *y = (double *)malloc(sizeof(double) * (size_t)n);
+  double inv_n = 1.0 / (double)n;
+  for (int32_t v = 0; v < n; v++) r[v] = t ? t[v] : inv_n;
+  int32_t it = 0;
+  double res = INFINITY;
+  while (it < max_iter) {
+    double dangling = 0.0;
+    memset(y, 0, sizeof(double) * (size_t)n);
+    for (int32_t u = 0; u < n; u++) {
+      int32_t deg = ro[u + 1] - ro[u];
+      if (deg == 0) {
+        dangling += r[u];
+        continue;
+      }
+      double inv_deg = 1.0 / (double)deg;
+      for (int32_t e = ro[u]; e < ro[u + 1]; e++)
+        y[col[e]] += (p ? p[e] : inv_deg) * r[u];
+    }
+    res = 0.0;
+    for (int32_t v = 0; v < n; v++) {
+      double tv = t ? t[v] : inv_n;
+      double rn = d * (y[v] + dangling * inv_n) + (1.0 - d) * tv;
+      res += fabs(rn - r[v]);
+      r[v] = rn;
+    }
+    it++;
+    if (res <= tol) break;
+  }
+  if (resid) *resid = res;
+  free(y);
+  return it;
+}
+
+int32_t oracle_pagerank_adjoint_f64(int32_t n, const int32_t *ro,
+                                    const int32_t *col, const double *w,
+                                    double d, const double *r, const double *c,
+                                    double tol, int32_t max_iter,
+                                    double *lambda, double *grad_w,
+                                    double *grad_t) {
+  double *ln = (double *)malloc(sizeof(double) * (size_t)n);
+  double *S = (double *)malloc(sizeof(double) * (size_t)n);
+  double inv_n = 1.0 / (double)n;
+  for (int32_t u = 0; u < n; u++) {
+    double s = 0.0;
+    if (w)
+      for (int32_t e = ro[u]; e < ro[u + 1]; e++) s += w[e];
+    else
+      s = (double)(ro[u + 1] - ro[u]);
+    S[u] = s;
+  }
+  for (int32_t v = 0; v < n; v++) lambda[v] = c[v];
+  int32_t it = 0;
+  while (it < max_iter) {
+    double total = 0.0;
+    for (int32_t v = 0; v < n; v++) total += lambda[v];
+    double res = 0.0;
+    for (int32_t u = 0; u < n; u++) {
+      double acc = 0.0;
+      int32_t deg = ro[u + 1] - ro[u];
+      if (deg == 0) {
+        acc = total * inv_n;
+      } else {
+        for (int32_t e = ro[u]; e < ro[u + 1]; e++) {
+          double puv = w ? w[e] / S[u] : 1.0 / S[u];
+          acc += puv * lambda[col[e]];
+        }
+      }
+      ln[u] = c[u] + d * acc;
+      res += fabs(ln[u] - lambda[u]);
+    }
+    memcpy(lambda, ln, sizeof(double) * (size_t)n);
+    it++;
+    if (res <= tol) break;
+  }
+  if (grad_t)
+    for (int32_t v = 0; v < n; v++) grad_t[v] = (1.0 - d) * lambda[v];
+  if (grad_w) {
+    for (int32_t u = 0; u < n; u++) {
+      if (ro[u + 1] == ro[u]) continue;
+      if (w) {
+        double mean = 0.0;
+        for (int32_t e = ro[u]; e < ro[u + 1]; e++)
+          mean += (w[e] / S[u]) * lambda[col[e]];
+        double scale = d * r[u] / S[u];
+        for (int32_t e = ro[u]; e < ro[u + 1]; e++)
+          grad_w[e] = scale * (lambda[col[e]] - mean);
+      } else {
+        for (int32_t e = ro[u]; e < ro[u + 1]; e++)
+          grad_w[e] = d * r[u] * lambda[col[e]];
+      }
+    }
+  }
+  free(S);
+  free(ln);
+  return it;
+}

@@ -1,0 +1,171 @@
+"""GPU parity of BFS / SSSP / CC through the C ABI: bit-exact against the
+oracle on the golden KATs and seeded graphs, through both expand strategies
+(LBS and tiered), and at full R-MAT scale-22 size through size-independent
+characterisations of the results."""
+import numpy as np
+import pytest
+
+import gpugraph_amd as gg
+import oracle
+from gpugraph_amd import _abi, workloads
+
+pytestmark = pytest.mark.gpu
+INF = gg.SSSP_INF
+
+
+def csr_of(ro, col, w=None):
+    return gg.CSR(np.asarray(ro, np.int32), np.asarray(col, np.int32), None if w is None else np.asarray(w, np.int32))
+
+
+def test_kats(gpu_available, kat_graphs):
+    for k in kat_graphs:
+        with gg.Graph(csr_of(k["row_offsets"], k["col_indices"], k["weights"]), directed=k["directed"]) as g:
+            for s, exp in k["bfs"].items():
+                for push in (False, True):
+                    d, _ = g.bfs(int(s), push_only=push)
+                    assert d.tolist() == exp, (k["name"], s, push)
+            for s, exp in k["sssp"].items():
+                d, _ = g.sssp(int(s))
+                assert d.tolist() == exp, (k["name"], s)
+        with gg.Graph(csr_of(k["sym_row_offsets"], k["sym_col_indices"])) as g:
+            for hook in (False, True):
+                lab, _ = g.cc(hook=hook)
+                assert lab.tolist() == k["cc_sym"].tolist(), (k["name"], hook)
+
+
+def test_config1_erdos_renyi(gpu_available):
+    c = workloads.config1_er().with_weights("i32", 1, 64, symmetric=True, seed=101)
+    with gg.Graph(c) as g:
+        d, st = g.bfs(0)
+        np.testing.assert_array_equal(d, oracle.bfs(c.row_offsets, c.col_indices, 0))
+        assert st["reached_vertices"] == int((d >= 0).sum())
+        assert st["reached_edges"] == int(c.degrees()[d >= 0].sum())
+        dist, _ = g.sssp(0)
+        np.testing.assert_array_equal(dist, oracle.sssp(c.row_offsets, c.col_indices, c.values, 0, selection=True))
+
+
+@pytest.mark.parametrize("undirected", [True, False])
+def test_rmat_bfs_sssp(gpu_available, undirected):
+    c = gg.gen_rmat(15, 16, undirected=undirected, seed=21).with_weights("i32", 1, 64, symmetric=undirected,
+                                                                         seed=22)
+    deg = c.degrees()
+    isolated = int(np.flatnonzero(deg == 0)[0])
+    with gg.Graph(c, directed=not undirected) as g:
+        for src in (workloads.max_degree_vertex(c), 1, isolated):
+            exp = oracle.bfs(c.row_offsets, c.col_indices, src)
+            for push in (False, True):
+                d, st = g.bfs(src, push_only=push)
+                np.testing.assert_array_equal(d, exp)
+            dist, _ = g.sssp(src)
+            np.testing.assert_array_equal(dist, oracle.sssp(c.row_offsets, c.col_indices, c.values, src))
+
+
+@pytest.mark.parametrize("threshold", [0, 1 << 30])
+def test_expand_strategies_agree(gpu_available, threshold):
+    """threshold 0 forces the LBS expand every iteration, 2^30 the tiered one."""
+    c = gg.gen_rmat(14, 16, undirected=True, seed=23).with_weights("i32", 1, 64, symmetric=True, seed=24)
+    src = workloads.max_degree_vertex(c)
+    with gg.Graph(c) as g:
+        for wave, block in ((32, 1024), (4, 16), (64, 1 << 20)):
+            g.set_options(threshold=threshold, wave_degree=wave, block_degree=block)
+            d, _ = g.bfs(src, push_only=True)
+            np.testing.assert_array_equal(d, oracle.bfs(c.row_offsets, c.col_indices, src))
+            dist, _ = g.sssp(src)
+            np.testing.assert_array_equal(dist, oracle.sssp(c.row_offsets, c.col_indices, c.values, src))
+            lab, _ = g.cc()
+            np.testing.assert_array_equal(lab, oracle.cc(c.row_offsets, c.col_indices)[0])
+
+
+def test_lattice_cc(gpu_available):
+    c = workloads.config3_lattice(side=256)
+    exp, _ = oracle.cc(c.row_offsets, c.col_indices)
+    with gg.Graph(c) as g:
+        for hook in (False, True):
+            lab, st = g.cc(hook=hook)
+            np.testing.assert_array_equal(lab, exp)
+        assert st["reached_vertices"] == c.n
+
+
+def test_direction_switch_parameters(gpu_available):
+    c = gg.gen_rmat(16, 16, undirected=True, seed=25)
+    src = workloads.max_degree_vertex(c)
+    exp = oracle.bfs(c.row_offsets, c.col_indices, src)
+    with gg.Graph(c) as g:
+        for alpha, beta in ((1, 1), (14, 24), (1000000, 1)):
+            g.set_options(bfs_alpha=alpha, bfs_beta=beta)
+            d, st = g.bfs(src)
+            np.testing.assert_array_equal(d, exp)
+        g.set_options(bfs_alpha=1, bfs_beta=1000000)
+        d, st = g.bfs(src)
+        np.testing.assert_array_equal(d, exp)
+
+
+def test_device_outputs_and_instrumentation(gpu_available):
+    import torch
+
+    c = workloads.config1_er().with_weights("i32", 1, 64, symmetric=True, seed=101)
+    out = torch.empty(c.n, dtype=torch.int32, device="cuda")
+    with gg.Graph(c) as g:
+        flags = _abi.GG_OUT_DEVICE | _abi.GG_INSTRUMENT
+        g.bfs(0, flags=flags, out=out.data_ptr())
+        g.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), oracle.bfs(c.row_offsets, c.col_indices, 0))
+        g.sssp(0, flags=flags, out=out.data_ptr())
+        g.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), oracle.sssp(c.row_offsets, c.col_indices, c.values, 0))
+        g.cc(hook=True, flags=flags, out=out.data_ptr())
+        g.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), oracle.cc(c.row_offsets, c.col_indices)[0])
+        prof = g.profile()
+        assert any(k.startswith("bfs.") for k in prof) and any(k.startswith("sssp.") for k in prof)
+        assert all(v["launches"] > 0 and v["total_ms"] >= 0 for v in prof.values())
+
+
+def test_error_paths(gpu_available):
+    c = workloads.config1_er()
+    with gg.Graph(c) as g:
+        with pytest.raises(gg.GGError) as e:
+            g.bfs(c.n)
+        assert e.value.code == _abi.GG_ERR_INVALID
+        with pytest.raises(gg.GGError) as e:
+            g.sssp(0)  # no int32 weights on this graph
+        assert e.value.code == _abi.GG_ERR_INVALID
+        with pytest.raises(gg.GGError):
+            g.set_options(wave_degree=0)
+    with gg.Graph(c, directed=True) as g:
+        with pytest.raises(gg.GGError) as e:
+            g.cc(hook=True)
+        assert e.value.code == _abi.GG_ERR_UNSUPPORTED
+
+
+@pytest.fixture(scope="module")
+def rmat22():
+    return workloads.config2_rmat(22)
+
+
+def test_config2_full_size(gpu_available, rmat22):
+    """R-MAT scale 22 (4.19M vertices, 128M CSR entries): BFS bit-exact
+    against the oracle; SSSP checked by the exact characterisation of
+    shortest-path distances with positive weights (no edge can relax them,
+    and every reached vertex except the source has a tight in-edge)."""
+    c = rmat22
+    src = workloads.max_degree_vertex(c)
+    ro, col, w = c.row_offsets, c.col_indices, c.values
+    rows = np.repeat(np.arange(c.n, dtype=np.int64), c.degrees())
+    with gg.Graph(c) as g:
+        d, st = g.bfs(src)
+        np.testing.assert_array_equal(d, oracle.bfs(ro, col, src))
+        assert st["pull_iterations"] >= 1
+        dist, st = g.sssp(src)
+    assert dist[src] == 0
+    du = dist[rows].astype(np.int64)
+    dv = dist[col].astype(np.int64)
+    reached_u = du < INF
+    assert np.all(dv[reached_u] <= du[reached_u] + w[reached_u])  # nothing relaxes
+    tight = np.zeros(c.n, bool)
+    hit = reached_u & (dv == du + w)
+    tight[col[hit]] = True
+    reached = dist < INF
+    reached[src] = False
+    assert np.all(tight[reached])  # every reached vertex has a shortest-path parent
+    assert np.array_equal(dist < INF, d >= 0)  # same reachable set as BFS (undirected)
