@@ -46,6 +46,7 @@ class Options(C.Structure):
         ("bfs_beta", C.c_int32),
         ("lbs_tile", C.c_int32),
         ("sssp_delta", C.c_int32),
+        ("pull_alpha", C.c_int32),
     ]
 
 
@@ -148,6 +149,26 @@ class GGError(RuntimeError):
         self.code = code
 
 
+def _share_torch_hip_runtime():
+    """Make libgasgraph.so bind to the HIP runtime torch ships, if torch is
+    installed.  torch's wheel carries its own libamdhip64.so with the same
+    soname (libamdhip64.so.7) as /opt/rocm's; loading it first (RTLD_GLOBAL)
+    lets the dynamic linker resolve our NEEDED entry to that copy, so one HIP
+    runtime serves both and device pointers pass freely between them.  Loaded
+    the other way round, the process would hold two runtimes and torch's
+    device initialisation fails."""
+    import importlib.util
+
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return
+    for loc in spec.submodule_search_locations:
+        hip = os.path.join(loc, "lib", "libamdhip64.so")
+        if os.path.exists(hip):
+            C.CDLL(hip, mode=C.RTLD_GLOBAL)
+            return
+
+
 def load_library(path: str | None = None):
     """Load libgasgraph.so and attach prototypes; raises if it is missing."""
     global _lib
@@ -158,6 +179,7 @@ def load_library(path: str | None = None):
         raise ImportError(
             f"{p} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
             "or `make -C gpugraph_amd/csrc`")
+    _share_torch_hip_runtime()
     lib = C.CDLL(p)
     for name, (res, args) in PROTOTYPES.items():
         fn = getattr(lib, name)

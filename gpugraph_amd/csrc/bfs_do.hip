@@ -4,15 +4,17 @@
 // checked against CPUBFS bfs.cu:57-111): depth[src] = 0, unreached = -1.
 //
 //   top-down  (queue -> queue): LBS expand over the frontier's out-edges
-//             (k_lbs_count / k_lbs_scan / k_lbs_expand with BfsPushOp);
-//             first touch wins an atomic test-and-set in the visited bitmap
-//             and stamps the depth.
+//             (GASengine::lbs_launch with BfsPushOp); first touch wins an
+//             atomic test-and-set in the visited bitmap and stamps the depth.
 //   bottom-up (bitmap -> bitmap): one wave per 64 consecutive vertices;
-//             each unvisited vertex scans its in-edges until it finds a
-//             parent in the frontier bitmap (n/8 bytes, L2-resident), the
-//             wave writes its two bitmap words with plain stores.
-//   switch    top-down -> bottom-up when m_f > m_u / alpha; back when the
-//             frontier shrinks below n / beta (Beamer et al., SC'12).
+//             each unvisited vertex looks for a parent in the frontier
+//             bitmap (n/8 bytes, L2-resident) among its in-edges, stopping
+//             at the first hit.  Degree <= wave_degree: one lane, 4 loads in
+//             flight per step; larger: the whole wave, 64 edges per step.
+//             The wave owns its two bitmap words and writes them with plain
+//             stores.
+//   switch    top-down -> bottom-up when m_f > m_u / alpha and the frontier
+//             grows; back when it shrinks below n / beta (Beamer et al.).
 //
 // Byte model (per level, F frontier, E edges examined, N newly visited,
 // U unvisited vertices scanned): top-down 12F + 8E + 8N; bottom-up
@@ -43,7 +45,8 @@ struct BfsPushOp {
   __device__ __forceinline__ void prologue(int32_t) {}
   __device__ __forceinline__ int32_t degree(int32_t v) { return ro[v + 1] - ro[v]; }
   __device__ __forceinline__ int32_t row_begin(int32_t v) { return ro[v]; }
-  __device__ __forceinline__ bool edge(int32_t, int32_t slot, int32_t &out) {
+  __device__ __forceinline__ int32_t item_value(int32_t) { return 0; }
+  __device__ __forceinline__ bool edge(int32_t, int32_t, int32_t slot, int32_t &out) {
     const int32_t nb = col[slot];
     if (!bit_test_and_set(visited, nb)) return false;
     depth[nb] = next_depth;
@@ -66,10 +69,15 @@ __global__ void k_bfs_init(int32_t *depth, uint32_t *visited, int32_t *queue, in
   counters[1] = (unsigned long long)(ro[src + 1] - ro[src]);
 }
 
+__device__ __forceinline__ uint32_t fbit(const uint32_t *__restrict__ b, int32_t u) {
+  return (b[u >> 5] >> (u & 31)) & 1u;
+}
+
 __global__ void __launch_bounds__(256)
 k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, const int32_t *__restrict__ ro,
                 int32_t *__restrict__ depth, uint32_t *__restrict__ visited, const uint32_t *__restrict__ front,
-                uint32_t *__restrict__ next, int32_t n, int32_t next_depth, unsigned long long *__restrict__ counters) {
+                uint32_t *__restrict__ next, int32_t n, int32_t next_depth, int32_t serial_max,
+                unsigned long long *__restrict__ counters) {
   const int32_t words = (n + 31) >> 5;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   const int lane = lane_id();
@@ -78,19 +86,46 @@ k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, 
     const int32_t w = (int32_t)(2 * p + (lane >> 5));
     const int32_t v = (int32_t)(p * 64 + lane);
     bool todo = false;
-    if (v < n) todo = !((visited[w] >> (lane & 31)) & 1u);
+    int32_t b = 0, e = 0;
+    if (v < n) {
+      todo = !((visited[w] >> (lane & 31)) & 1u);
+      if (todo) {
+        b = co[v];
+        e = co[v + 1];
+        todo = e > b;
+      }
+    }
+    if (todo) scanned++;
+    const bool coop = todo && (e - b) > serial_max;
     bool found = false;
-    if (todo) {
-      scanned++;
-      const int32_t b = co[v], e = co[v + 1];
-      for (int32_t k = b; k < e; k++) {
+    if (todo && !coop) {
+      int32_t k = b;
+      for (; k + 4 <= e && !found; k += 4) {
+        const int32_t u0 = ri[k], u1 = ri[k + 1], u2 = ri[k + 2], u3 = ri[k + 3];
+        found = (fbit(front, u0) | fbit(front, u1) | fbit(front, u2) | fbit(front, u3)) != 0;
+        examined += 4;
+      }
+      for (; k < e && !found; k++) {
+        found = fbit(front, ri[k]) != 0;
         examined++;
-        const int32_t u = ri[k];
-        if ((front[u >> 5] >> (u & 31)) & 1u) {
-          found = true;
+      }
+    }
+    uint64_t cm = ballot(coop);
+    while (cm) {
+      const int l = __builtin_ctzll(cm);
+      cm &= cm - 1;
+      const int32_t wb = __shfl(b, l), we = __shfl(e, l);
+      bool hit = false;
+      for (int32_t k0 = wb; k0 < we; k0 += kWave) {
+        const int32_t k = k0 + lane;
+        const bool h = k < we && fbit(front, ri[k]);
+        if (lane == 0) examined += (unsigned long long)min(kWave, we - k0);
+        if (ballot(h)) {
+          hit = true;
           break;
         }
       }
+      if (lane == l) found = hit;
     }
     const uint64_t mask = ballot(found);
     if (found) {
@@ -141,9 +176,9 @@ k_bits_to_queue(const uint32_t *__restrict__ bits, int32_t words, int32_t *__res
     if (lane_id() == 0 && total) start = atomicAdd(qsize, total);
     start = __shfl(start, 0) + incl - cnt;
     while (word) {
-      const int b = __builtin_ctz(word);
+      const int bb = __builtin_ctz(word);
       word &= word - 1;
-      queue[start++] = (int32_t)(w * 32 + b);
+      queue[start++] = (int32_t)(w * 32 + bb);
     }
   }
 }
@@ -155,15 +190,11 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
   const int32_t n = g->n;
   const int32_t words = (n + 31) / 32;
   hipStream_t s = g->stream;
-  GG_TRY(g->frontier[0].ensure(4 * (size_t)n));
-  GG_TRY(g->frontier[1].ensure(4 * (size_t)n));
+  LbsBuffers lb;
+  GG_TRY(lbs_buffers(g, &lb));
   GG_TRY(g->bits[0].ensure(4 * (size_t)words));  // visited
   GG_TRY(g->bits[1].ensure(4 * (size_t)words));  // frontier bitmaps (ping-pong)
   GG_TRY(g->bits[2].ensure(4 * (size_t)words));
-  GG_TRY(g->scan_local.ensure(4 * (size_t)n));
-  GG_TRY(g->partials.ensure(4 * ((size_t)n / kCountTile + 2)));
-  GG_TRY(g->info.ensure(sizeof(LbsInfo)));
-  GG_TRY(g->counters.ensure(64 * sizeof(unsigned long long)));
   int32_t *depth;
   if (flags & GG_OUT_DEVICE) {
     GG_CHECK_ARG(depth_out, "depth is NULL");
@@ -175,10 +206,11 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
   uint32_t *visited = g->bits[0].as<uint32_t>();
   uint32_t *fbits[2] = {g->bits[1].as<uint32_t>(), g->bits[2].as<uint32_t>()};
   int32_t *queue[2] = {g->frontier[0].as<int32_t>(), g->frontier[1].as<int32_t>()};
-  // counters: [0..3] per level (see BfsPushOp), [8],[9]: queue sizes
+  // counters: [0..3] per level (see BfsPushOp), [8],[9] as int32: queue sizes
   unsigned long long *ctr = g->counters.as<unsigned long long>();
   int32_t *qsize = reinterpret_cast<int32_t *>(ctr + 8);
   g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
+  LbsPhaseTimer lbs_timer(g->prof, "bfs", "top_down");
 
   hipEvent_t e0, e1;
   GG_HIP(hipEventCreate(&e0));
@@ -232,7 +264,7 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
         TimedRegion t(g->prof, "bfs.bottom_up");
         hipLaunchKernelGGL(k_bfs_bottom_up, dim3(grid_for((int64_t)words * 32, 256, 8192)), dim3(256), 0, s,
                            g->dg.column_offsets, g->dg.row_indices, g->dg.row_offsets, depth, visited, fbits[cur],
-                           fbits[nxt], n, levels + 1, ctr);
+                           fbits[nxt], n, levels + 1, g->opt.wave_degree, ctr);
         GG_HIP(hipGetLastError());
       }
       GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
@@ -246,22 +278,8 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
       pull_levels++;
     } else {
       BfsPushOp op{g->dg.row_offsets, g->dg.column_indices, depth, visited, ctr, levels + 1, 0ull};
-      {
-        TimedRegion t(g->prof, "bfs.lbs_count");
-        hipLaunchKernelGGL((k_lbs_count<BfsPushOp, false>), dim3(grid_for(nf, kCountTile)), dim3(kCountBlock), 0, s,
-                           op, queue[cur], qsize + cur, g->scan_local.as<int32_t>(), g->partials.as<int32_t>());
-      }
-      {
-        TimedRegion t(g->prof, "bfs.lbs_scan");
-        hipLaunchKernelGGL(k_lbs_scan, dim3(1), dim3(kScanBlock), 0, s, qsize + cur, g->partials.as<int32_t>(),
-                           g->info.as<LbsInfo>(), ctr + 2);
-      }
-      {
-        TimedRegion t(g->prof, "bfs.top_down");
-        hipLaunchKernelGGL((k_lbs_expand<BfsPushOp, 8>), dim3(2048), dim3(kExpandBlock), 0, s, op, queue[cur],
-                           qsize + cur, g->scan_local.as<int32_t>(), g->partials.as<int32_t>(),
-                           g->info.as<LbsInfo>(), queue[nxt], qsize + nxt);
-      }
+      lbs_launch(op, false, queue[cur], qsize + cur, nf, (int64_t)n + g->m, lb, queue[nxt], qsize + nxt, s,
+                 lbs_timer);
       GG_HIP(hipGetLastError());
       GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 10 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
       GG_HIP(hipStreamSynchronize(s));

@@ -154,6 +154,7 @@ int run_program(gg_graph *g, int num_srcs, int *srcs, typename P::DataType *out,
 namespace gg {
 int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth, uint32_t flags, gg_stats *stats);
 int cc_hooking(gg_graph *g, int32_t *label, uint32_t flags, gg_stats *stats);
+int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist, uint32_t flags, gg_stats *stats);
 }  // namespace gg
 
 extern "C" {
@@ -173,6 +174,7 @@ int gg_sssp(gg_graph *g, int32_t src, int32_t *dist, uint32_t flags, gg_stats *s
   GG_CHECK_ARG(src >= 0 && src < g->n, "src %d out of range [0, %d)", src, g->n);
   GG_CHECK_ARG(g->value_dtype == GG_VALUES_I32, "SSSP needs int32 edge weights");
   int s = src;
+  if (!(flags & GG_SSSP_GAS)) return gg::sssp_near_far(g, src, dist, flags, stats);
   return gg::run_program<sssp>(g, 1, &s, dist, flags, stats, sssp::INIT_VALUE, true);
 }
 

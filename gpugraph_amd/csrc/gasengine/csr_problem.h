@@ -47,6 +47,7 @@ struct Workspace {
   void *gather_tmp = nullptr;
   int32_t *scan_local = nullptr;
   int32_t *partials = nullptr;
+  int32_t *parts = nullptr;  // LBS tile boundaries
   LbsInfo *info = nullptr;
   int32_t *sizes = nullptr;                   // [0],[1]: frontier sizes
   unsigned long long *counters = nullptr;     // [0]: edges inspected
@@ -55,7 +56,7 @@ struct Workspace {
 
   static int32_t words(int32_t n) { return (n + 31) / 32; }
 
-  hipError_t Allocate(int32_t n, size_t gather_elem_bytes) {
+  hipError_t Allocate(int32_t n, int64_t m, size_t gather_elem_bytes) {
     if (capacity >= n && gather_bytes >= gather_elem_bytes * (size_t)n && frontier[0]) return hipSuccess;
     Release();
     const size_t nn = (size_t)(n > 0 ? n : 1);
@@ -69,6 +70,7 @@ struct Workspace {
     GAS_RETURN_IF(hipMalloc(&gather_tmp, nn * (gather_elem_bytes > 4 ? gather_elem_bytes : 4)));
     GAS_RETURN_IF(hipMalloc(&scan_local, nn * 4));
     GAS_RETURN_IF(hipMalloc(&partials, np * 4));
+    GAS_RETURN_IF(hipMalloc(&parts, lbs_partition_entries(n, m) * 4));
     GAS_RETURN_IF(hipMalloc(&info, sizeof(LbsInfo)));
     GAS_RETURN_IF(hipMalloc(&sizes, 4 * sizeof(int32_t)));
     GAS_RETURN_IF(hipMalloc(&counters, 8 * sizeof(unsigned long long)));
@@ -96,7 +98,7 @@ struct Workspace {
       if (frontier_values[i]) (void)hipFree(frontier_values[i]);
       frontier_values[i] = nullptr;
     }
-    void *ps[] = {changed, gather_tmp, scan_local, partials, info, sizes, counters};
+    void *ps[] = {changed, gather_tmp, scan_local, partials, parts, info, sizes, counters};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     if (h_sizes) (void)hipHostFree(h_sizes);
@@ -104,6 +106,7 @@ struct Workspace {
     gather_tmp = nullptr;
     scan_local = nullptr;
     partials = nullptr;
+    parts = nullptr;
     info = nullptr;
     sizes = nullptr;
     counters = nullptr;
@@ -156,7 +159,7 @@ struct CsrProblem {
   hipError_t Attach(const DeviceGraph &g, hipStream_t s) {
     graph = g;
     stream = s;
-    return ws.Allocate(g.nodes, sizeof(typename Program::GatherType));
+    return ws.Allocate(g.nodes, g.edges, sizeof(typename Program::GatherType));
   }
 
   // Reference form (csr_problem.cuh:512-524, upstream 10-argument overload):

@@ -142,21 +142,12 @@ class EnactorVertexCentric {
         ProgramOp<Program> op{st};
         const bool first = p == 0;
         if (lbs) {
-          tb("lbs_count");
-          if (first)
-            hipLaunchKernelGGL((k_lbs_count<ProgramOp<Program>, true>), dim3(grid_for(nf, kCountTile)),
-                               dim3(kCountBlock), 0, s, op, f_in, sz_in, ws.scan_local, ws.partials);
-          else
-            hipLaunchKernelGGL((k_lbs_count<ProgramOp<Program>, false>), dim3(grid_for(nf, kCountTile)),
-                               dim3(kCountBlock), 0, s, op, f_in, sz_in, ws.scan_local, ws.partials);
-          te("lbs_count");
-          tb("lbs_scan");
-          hipLaunchKernelGGL(k_lbs_scan, dim3(1), dim3(kScanBlock), 0, s, sz_in, ws.partials, ws.info, ws.counters);
-          te("lbs_scan");
-          tb("lbs_expand");
-          hipLaunchKernelGGL((k_lbs_expand<ProgramOp<Program>, 8>), dim3(config.expand_grid), dim3(kExpandBlock), 0,
-                             s, op, f_in, sz_in, ws.scan_local, ws.partials, ws.info, f_out, sz_out);
-          te("lbs_expand");
+          static const char *names[4] = {"lbs_count", "lbs_scan", "lbs_partition", "lbs_expand"};
+          LbsBuffers b{ws.scan_local, ws.partials, ws.info, ws.parts, ws.counters, config.expand_grid};
+          lbs_launch(op, first, f_in, sz_in, nf, (int64_t)n + g.edges, b, f_out, sz_out, s, [&](int phase) {
+            if (phase > 0) te(names[phase - 1]);
+            if (phase < 4) tb(names[phase]);
+          });
         } else {
           if (first && (st.do_apply || st.do_post)) {
             tb("prologue");

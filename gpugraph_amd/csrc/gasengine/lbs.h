@@ -6,14 +6,17 @@
 //  * Load-balance search (LBS), the role of the reference's two-phase
 //    "mgpu" expand (mgpukernel/kernel.cuh:63-180 + moderngpu Scan /
 //    MergePathPartitions / CTALoadBalance):
-//      k_lbs_count   per frontier item: prologue + degree, workgroup-local
-//                    exclusive scan, one partial per workgroup
-//      k_lbs_scan    one workgroup scans the partials -> global offsets
-//      k_lbs_expand  the merge path of (items + edges) is cut into equal
-//                    tiles; each workgroup stages its items in LDS, builds an
-//                    edge->item map with a per-thread serial merge, then walks
-//                    its edges with consecutive lanes on consecutive edges
-//                    (coalesced column reads), whatever the degree skew.
+//      k_lbs_count      per frontier item: prologue + degree, workgroup-local
+//                       exclusive scan, one partial per 1024 items
+//      k_lbs_scan       one workgroup scans the partials -> global offsets
+//      k_lbs_partition  the merge path of (items + edges) is cut into tiles
+//                       of kLbsTile positions; one thread per tile boundary
+//                       finds its (item, edge) split by binary search
+//      k_lbs_expand     each workgroup stages its tile's items in LDS, builds
+//                       an edge->item map with a per-thread serial merge,
+//                       then walks its edges with consecutive lanes on
+//                       consecutive edges (coalesced column reads), whatever
+//                       the degree skew.
 //  * Tiered dynamic expand, the role of expand_atomic (expand_atomic/cta.cuh:
 //    264-417, 701-754), re-tiled for wave64: degree >= block_degree -> whole
 //    workgroup per vertex, >= wave_degree -> one wave per vertex, rest -> an
@@ -26,7 +29,9 @@
 //   void    prologue(int32_t v)          once per frontier item before degree
 //   int32_t degree(int32_t v)            edges to expand (0 = none)
 //   int32_t row_begin(int32_t v)         first edge slot of v
-//   bool    edge(int32_t v, int32_t slot, int32_t &out)
+//   int32_t item_value(int32_t v)        per-item value staged in LDS and
+//                                        handed to edge() (e.g. dist[v])
+//   bool    edge(int32_t v, int32_t val, int32_t slot, int32_t &out)
 //   void    finish()                     once per thread at kernel end (all
 //                                        threads; per-thread accumulators)
 #pragma once
@@ -40,13 +45,18 @@ constexpr int kCountIpt = 4;
 constexpr int kCountTile = kCountBlock * kCountIpt;  // items per scan partial
 constexpr int kScanBlock = 1024;
 constexpr int kExpandBlock = 256;
+constexpr int kExpandVt = 8;
+constexpr int kLbsTile = kExpandBlock * kExpandVt;  // merge-path positions per tile
 
 // Device-side counters of one expansion (written by the kernels).
 struct LbsInfo {
-  int32_t edges;       // total edges of the frontier (sum of degrees)
+  int32_t edges;  // total edges of the frontier (sum of degrees)
   int32_t pad;
-  int64_t path;        // merge path length = items + edges
+  int64_t path;   // merge path length = items + edges
 };
+
+// Partition-array entries needed for a frontier of at most n items over m edges.
+inline size_t lbs_partition_entries(int64_t n, int64_t m) { return (size_t)((n + m) / kLbsTile + 2); }
 
 template <class Op, bool PROLOGUE>
 __global__ void __launch_bounds__(kCountBlock)
@@ -79,20 +89,21 @@ k_lbs_count(Op op, const int32_t *__restrict__ frontier, const int32_t *__restri
   }
 }
 
-// One workgroup: exclusive scan of the per-tile partials in place, then the
-// totals into *info and the running edge counter.
-static __global__ void __launch_bounds__(kScanBlock)
+// One workgroup: exclusive scan of the per-1024-item partials in place, then
+// the totals into *info and the running edge counter.
+template <int BLOCK = kScanBlock>
+__global__ void __launch_bounds__(BLOCK)
 k_lbs_scan(const int32_t *__restrict__ fsize, int32_t *__restrict__ partials, LbsInfo *__restrict__ info,
            unsigned long long *__restrict__ edge_counter) {
-  __shared__ int32_t smem[kScanBlock / kWave + 1];
+  __shared__ int32_t smem[BLOCK / kWave + 1];
   const int32_t nf = *fsize;
   const int32_t nb = (int32_t)(((int64_t)nf + kCountTile - 1) / kCountTile);
   int32_t carry = 0;
-  for (int32_t base = 0; base < nb; base += kScanBlock) {
+  for (int32_t base = 0; base < nb; base += BLOCK) {
     int32_t i = base + threadIdx.x;
     int32_t v = i < nb ? partials[i] : 0;
     int32_t tot;
-    int32_t ex = block_exclusive_scan<kScanBlock>(v, smem, &tot);
+    int32_t ex = block_exclusive_scan<BLOCK>(v, smem, &tot);
     if (i < nb) partials[i] = carry + ex;
     carry += tot;
   }
@@ -122,28 +133,40 @@ __device__ __forceinline__ int32_t lbs_path_search(const int32_t *scan_local, co
   return (int32_t)lo;
 }
 
-template <class Op, int VT>
+// parts[t] = items consumed at diagonal min(t * kLbsTile, path), t = 0..ntiles.
+template <int DUMMY = 0>
+__global__ void __launch_bounds__(256)
+k_lbs_partition(const int32_t *__restrict__ fsize, const int32_t *__restrict__ scan_local,
+                const int32_t *__restrict__ partials, const LbsInfo *__restrict__ info, int32_t *__restrict__ parts) {
+  const int32_t nf = *fsize;
+  const int32_t ne = info->edges;
+  const int64_t path = info->path;
+  const int64_t ntiles = (path + kLbsTile - 1) / kLbsTile;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t <= ntiles; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t d = t * kLbsTile < path ? t * kLbsTile : path;
+    parts[t] = lbs_path_search(scan_local, partials, d, nf, ne);
+  }
+}
+
+template <class Op>
 __global__ void __launch_bounds__(kExpandBlock)
 k_lbs_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__restrict__ fsize,
              const int32_t *__restrict__ scan_local, const int32_t *__restrict__ partials,
-             const LbsInfo *__restrict__ info, int32_t *__restrict__ out_queue, int32_t *__restrict__ out_size) {
-  constexpr int TILE = kExpandBlock * VT;
+             const LbsInfo *__restrict__ info, const int32_t *__restrict__ parts, int32_t *__restrict__ out_queue,
+             int32_t *__restrict__ out_size) {
+  constexpr int TILE = kLbsTile;
+  constexpr int VT = kExpandVt;
   // slot k <-> frontier item i0 - 1 + k (slot 0: the item owning the tile's first edges)
   __shared__ int32_t s_offs[TILE + 1];
   __shared__ int32_t s_vert[TILE + 1];
   __shared__ int32_t s_begin[TILE + 1];
+  __shared__ int32_t s_val[TILE + 1];
   __shared__ int32_t s_slot[TILE];
-  __shared__ int32_t s_bounds[2];
-  const int32_t nf = *fsize;
-  const int32_t ne_total = info->edges;
   const int64_t path = info->path;
   for (int64_t t = blockIdx.x; t * TILE < path; t += gridDim.x) {
     const int64_t d0 = t * TILE;
     const int64_t d1 = d0 + TILE < path ? d0 + TILE : path;
-    if (threadIdx.x < 2)
-      s_bounds[threadIdx.x] = lbs_path_search(scan_local, partials, threadIdx.x ? d1 : d0, nf, ne_total);
-    __syncthreads();
-    const int32_t i0 = s_bounds[0], i1 = s_bounds[1];
+    const int32_t i0 = parts[t], i1 = parts[t + 1];
     const int32_t j0 = (int32_t)(d0 - i0), j1 = (int32_t)(d1 - i1);
     const int32_t ni = i1 - i0, ne = j1 - j0;
     for (int k = threadIdx.x; k <= ni; k += kExpandBlock) {
@@ -153,10 +176,12 @@ k_lbs_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__restr
         s_vert[k] = v;
         s_offs[k] = lbs_offset(scan_local, partials, i);
         s_begin[k] = op.row_begin(v);
+        s_val[k] = op.item_value(v);
       } else {
         s_vert[k] = -1;
         s_offs[k] = 0;
         s_begin[k] = 0;
+        s_val[k] = 0;
       }
     }
     __syncthreads();
@@ -191,7 +216,7 @@ k_lbs_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__restr
       int32_t out = 0;
       if (k < ne) {
         const int32_t s = s_slot[k];
-        keep = op.edge(s_vert[s], s_begin[s] + (j0 + k - s_offs[s]), out);
+        keep = op.edge(s_vert[s], s_val[s], s_begin[s] + (j0 + k - s_offs[s]), out);
       }
       wave_append(keep, out, out_queue, out_size);
     }
@@ -209,20 +234,24 @@ k_tiered_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__re
   __shared__ int32_t s_offs[B];
   __shared__ int32_t s_vert[B];
   __shared__ int32_t s_begin[B];
+  __shared__ int32_t s_val[B];
   __shared__ int32_t s_scan[B / kWave + 1];
   __shared__ int32_t s_owner;
-  __shared__ int32_t s_big[3];
+  __shared__ int32_t s_big[4];
   __shared__ unsigned long long s_red[B / kWave];
   const int32_t nf = *fsize;
   unsigned long long inspected = 0;
   for (int64_t base = (int64_t)blockIdx.x * B; base < nf; base += (int64_t)gridDim.x * B) {
     const int64_t i = base + threadIdx.x;
-    int32_t v = -1, deg = 0, begin = 0;
+    int32_t v = -1, deg = 0, begin = 0, val = 0;
     if (i < nf) {
       v = frontier[i];
       if (PROLOGUE) op.prologue(v);
       deg = op.degree(v);
-      if (deg > 0) begin = op.row_begin(v);
+      if (deg > 0) {
+        begin = op.row_begin(v);
+        val = op.item_value(v);
+      }
     }
     inspected += (unsigned long long)deg;
     // Tier 1: whole workgroup per high-degree vertex.
@@ -237,15 +266,16 @@ k_tiered_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__re
         s_big[0] = v;
         s_big[1] = begin;
         s_big[2] = deg;
+        s_big[3] = val;
         deg = 0;
       }
       __syncthreads();
-      const int32_t bv = s_big[0], bb = s_big[1], bd = s_big[2];
+      const int32_t bv = s_big[0], bb = s_big[1], bd = s_big[2], bval = s_big[3];
       for (int32_t k0 = 0; k0 < bd; k0 += B) {
         const int32_t k = k0 + threadIdx.x;
         bool keep = false;
         int32_t out = 0;
-        if (k < bd) keep = op.edge(bv, bb + k, out);
+        if (k < bd) keep = op.edge(bv, bval, bb + k, out);
         wave_append(keep, out, out_queue, out_size);
       }
       __syncthreads();
@@ -255,12 +285,12 @@ k_tiered_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__re
     while (mask) {
       const int l = __builtin_ctzll(mask);
       mask &= mask - 1;
-      const int32_t wv = __shfl(v, l), wb = __shfl(begin, l), wd = __shfl(deg, l);
+      const int32_t wv = __shfl(v, l), wb = __shfl(begin, l), wd = __shfl(deg, l), wval = __shfl(val, l);
       for (int32_t k0 = 0; k0 < wd; k0 += kWave) {
         const int32_t k = k0 + lane_id();
         bool keep = false;
         int32_t out = 0;
-        if (k < wd) keep = op.edge(wv, wb + k, out);
+        if (k < wd) keep = op.edge(wv, wval, wb + k, out);
         wave_append(keep, out, out_queue, out_size);
       }
     }
@@ -271,6 +301,7 @@ k_tiered_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__re
     s_offs[threadIdx.x] = ex;
     s_vert[threadIdx.x] = v;
     s_begin[threadIdx.x] = begin;
+    s_val[threadIdx.x] = val;
     __syncthreads();
     for (int32_t k0 = 0; k0 < total; k0 += B) {
       const int32_t k = k0 + threadIdx.x;
@@ -286,7 +317,7 @@ k_tiered_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__re
             hi = mid;
         }
         const int32_t j = lo - 1;
-        keep = op.edge(s_vert[j], s_begin[j] + (k - s_offs[j]), out);
+        keep = op.edge(s_vert[j], s_val[j], s_begin[j] + (k - s_offs[j]), out);
       }
       wave_append(keep, out, out_queue, out_size);
     }
@@ -307,6 +338,141 @@ k_frontier_prologue(Op op, const int32_t *__restrict__ frontier, const int32_t *
   const int32_t nf = *fsize;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += (int64_t)gridDim.x * blockDim.x)
     op.prologue(frontier[i]);
+}
+
+// ---- pull over all vertices: merge path of (n vertices, m in-edges) -------
+//
+// Items are the vertices 0..n-1 themselves, so their edge offsets are the
+// CSC column_offsets array and the tile partition depends only on the graph:
+// k_static_partition runs once per graph.  k_pull_reduce maps each tile's
+// edges to vertex slots like k_lbs_expand, folds per-edge values into an LDS
+// accumulator per vertex, and reports each vertex's result with a flag
+// telling whether all of its in-edges fell inside the tile (then the tile is
+// the only writer of that vertex and may store plainly).
+//
+// PullOp concept:
+//   T    identity()
+//   T    edge(int32_t v, int32_t slot)             value of in-edge slot of v
+//   void combine(T *lds_acc, T val)               LDS reduction (e.g. atomicMin)
+//   void item_done(int32_t v, T acc, bool whole)  result of v for this tile
+//   void finish()
+
+template <int DUMMY = 0>
+__global__ void __launch_bounds__(256)
+k_static_partition(const int32_t *__restrict__ offs, int32_t n, int64_t m, int32_t *__restrict__ parts) {
+  const int64_t path = (int64_t)n + m;
+  const int64_t ntiles = (path + kLbsTile - 1) / kLbsTile;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t <= ntiles; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t d = t * kLbsTile < path ? t * kLbsTile : path;
+    int64_t lo = d - m > 0 ? d - m : 0, hi = d < n ? d : n;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if ((int64_t)offs[mid] <= d - mid - 1)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    parts[t] = (int32_t)lo;
+  }
+}
+
+template <class Op, class T>
+__global__ void __launch_bounds__(kExpandBlock)
+k_pull_reduce(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const int32_t *__restrict__ parts) {
+  constexpr int TILE = kLbsTile;
+  constexpr int VT = kExpandVt;
+  __shared__ int32_t s_offs[TILE + 2];  // s_offs[k] = offs[i0 - 1 + k], k = 0..ni+1
+  __shared__ T s_acc[TILE + 1];
+  __shared__ int32_t s_slot[TILE];
+  const int64_t path = (int64_t)n + m;
+  for (int64_t t = blockIdx.x; t * TILE < path; t += gridDim.x) {
+    const int64_t d0 = t * TILE;
+    const int64_t d1 = d0 + TILE < path ? d0 + TILE : path;
+    const int32_t i0 = parts[t], i1 = parts[t + 1];
+    const int32_t j0 = (int32_t)(d0 - i0), j1 = (int32_t)(d1 - i1);
+    const int32_t ni = i1 - i0, ne = j1 - j0;
+    for (int k = threadIdx.x; k <= ni + 1; k += kExpandBlock) {
+      const int32_t i = i0 - 1 + k;
+      s_offs[k] = i < 0 ? 0 : offs[i];  // i <= i1 <= n: offs has n + 1 entries
+      if (k <= ni) s_acc[k] = op.identity();
+    }
+    __syncthreads();
+    {
+      const int32_t dl = (int32_t)threadIdx.x * VT;
+      if (dl < ni + ne) {
+        int32_t lo = dl - ne > 0 ? dl - ne : 0, hi = dl < ni ? dl : ni;
+        while (lo < hi) {
+          int32_t mid = (lo + hi) >> 1;
+          if (s_offs[mid + 1] <= j0 + (dl - mid) - 1)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        int32_t li = lo, lj = dl - lo;
+#pragma unroll
+        for (int k = 0; k < VT; k++) {
+          if (li + lj >= ni + ne) break;
+          if (li < ni && s_offs[li + 1] <= j0 + lj) {
+            li++;
+          } else {
+            s_slot[lj] = li;
+            lj++;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int32_t k = threadIdx.x; k < ne; k += kExpandBlock) {
+      const int32_t s = s_slot[k];
+      const T val = op.edge(i0 - 1 + s, j0 + k);
+      op.combine(&s_acc[s], val);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k <= ni; k += kExpandBlock) {
+      const int32_t v = i0 - 1 + k;
+      if (v < 0) continue;
+      const bool whole = k >= 1 && s_offs[k] >= j0 && s_offs[k + 1] <= j1;
+      op.item_done(v, s_acc[k], whole);
+    }
+    __syncthreads();
+  }
+  op.finish();
+}
+
+// Launch the four LBS kernels on stream s: count (with or without the
+// prologue), scan, partition, expand.  `grid_items` = host-known frontier
+// size (for grid sizing; kernels read the device-side size).  `max_path`
+// bounds items + edges for the partition grid.
+struct LbsBuffers {
+  int32_t *scan_local;
+  int32_t *partials;
+  LbsInfo *info;
+  int32_t *parts;
+  unsigned long long *edge_counter;
+  int expand_grid;
+};
+
+template <class Op, class Timer>
+inline void lbs_launch(const Op &op, bool prologue, const int32_t *frontier, const int32_t *fsize, int64_t nf_host,
+                       int64_t max_path, const LbsBuffers &b, int32_t *out_queue, int32_t *out_size, hipStream_t s,
+                       Timer &&timer) {
+  timer(0);
+  if (prologue)
+    hipLaunchKernelGGL((k_lbs_count<Op, true>), dim3(grid_for(nf_host, kCountTile)), dim3(kCountBlock), 0, s, op,
+                       frontier, fsize, b.scan_local, b.partials);
+  else
+    hipLaunchKernelGGL((k_lbs_count<Op, false>), dim3(grid_for(nf_host, kCountTile)), dim3(kCountBlock), 0, s, op,
+                       frontier, fsize, b.scan_local, b.partials);
+  timer(1);
+  hipLaunchKernelGGL(k_lbs_scan<kScanBlock>, dim3(1), dim3(kScanBlock), 0, s, fsize, b.partials, b.info,
+                     b.edge_counter);
+  timer(2);
+  hipLaunchKernelGGL(k_lbs_partition<0>, dim3(grid_for(max_path / kLbsTile + 2, 256, 4096)), dim3(256), 0, s, fsize,
+                     b.scan_local, b.partials, b.info, b.parts);
+  timer(3);
+  hipLaunchKernelGGL(k_lbs_expand<Op>, dim3(b.expand_grid), dim3(kExpandBlock), 0, s, op, frontier, fsize,
+                     b.scan_local, b.partials, b.info, b.parts, out_queue, out_size);
+  timer(4);
 }
 
 }  // namespace GASengine

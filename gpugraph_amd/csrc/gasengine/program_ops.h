@@ -136,7 +136,9 @@ struct ProgramOp {
     return (s.in_edges ? s.g.column_offsets : s.g.row_offsets)[v];
   }
 
-  __device__ __forceinline__ bool edge(int32_t v, int32_t slot, int32_t &out) {
+  __device__ __forceinline__ int32_t item_value(int32_t) { return 0; }
+
+  __device__ __forceinline__ bool edge(int32_t v, int32_t, int32_t slot, int32_t &out) {
     int32_t nb, eid;
     if (s.in_edges) {
       nb = s.g.row_indices[slot];

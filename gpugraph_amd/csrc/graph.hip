@@ -82,6 +82,59 @@ void finish_profile(gg_graph *g) {
   if (g->prof.enabled) g->prof.flush();
 }
 
+int pull_partition(gg_graph *g, const int32_t **parts) {
+  if (!g->pull_parts_ready) {
+    GG_TRY(g->pull_parts.ensure(4 * GASengine::lbs_partition_entries(g->n, g->m)));
+    const int64_t tiles = ((int64_t)g->n + g->m) / GASengine::kLbsTile + 2;
+    hipLaunchKernelGGL(GASengine::k_static_partition<0>, dim3(GASengine::grid_for(tiles, 256, 4096)), dim3(256), 0,
+                       g->stream, g->dg.column_offsets, g->n, g->m, g->pull_parts.as<int32_t>());
+    GG_HIP(hipGetLastError());
+    g->pull_parts_ready = true;
+  }
+  *parts = g->pull_parts.as<int32_t>();
+  return GG_OK;
+}
+
+__global__ void k_permute4(const uint32_t *__restrict__ in, const int32_t *__restrict__ idx, uint32_t *__restrict__ out,
+                           int64_t m) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x)
+    out[k] = in[idx[k]];
+}
+
+int csc_values(gg_graph *g, const void **vals) {
+  if (!g->directed || !g->dg.csc_edge_ids) {
+    *vals = g->val.p;
+    return GG_OK;
+  }
+  if (g->w_csc_version != g->values_version) {
+    GG_TRY(g->w_csc.ensure(4 * (size_t)std::max<int64_t>(g->m, 1)));
+    hipLaunchKernelGGL(k_permute4, dim3(GASengine::grid_for(g->m, 256, 16384)), dim3(256), 0, g->stream,
+                       g->val.as<uint32_t>(), g->dg.csc_edge_ids, g->w_csc.as<uint32_t>(), g->m);
+    GG_HIP(hipGetLastError());
+    g->w_csc_version = g->values_version;
+  }
+  *vals = g->w_csc.p;
+  return GG_OK;
+}
+
+int lbs_buffers(gg_graph *g, GASengine::LbsBuffers *b) {
+  const size_t n = (size_t)g->n;
+  GG_TRY(g->frontier[0].ensure(4 * n));
+  GG_TRY(g->frontier[1].ensure(4 * n));
+  GG_TRY(g->scan_local.ensure(4 * n));
+  GG_TRY(g->partials.ensure(4 * (n / GASengine::kCountTile + 2)));
+  GG_TRY(g->parts.ensure(4 * GASengine::lbs_partition_entries(g->n, g->m)));
+  GG_TRY(g->info.ensure(sizeof(GASengine::LbsInfo)));
+  GG_TRY(g->counters.ensure(64 * sizeof(unsigned long long)));
+  b->scan_local = g->scan_local.as<int32_t>();
+  b->partials = g->partials.as<int32_t>();
+  b->info = g->info.as<GASengine::LbsInfo>();
+  b->parts = g->parts.as<int32_t>();
+  b->edge_counter = g->counters.as<unsigned long long>() + 2;
+  b->expand_grid = 2048;
+  return GG_OK;
+}
+
 // ------------------------------------------------------------- profiler --
 EventProfiler::~EventProfiler() { reset(); }
 
@@ -160,6 +213,7 @@ int gg_default_options(gg_options *o) {
   o->bfs_beta = 24;
   o->lbs_tile = 2048;
   o->sssp_delta = 0;
+  o->pull_alpha = 8;
   return GG_OK;
 }
 
@@ -287,6 +341,7 @@ int gg_graph_set_options(gg_graph *g, const gg_options *o) {
   GG_CHECK_ARG(o->bfs_alpha >= 1 && o->bfs_beta >= 1, "bfs_alpha/bfs_beta must be >= 1");
   GG_CHECK_ARG(o->lbs_tile == 2048, "lbs_tile: only 2048 is compiled in");
   GG_CHECK_ARG(o->sssp_delta >= 0, "sssp_delta must be >= 0");
+  GG_CHECK_ARG(o->pull_alpha >= 0, "pull_alpha must be >= 0");
   g->opt = *o;
   return GG_OK;
 }
@@ -307,6 +362,7 @@ int gg_graph_update_values(gg_graph *g, const void *values, uint32_t flags) {
                           (flags & GG_OUT_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, g->stream));
   GG_HIP(hipStreamSynchronize(g->stream));
   g->pr_cache_valid = false;
+  g->values_version++;
   return GG_OK;
 }
 

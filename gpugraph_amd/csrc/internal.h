@@ -99,6 +99,23 @@ struct TimedRegion {
   ~TimedRegion() { p.end(name); }
 };
 
+// Phase callback for GASengine::lbs_launch: times the four LBS kernels as
+// "<prefix>.lbs_count", ".lbs_scan", ".lbs_partition", "<prefix>.<expand_name>".
+struct LbsPhaseTimer {
+  EventProfiler &p;
+  std::string names[4];
+  LbsPhaseTimer(EventProfiler &p_, const char *prefix, const char *expand_name) : p(p_) {
+    names[0] = std::string(prefix) + ".lbs_count";
+    names[1] = std::string(prefix) + ".lbs_scan";
+    names[2] = std::string(prefix) + ".lbs_partition";
+    names[3] = std::string(prefix) + "." + expand_name;
+  }
+  void operator()(int phase) {
+    if (phase > 0) p.end(names[phase - 1].c_str());
+    if (phase < 4) p.begin(names[phase].c_str());
+  }
+};
+
 }  // namespace gg
 
 struct gg_graph {
@@ -116,11 +133,18 @@ struct gg_graph {
   GASengine::DeviceGraph dg;  // view of the buffers above
 
   // Shared scratch for the specialised paths (BFS, CC hooking, PageRank).
-  gg::DevBuf frontier[2], bits[4], vtx[4], scan_local, partials, info, counters;
+  gg::DevBuf frontier[2], bits[4], vtx[4], scan_local, partials, parts, info, counters;
   gg::DevBuf pr[8];    // PageRank vectors / CSC-ordered transition weights
   gg::DevBuf batch[8]; // batched-instance state
   int32_t *h_pinned = nullptr;  // pinned readback area (64 x int64)
   bool pr_cache_valid = false;
+
+  // Graph-derived caches: the static merge-path partition of the CSC (pull
+  // steps over all vertices) and the edge values permuted into CSC order
+  // (directed graphs; undirected graphs read the CSR values directly).
+  gg::DevBuf pull_parts, w_csc;
+  bool pull_parts_ready = false;
+  int64_t values_version = 0, w_csc_version = -1;
 
   // Generic-program state kept across runs (one per program type).
   std::shared_ptr<void> prog_state[4];
@@ -142,5 +166,13 @@ int copy_out(gg_graph *g, void *dst, const void *src, size_t bytes, uint32_t fla
 int reach_stats(gg_graph *g, const int32_t *result, int32_t sentinel, gg_stats *st);
 // Finish a run: resolve profiler events.
 void finish_profile(gg_graph *g);
+// Frontier queues, scan/partition buffers and counters of the specialised
+// paths; fills *b for GASengine::lbs_launch (edge counter = counters[2]).
+int lbs_buffers(gg_graph *g, GASengine::LbsBuffers *b);
+// Static merge-path partition of (vertices, CSC in-edges) for k_pull_reduce.
+int pull_partition(gg_graph *g, const int32_t **parts);
+// Edge values in CSC order (4-byte values): the CSR array itself for
+// undirected graphs, a cached permuted copy for directed ones.
+int csc_values(gg_graph *g, const void **vals);
 
 }  // namespace gg

@@ -1,0 +1,316 @@
+// SSSP by bucketed, direction-optimising relaxation (default path of gg_sssp).
+//
+// Output contract of the reference's SSSP program (Algorithms/SSSP/sssp.h;
+// CPUSSSP sssp.cu:126-183): dist[src] = 0, unreached = 100000000, int32
+// weights, a path length >= 100000000 never relaxes (CPUSSSP compares a
+// 64-bit sum against the sentinel).  Shortest-path distances are the unique
+// fixed point, so any schedule of monotone relaxations that runs to
+// quiescence reproduces them bit for bit; the generic GAS path
+// (GG_SSSP_GAS) keeps the reference's own schedule.
+//
+// State: dist[n], dirty[n] (byte: distance dropped since the vertex was last
+// expanded), frontier queue + frontier bitmap.  Per iteration:
+//   collect  every dirty vertex with dist < bound joins the frontier (queue
+//            and bitmap, dirty cleared); counts the frontier's out-edges and
+//            the smallest distance still dirty.  Empty frontier with dirty
+//            vertices left: bound jumps to the bucket of that distance.
+//   push     frontier edges <= m / pull_alpha: LBS expand over the queue,
+//            nd = dist[v] + w, plain pre-check then atomicMin(dist[nb]); an
+//            improvement stores dirty[nb] = 1 (no atomic, no enqueue).
+//   pull     otherwise: merge-path pass over every vertex's in-edges,
+//            min over frontier in-neighbours of dist[u] + w reduced in LDS,
+//            written with a plain store when the tile holds all of the
+//            vertex's in-edges (atomicMin at tile seams).
+//
+// Byte model: collect n + n/8 + 4(dirty) + 4F; push 16F + 12E + 8N;
+// pull 4n + 4m + n/8 + 8E_F + 8N (E_F = out-edges of the frontier).
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstring>
+
+#include "internal.h"
+
+namespace gg {
+namespace {
+
+using namespace GASengine;
+constexpr int32_t kInf = 100000000;
+
+// counters: [2] LBS edge counter, [4] improvements, [5] newly reached,
+// [6] out-degree sum of newly reached, [16] dirty left, [17] min dist left,
+// [18] out-degree sum of the frontier.
+struct Tally {
+  unsigned long long improved = 0, reached = 0, mf = 0;
+  __device__ __forceinline__ void add(int32_t old, const int32_t *ro, int32_t v) {
+    improved++;
+    if (old == kInf) {
+      reached++;
+      mf += (unsigned long long)(ro[v + 1] - ro[v]);
+    }
+  }
+  __device__ __forceinline__ void flush(unsigned long long *counters) {
+    unsigned long long a = wave_sum(improved), b = wave_sum(reached), c = wave_sum(mf);
+    if (lane_id() == 0) {
+      if (a) atomicAdd(&counters[4], a);
+      if (b) atomicAdd(&counters[5], b);
+      if (c) atomicAdd(&counters[6], c);
+    }
+  }
+};
+
+struct PushOp {
+  const int32_t *ro;
+  const int32_t *col;
+  const int32_t *w;
+  int32_t *dist;
+  uint8_t *dirty;
+  unsigned long long *counters;
+  Tally tally;
+
+  __device__ __forceinline__ void prologue(int32_t) {}
+  __device__ __forceinline__ int32_t degree(int32_t v) { return ro[v + 1] - ro[v]; }
+  __device__ __forceinline__ int32_t row_begin(int32_t v) { return ro[v]; }
+  __device__ __forceinline__ int32_t item_value(int32_t v) { return dist[v]; }
+  __device__ __forceinline__ bool edge(int32_t, int32_t dv, int32_t slot, int32_t &) {
+    const int32_t nb = col[slot];
+    const int64_t nd64 = (int64_t)dv + w[slot];
+    const int32_t cur = __atomic_load_n(&dist[nb], __ATOMIC_RELAXED);
+    if (nd64 >= cur) return false;
+    const int32_t nd = (int32_t)nd64;
+    const int32_t old = atomicMin(&dist[nb], nd);
+    if (nd >= old) return false;
+    tally.add(old, ro, nb);
+    dirty[nb] = 1;
+    return false;
+  }
+  __device__ __forceinline__ void finish() { tally.flush(counters); }
+};
+
+__device__ __forceinline__ bool fbit(const uint32_t *__restrict__ b, int32_t u) { return (b[u >> 5] >> (u & 31)) & 1u; }
+
+struct PullOp {
+  const int32_t *ro;
+  const int32_t *ri;
+  const int32_t *wc;
+  const uint32_t *front;
+  int32_t *dist;
+  uint8_t *dirty;
+  unsigned long long *counters;
+  Tally tally;
+
+  __device__ __forceinline__ int32_t identity() { return INT_MAX; }
+  __device__ __forceinline__ int32_t edge(int32_t, int32_t slot) {
+    const int32_t u = ri[slot];
+    if (!fbit(front, u)) return INT_MAX;
+    const int64_t c = (int64_t)dist[u] + wc[slot];
+    return c < kInf ? (int32_t)c : INT_MAX;
+  }
+  __device__ __forceinline__ void combine(int32_t *acc, int32_t val) {
+    if (val < *acc) atomicMin(acc, val);
+  }
+  __device__ __forceinline__ void item_done(int32_t v, int32_t acc, bool whole) {
+    if (acc == INT_MAX) return;
+    const int32_t cur = __atomic_load_n(&dist[v], __ATOMIC_RELAXED);
+    if (acc >= cur) return;
+    int32_t old = cur;
+    if (whole) {
+      dist[v] = acc;
+    } else {
+      old = atomicMin(&dist[v], acc);
+      if (acc >= old) return;
+    }
+    tally.add(old, ro, v);
+    dirty[v] = 1;
+  }
+  __device__ __forceinline__ void finish() { tally.flush(counters); }
+};
+
+__global__ void k_sssp_init(int32_t *dist, uint8_t *dirty, int32_t src) {
+  dist[src] = 0;
+  dirty[src] = 1;
+}
+
+// One wave per 64 consecutive vertices (it owns their two bitmap words).
+__global__ void __launch_bounds__(256)
+k_sssp_collect(uint8_t *__restrict__ dirty, const int32_t *__restrict__ dist, const int32_t *__restrict__ ro,
+               int32_t n, int32_t bound, int32_t *__restrict__ queue, int32_t *__restrict__ qsize,
+               uint32_t *__restrict__ front, unsigned long long *__restrict__ counters) {
+  const int32_t words = (n + 31) >> 5;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int lane = lane_id();
+  unsigned long long remain = 0, degsum = 0;
+  int32_t dmin = INT_MAX;
+  for (int64_t p = (((int64_t)blockIdx.x * blockDim.x) + threadIdx.x) >> 6; p * 64 < n; p += nwaves) {
+    const int32_t v = (int32_t)(p * 64 + lane);
+    bool sel = false;
+    if (v < n && dirty[v]) {
+      const int32_t d = dist[v];
+      if (d < bound) {
+        sel = true;
+        dirty[v] = 0;
+        degsum += (unsigned long long)(ro[v + 1] - ro[v]);
+      } else {
+        remain++;
+        dmin = min(dmin, d);
+      }
+    }
+    const uint64_t mask = ballot(sel);
+    if (lane < 2) {
+      const int32_t ww = (int32_t)(2 * p + lane);
+      if (ww < words) front[ww] = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
+    }
+    wave_append(sel, v, queue, qsize);
+  }
+  remain = wave_sum(remain);
+  degsum = wave_sum(degsum);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) dmin = min(dmin, __shfl_xor(dmin, off));
+  if (lane == 0) {
+    if (remain) atomicAdd(&counters[16], remain);
+    if (degsum) atomicAdd(&counters[18], degsum);
+    if (dmin != INT_MAX) atomicMin(&counters[17], (unsigned long long)dmin);
+  }
+}
+
+}  // namespace
+
+int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, gg_stats *stats) {
+  GG_TRY(activate(g));
+  const int32_t n = g->n;
+  const int64_t m = g->m;
+  const int32_t words = (n + 31) / 32;
+  hipStream_t s = g->stream;
+  LbsBuffers lb;
+  GG_TRY(lbs_buffers(g, &lb));
+  GG_TRY(g->bits[3].ensure(4 * (size_t)words));
+  GG_TRY(g->vtx[3].ensure((size_t)n));
+  const int32_t *pparts = nullptr;
+  const void *wcv = nullptr;
+  const int64_t pull_alpha = g->opt.pull_alpha;
+  if (pull_alpha > 0) {
+    GG_TRY(pull_partition(g, &pparts));
+    GG_TRY(csc_values(g, &wcv));
+  }
+  int32_t *dist;
+  if (flags & GG_OUT_DEVICE) {
+    GG_CHECK_ARG(dist_out, "dist is NULL");
+    dist = dist_out;
+  } else {
+    GG_TRY(g->vtx[2].ensure(4 * (size_t)n));
+    dist = g->vtx[2].as<int32_t>();
+  }
+  uint32_t *front = g->bits[3].as<uint32_t>();
+  uint8_t *dirty = g->vtx[3].as<uint8_t>();
+  int32_t *queue = g->frontier[0].as<int32_t>();
+  unsigned long long *ctr = g->counters.as<unsigned long long>();
+  int32_t *qsize = reinterpret_cast<int32_t *>(ctr + 8);
+  const int64_t delta = g->opt.sssp_delta > 0 ? g->opt.sssp_delta : 32;
+  g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
+  LbsPhaseTimer lbs_timer(g->prof, "sssp", "push");
+
+  hipEvent_t e0, e1;
+  GG_HIP(hipEventCreate(&e0));
+  GG_HIP(hipEventCreate(&e1));
+  GG_HIP(hipEventRecord(e0, s));
+  {
+    TimedRegion t(g->prof, "sssp.init");
+    GG_HIP(fill_async<int32_t>(dist, kInf, n, s));
+    GG_HIP(fill_async<uint8_t>(dirty, (uint8_t)0, n, s));
+    GG_HIP(fill_async<unsigned long long>(ctr, 0ull, 24, s));
+    hipLaunchKernelGGL(k_sssp_init, dim3(1), dim3(1), 0, s, dist, dirty, src);
+    GG_HIP(hipGetLastError());
+  }
+  g->prof.add_bytes("sssp.init", 5.0 * n);
+  int64_t bound = delta;
+  int iterations = 0, pulls = 0;
+  int64_t edges = 0, prev_edges = 0, prev_improved = 0;
+  double bytes = 5.0 * n;
+  int64_t reached_edges0;
+  {
+    int32_t h[2];
+    GG_HIP(hipMemcpyAsync(h, g->dg.row_offsets + src, 8, hipMemcpyDeviceToHost, s));
+    GG_HIP(hipStreamSynchronize(s));
+    reached_edges0 = h[1] - h[0];
+  }
+  const int collect_grid = grid_for((int64_t)words * 32, 256, 8192);
+  while (iterations < g->opt.iter_num) {
+    GG_HIP(fill_async<int32_t>(qsize, 0, 1, s));
+    GG_HIP(fill_async<unsigned long long>(ctr + 16, 0ull, 3, s));
+    GG_HIP(fill_async<unsigned long long>(ctr + 17, (unsigned long long)INT_MAX, 1, s));
+    {
+      TimedRegion t(g->prof, "sssp.collect");
+      hipLaunchKernelGGL(k_sssp_collect, dim3(collect_grid), dim3(256), 0, s, dirty, dist, g->dg.row_offsets, n,
+                         (int32_t)std::min<int64_t>(bound, kInf), queue, qsize, front, ctr);
+      GG_HIP(hipGetLastError());
+    }
+    GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 20 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    GG_HIP(hipStreamSynchronize(s));
+    const int64_t nf = reinterpret_cast<int32_t *>(g->h_pinned + 16)[0];
+    const int64_t remain = g->h64()[16], dmin = g->h64()[17], degsum = g->h64()[18];
+    const int64_t improved_so_far = g->h64()[4];
+    const double cb = n + n / 8.0 + 4.0 * (nf + remain) + 4.0 * nf;
+    g->prof.add_bytes("sssp.collect", cb);
+    bytes += cb;
+    if (nf == 0) {
+      if (remain == 0) break;
+      bound = std::min<int64_t>((dmin / delta + 1) * delta, kInf);
+      continue;
+    }
+    const int64_t N = improved_so_far - prev_improved;  // improvements of the previous step
+    (void)N;
+    prev_improved = improved_so_far;
+    if (pull_alpha > 0 && degsum > m / pull_alpha) {
+      PullOp op{g->dg.row_offsets, g->dg.row_indices, static_cast<const int32_t *>(wcv), front, dist, dirty, ctr, {}};
+      {
+        TimedRegion t(g->prof, "sssp.pull");
+        hipLaunchKernelGGL((k_pull_reduce<PullOp, int32_t>), dim3(2048), dim3(kExpandBlock), 0, s, op,
+                           g->dg.column_offsets, n, m, pparts);
+        GG_HIP(hipGetLastError());
+      }
+      const double pb = 4.0 * n + 4.0 * m + n / 8.0 + 8.0 * degsum;
+      g->prof.add_bytes("sssp.pull", pb);
+      bytes += pb;
+      edges += m;
+      pulls++;
+    } else {
+      PushOp op{g->dg.row_offsets, g->dg.column_indices, static_cast<const int32_t *>(g->dg.edge_values), dist, dirty,
+                ctr, {}};
+      // PushOp never enqueues: the output queue stays empty.
+      lbs_launch(op, false, queue, qsize, nf, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(), qsize + 1, s,
+                 lbs_timer);
+      GG_HIP(hipGetLastError());
+      const double pb = 16.0 * nf + 12.0 * degsum;
+      g->prof.add_bytes("sssp.lbs_count", 16.0 * nf);
+      g->prof.add_bytes("sssp.push", 12.0 * degsum);
+      bytes += pb;
+      edges += degsum;
+    }
+    iterations++;
+  }
+  GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  GG_HIP(hipEventRecord(e1, s));
+  GG_HIP(hipEventSynchronize(e1));
+  const int64_t improved = g->h64()[4], reached = 1 + g->h64()[5], reached_edges = reached_edges0 + g->h64()[6];
+  bytes += 8.0 * improved;  // dist + dirty writes of every improvement
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)prev_edges;
+  finish_profile(g);
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->iterations = iterations;
+    stats->pull_iterations = pulls;
+    stats->reached_vertices = reached;
+    stats->reached_edges = reached_edges;
+    stats->edges_inspected = edges;
+    stats->alg_bytes = bytes;
+    stats->gpu_ms = ms;
+  }
+  if (flags & GG_OUT_DEVICE) return GG_OK;
+  return copy_out(g, dist_out, dist, 4 * (size_t)n, flags);
+}
+
+}  // namespace gg

@@ -73,7 +73,8 @@ typedef struct gg_options {
   int32_t bfs_alpha;     /* top-down -> bottom-up when m_f > m_u / alpha */
   int32_t bfs_beta;      /* bottom-up -> top-down when n_f < n / beta */
   int32_t lbs_tile;      /* merge-path positions per LBS workgroup tile (multiple of 256) */
-  int32_t sssp_delta;    /* near/far split width for SSSP (0 = plain frontier Bellman-Ford) */
+  int32_t sssp_delta;    /* SSSP bucket width (0 = auto: 32); INT32_MAX = one bucket */
+  int32_t pull_alpha;    /* SSSP: pull step when frontier edges > m / pull_alpha (0 = never) */
 } gg_options;
 
 /* Per-run statistics (the reference prints these with INSTRUMENT/verbose). */

@@ -56,8 +56,27 @@ def test_rmat_bfs_sssp(gpu_available, undirected):
             for push in (False, True):
                 d, st = g.bfs(src, push_only=push)
                 np.testing.assert_array_equal(d, exp)
-            dist, _ = g.sssp(src)
-            np.testing.assert_array_equal(dist, oracle.sssp(c.row_offsets, c.col_indices, c.values, src))
+            exp_d = oracle.sssp(c.row_offsets, c.col_indices, c.values, src)
+            for fl in (0, _abi.GG_SSSP_GAS):
+                dist, _ = g.sssp(src, flags=fl)
+                np.testing.assert_array_equal(dist, exp_d)
+
+
+@pytest.mark.parametrize("delta", [1, 7, 32, 200, 2**31 - 1])
+def test_sssp_near_far_bucket_widths(gpu_available, delta):
+    c = gg.gen_rmat(14, 16, undirected=False, seed=26).with_weights("i32", 1, 64, symmetric=False, seed=27)
+    src = workloads.max_degree_vertex(c)
+    exp = oracle.sssp(c.row_offsets, c.col_indices, c.values, src)
+    with gg.Graph(c, directed=True) as g:
+        # pull_alpha 0: push only; 8: default switch; 2^20: pull whenever the frontier has edges
+        for pull_alpha in (0, 8, 1 << 20):
+            g.set_options(sssp_delta=delta, pull_alpha=pull_alpha)
+            dist, st = g.sssp(src)
+            np.testing.assert_array_equal(dist, exp)
+            assert st["reached_vertices"] == int((exp < INF).sum())
+            assert st["reached_edges"] == int(c.degrees()[exp < INF].sum())
+            if pull_alpha == 1 << 20:
+                assert st["pull_iterations"] >= 1
 
 
 @pytest.mark.parametrize("threshold", [0, 1 << 30])
@@ -70,7 +89,7 @@ def test_expand_strategies_agree(gpu_available, threshold):
             g.set_options(threshold=threshold, wave_degree=wave, block_degree=block)
             d, _ = g.bfs(src, push_only=True)
             np.testing.assert_array_equal(d, oracle.bfs(c.row_offsets, c.col_indices, src))
-            dist, _ = g.sssp(src)
+            dist, _ = g.sssp(src, flags=_abi.GG_SSSP_GAS)
             np.testing.assert_array_equal(dist, oracle.sssp(c.row_offsets, c.col_indices, c.values, src))
             lab, _ = g.cc()
             np.testing.assert_array_equal(lab, oracle.cc(c.row_offsets, c.col_indices)[0])
