@@ -34,6 +34,7 @@ using namespace GASengine;
 // counters: [0] next frontier size, [1] out-degree sum of the new frontier
 // (m_f), [2] edges examined, [3] unvisited vertices scanned.
 struct BfsPushOp {
+  static constexpr bool kItemValues = false;
   const int32_t *ro;
   const int32_t *col;
   int32_t *depth;
@@ -54,10 +55,7 @@ struct BfsPushOp {
     out = nb;
     return true;
   }
-  __device__ __forceinline__ void finish() {
-    unsigned long long t = wave_sum(mf);
-    if (lane_id() == 0 && t) atomicAdd(&counters[1], t);
-  }
+  __device__ __forceinline__ void finish() { block_counter_add<kExpandBlock>(&counters[1], mf); }
 };
 
 __global__ void k_bfs_init(int32_t *depth, uint32_t *visited, int32_t *queue, int32_t *qsize, const int32_t *ro,
@@ -142,16 +140,10 @@ k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, 
     }
     if (lane == 0) found_cnt += (unsigned long long)__popcll(mask);
   }
-  found_cnt = wave_sum(found_cnt);
-  mf = wave_sum(mf);
-  examined = wave_sum(examined);
-  scanned = wave_sum(scanned);
-  if (lane == 0) {
-    if (found_cnt) atomicAdd(&counters[0], found_cnt);
-    if (mf) atomicAdd(&counters[1], mf);
-    if (examined) atomicAdd(&counters[2], examined);
-    if (scanned) atomicAdd(&counters[3], scanned);
-  }
+  block_counter_add<256>(&counters[0], found_cnt);
+  block_counter_add<256>(&counters[1], mf);
+  block_counter_add<256>(&counters[2], examined);
+  block_counter_add<256>(&counters[3], scanned);
 }
 
 __global__ void k_queue_to_bits(const int32_t *__restrict__ queue, const int32_t *__restrict__ qsize,
@@ -262,7 +254,7 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
     if (bottom_up) {
       {
         TimedRegion t(g->prof, "bfs.bottom_up");
-        hipLaunchKernelGGL(k_bfs_bottom_up, dim3(grid_for((int64_t)words * 32, 256, 8192)), dim3(256), 0, s,
+        hipLaunchKernelGGL(k_bfs_bottom_up, dim3(grid_for((int64_t)words * 32, 256, 2048)), dim3(256), 0, s,
                            g->dg.column_offsets, g->dg.row_indices, g->dg.row_offsets, depth, visited, fbits[cur],
                            fbits[nxt], n, levels + 1, g->opt.wave_degree, ctr);
         GG_HIP(hipGetLastError());

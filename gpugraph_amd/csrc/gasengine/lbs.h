@@ -45,7 +45,7 @@ constexpr int kCountIpt = 4;
 constexpr int kCountTile = kCountBlock * kCountIpt;  // items per scan partial
 constexpr int kScanBlock = 1024;
 constexpr int kExpandBlock = 256;
-constexpr int kExpandVt = 8;
+constexpr int kExpandVt = 4;  // 1024-position tiles: ~21 KB LDS, 7 workgroups per CU
 constexpr int kLbsTile = kExpandBlock * kExpandVt;  // merge-path positions per tile
 
 // Device-side counters of one expansion (written by the kernels).
@@ -160,8 +160,10 @@ k_lbs_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__restr
   __shared__ int32_t s_offs[TILE + 1];
   __shared__ int32_t s_vert[TILE + 1];
   __shared__ int32_t s_begin[TILE + 1];
-  __shared__ int32_t s_val[TILE + 1];
+  __shared__ int32_t s_val[Op::kItemValues ? TILE + 1 : 1];
   __shared__ int32_t s_slot[TILE];
+  __shared__ int32_t s_out[TILE];  // this tile's outputs, flushed with one global atomic
+  __shared__ int32_t s_nout, s_obase;
   const int64_t path = info->path;
   for (int64_t t = blockIdx.x; t * TILE < path; t += gridDim.x) {
     const int64_t d0 = t * TILE;
@@ -176,14 +178,15 @@ k_lbs_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__restr
         s_vert[k] = v;
         s_offs[k] = lbs_offset(scan_local, partials, i);
         s_begin[k] = op.row_begin(v);
-        s_val[k] = op.item_value(v);
+        if (Op::kItemValues) s_val[k] = op.item_value(v);
       } else {
         s_vert[k] = -1;
         s_offs[k] = 0;
         s_begin[k] = 0;
-        s_val[k] = 0;
+        if (Op::kItemValues) s_val[k] = 0;
       }
     }
+    if (threadIdx.x == 0) s_nout = 0;
     __syncthreads();
     {
       const int32_t dl = (int32_t)threadIdx.x * VT;
@@ -216,11 +219,19 @@ k_lbs_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__restr
       int32_t out = 0;
       if (k < ne) {
         const int32_t s = s_slot[k];
-        keep = op.edge(s_vert[s], s_val[s], s_begin[s] + (j0 + k - s_offs[s]), out);
+        keep = op.edge(s_vert[s], Op::kItemValues ? s_val[s] : 0, s_begin[s] + (j0 + k - s_offs[s]), out);
       }
-      wave_append(keep, out, out_queue, out_size);
+      wave_append(keep, out, s_out, &s_nout);  // LDS atomics, one per wave
     }
     __syncthreads();
+    const int32_t nout = s_nout;
+    if (nout) {
+      if (threadIdx.x == 0) s_obase = atomicAdd(out_size, nout);
+      __syncthreads();
+      const int32_t ob = s_obase;
+      for (int32_t k = threadIdx.x; k < nout; k += kExpandBlock) out_queue[ob + k] = s_out[k];
+      __syncthreads();
+    }
   }
   op.finish();
 }

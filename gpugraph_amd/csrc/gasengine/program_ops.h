@@ -110,6 +110,7 @@ struct GasState {
 // contract_atomic/cta.cuh:529-629).
 template <class Program>
 struct ProgramOp {
+  static constexpr bool kItemValues = false;
   GasState<Program> s;
 
   __device__ __forceinline__ void prologue(int32_t v) {

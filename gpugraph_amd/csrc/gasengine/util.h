@@ -94,6 +94,31 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v
   return t;
 }
 
+// Add a per-thread value to a global counter with one atomic per workgroup
+// (same-address atomics from every wave serialise at the memory side).
+// All threads of the workgroup must call it.
+template <int BLOCK>
+__device__ __forceinline__ void block_counter_add(unsigned long long *counter, unsigned long long v) {
+  __shared__ unsigned long long red[BLOCK / kWave];
+  const unsigned long long t = block_sum_u64<BLOCK>(v, red);
+  if (threadIdx.x == 0 && t) atomicAdd(counter, t);
+}
+
+// Workgroup-aggregated queue append: one global atomic per call.  All
+// threads of the workgroup must call it.
+template <int BLOCK>
+__device__ __forceinline__ void block_append(bool keep, int32_t value, int32_t *queue, int32_t *size) {
+  __shared__ int32_t scan[BLOCK / kWave + 1];
+  __shared__ int32_t base;
+  int32_t total;
+  const int32_t off = block_exclusive_scan<BLOCK>(keep ? 1 : 0, scan, &total);
+  if (total == 0) return;
+  if (threadIdx.x == 0) base = atomicAdd(size, total);
+  __syncthreads();
+  if (keep) queue[base + off] = value;
+  __syncthreads();
+}
+
 // ---- bitmaps (bit v of word v>>5) --------------------------------------
 
 __device__ __forceinline__ bool bit_test(const uint32_t *bits, int32_t v) {

@@ -95,6 +95,51 @@ int pull_partition(gg_graph *g, const int32_t **parts) {
   return GG_OK;
 }
 
+// Mirror check of an undirected CSR, run over every (v, slot) with the pull
+// merge path: for slot k of v's row (neighbour u, its j-th occurrence there)
+// the mirror is the j-th occurrence of v in u's row.  flags: 1 = a row is
+// not sorted (no check possible), 2 = an entry has no mirror, 4 = a mirrored
+// pair carries different values.
+struct MirrorOp {
+  const int32_t *ro;
+  const int32_t *col;
+  const uint32_t *val;
+  int32_t *mir;
+  unsigned int *flags;
+
+  __device__ static int32_t lower_bound(const int32_t *a, int32_t lo, int32_t hi, int32_t x) {
+    while (lo < hi) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (a[mid] < x)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    return lo;
+  }
+  __device__ __forceinline__ uint32_t identity() { return 0u; }
+  __device__ __forceinline__ uint32_t edge(int32_t v, int32_t k) {
+    const int32_t u = col[k];
+    const int32_t b = ro[v];
+    if (k > b && col[k - 1] > u) {
+      atomicOr(flags, 1u);
+      return 0u;
+    }
+    const int32_t j = k - lower_bound(col, b, k, u);  // occurrence index of u in v's row
+    const int32_t pos = lower_bound(col, ro[u], ro[u + 1], v) + j;
+    if (pos >= ro[u + 1] || col[pos] != v) {
+      atomicOr(flags, 2u);
+      return 0u;
+    }
+    mir[k] = pos;
+    if (val && val[pos] != val[k]) atomicOr(flags, 4u);
+    return 0u;
+  }
+  __device__ __forceinline__ void combine(uint32_t *, uint32_t) {}
+  __device__ __forceinline__ void item_done(int32_t, uint32_t, bool) {}
+  __device__ __forceinline__ void finish() {}
+};
+
 __global__ void k_permute4(const uint32_t *__restrict__ in, const int32_t *__restrict__ idx, uint32_t *__restrict__ out,
                            int64_t m) {
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x)
@@ -102,18 +147,53 @@ __global__ void k_permute4(const uint32_t *__restrict__ in, const int32_t *__res
 }
 
 int csc_values(gg_graph *g, const void **vals) {
-  if (!g->directed || !g->dg.csc_edge_ids) {
+  const int32_t *perm = g->directed ? g->dg.csc_edge_ids : (g->values_asymmetric ? g->eid.as<int32_t>() : nullptr);
+  if (!g->directed && g->values_asymmetric && !g->mirror_ok)
+    return set_error(GG_ERR_UNSUPPORTED, "undirected graph with unsorted rows: in-edge values unknown");
+  if (!perm) {
     *vals = g->val.p;
     return GG_OK;
   }
   if (g->w_csc_version != g->values_version) {
     GG_TRY(g->w_csc.ensure(4 * (size_t)std::max<int64_t>(g->m, 1)));
     hipLaunchKernelGGL(k_permute4, dim3(GASengine::grid_for(g->m, 256, 16384)), dim3(256), 0, g->stream,
-                       g->val.as<uint32_t>(), g->dg.csc_edge_ids, g->w_csc.as<uint32_t>(), g->m);
+                       g->val.as<uint32_t>(), perm, g->w_csc.as<uint32_t>(), g->m);
     GG_HIP(hipGetLastError());
     g->w_csc_version = g->values_version;
   }
   *vals = g->w_csc.p;
+  return GG_OK;
+}
+
+// Undirected graphs: confirm the CSR is symmetric (every u->v has a v->u)
+// and record whether mirrored entries carry equal values; keeps the mirror
+// permutation in g->eid when they do not.  Rows that are not sorted cannot
+// be checked and leave mirror_ok = false.
+int check_undirected(gg_graph *g) {
+  const int32_t *parts = nullptr;
+  GG_TRY(pull_partition(g, &parts));
+  GG_TRY(g->eid.ensure(4 * (size_t)std::max<int64_t>(g->m, 1)));
+  GG_TRY(g->counters.ensure(64 * sizeof(unsigned long long)));
+  unsigned int *flags = reinterpret_cast<unsigned int *>(g->counters.as<unsigned long long>() + 40);
+  GG_HIP(GASengine::fill_async<unsigned int>(flags, 0u, 1, g->stream));
+  MirrorOp op{g->dg.row_offsets, g->dg.column_indices, g->val.as<uint32_t>(), g->eid.as<int32_t>(), flags};
+  hipLaunchKernelGGL((GASengine::k_pull_reduce<MirrorOp, uint32_t>), dim3(2048), dim3(GASengine::kExpandBlock), 0,
+                     g->stream, op, g->dg.row_offsets, g->n, g->m, parts);
+  GG_HIP(hipGetLastError());
+  unsigned int h = 0;
+  GG_HIP(hipMemcpyAsync(&h, flags, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+  GG_HIP(hipStreamSynchronize(g->stream));
+  if (h & 1u) {
+    g->mirror_ok = false;
+    g->values_asymmetric = g->value_dtype != GG_VALUES_NONE;
+    g->eid.reset();
+    return GG_OK;
+  }
+  if (h & 2u)
+    return set_error(GG_ERR_INVALID, "CSR is not symmetric (an edge u->v has no v->u); pass GG_GRAPH_DIRECTED");
+  g->mirror_ok = true;
+  g->values_asymmetric = (h & 4u) != 0;
+  if (!g->values_asymmetric) g->eid.reset();
   return GG_OK;
 }
 
@@ -295,6 +375,7 @@ int gg_graph_create(int32_t n, int64_t m, const int32_t *row_offsets, const int3
     d.column_offsets = d.row_offsets;
     d.row_indices = d.column_indices;
     d.csc_edge_ids = nullptr;
+    if (m) GG_TRY(gg::check_undirected(g));
   }
   *out = guard.release();
   return GG_OK;
@@ -363,6 +444,7 @@ int gg_graph_update_values(gg_graph *g, const void *values, uint32_t flags) {
   GG_HIP(hipStreamSynchronize(g->stream));
   g->pr_cache_valid = false;
   g->values_version++;
+  if (!g->directed && g->m) GG_TRY(gg::check_undirected(g));  // value symmetry may have changed
   return GG_OK;
 }
 

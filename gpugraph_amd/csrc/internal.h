@@ -145,6 +145,10 @@ struct gg_graph {
   gg::DevBuf pull_parts, w_csc;
   bool pull_parts_ready = false;
   int64_t values_version = 0, w_csc_version = -1;
+  // Undirected graphs: rows sorted and symmetry confirmed (mirror_ok); the
+  // mirrored entries' values differ (values_asymmetric; mirror permutation
+  // then kept in eid).
+  bool mirror_ok = false, values_asymmetric = false;
 
   // Generic-program state kept across runs (one per program type).
   std::shared_ptr<void> prog_state[4];
@@ -174,5 +178,8 @@ int pull_partition(gg_graph *g, const int32_t **parts);
 // Edge values in CSC order (4-byte values): the CSR array itself for
 // undirected graphs, a cached permuted copy for directed ones.
 int csc_values(gg_graph *g, const void **vals);
+// Symmetry / value-symmetry check of an undirected graph (sets mirror_ok,
+// values_asymmetric); GG_ERR_INVALID when the CSR is provably not symmetric.
+int check_undirected(gg_graph *g);
 
 }  // namespace gg

@@ -49,17 +49,16 @@ struct Tally {
       mf += (unsigned long long)(ro[v + 1] - ro[v]);
     }
   }
+  // All threads of the workgroup must call it.
   __device__ __forceinline__ void flush(unsigned long long *counters) {
-    unsigned long long a = wave_sum(improved), b = wave_sum(reached), c = wave_sum(mf);
-    if (lane_id() == 0) {
-      if (a) atomicAdd(&counters[4], a);
-      if (b) atomicAdd(&counters[5], b);
-      if (c) atomicAdd(&counters[6], c);
-    }
+    block_counter_add<kExpandBlock>(&counters[4], improved);
+    block_counter_add<kExpandBlock>(&counters[5], reached);
+    block_counter_add<kExpandBlock>(&counters[6], mf);
   }
 };
 
 struct PushOp {
+  static constexpr bool kItemValues = true;
   const int32_t *ro;
   const int32_t *col;
   const int32_t *w;
@@ -131,45 +130,68 @@ __global__ void k_sssp_init(int32_t *dist, uint8_t *dirty, int32_t src) {
   dirty[src] = 1;
 }
 
-// One wave per 64 consecutive vertices (it owns their two bitmap words).
-__global__ void __launch_bounds__(256)
+// A workgroup takes 1024 consecutive vertices per step, four coalesced passes
+// of 256; each wave owns the two bitmap words of its 64 vertices per pass.
+// One global atomic per step for the queue, one per workgroup per counter.
+constexpr int kCollectBlock = 256;
+constexpr int kCollectStep = 4 * kCollectBlock;
+
+__global__ void __launch_bounds__(kCollectBlock)
 k_sssp_collect(uint8_t *__restrict__ dirty, const int32_t *__restrict__ dist, const int32_t *__restrict__ ro,
                int32_t n, int32_t bound, int32_t *__restrict__ queue, int32_t *__restrict__ qsize,
                uint32_t *__restrict__ front, unsigned long long *__restrict__ counters) {
+  __shared__ int32_t s_scan[kCollectBlock / kWave + 1];
+  __shared__ int32_t s_base;
+  __shared__ int32_t s_min[kCollectBlock / kWave];
   const int32_t words = (n + 31) >> 5;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  const int lane = lane_id();
+  const int lane = lane_id(), wid = threadIdx.x / kWave;
   unsigned long long remain = 0, degsum = 0;
   int32_t dmin = INT_MAX;
-  for (int64_t p = (((int64_t)blockIdx.x * blockDim.x) + threadIdx.x) >> 6; p * 64 < n; p += nwaves) {
-    const int32_t v = (int32_t)(p * 64 + lane);
-    bool sel = false;
-    if (v < n && dirty[v]) {
-      const int32_t d = dist[v];
-      if (d < bound) {
-        sel = true;
-        dirty[v] = 0;
-        degsum += (unsigned long long)(ro[v + 1] - ro[v]);
-      } else {
-        remain++;
-        dmin = min(dmin, d);
+  for (int64_t base = (int64_t)blockIdx.x * kCollectStep; base < n; base += (int64_t)gridDim.x * kCollectStep) {
+    int32_t picked[4];
+    int32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int64_t v = base + k * kCollectBlock + threadIdx.x;
+      bool sel = false;
+      if (v < n && dirty[v]) {
+        const int32_t d = dist[v];
+        if (d < bound) {
+          sel = true;
+          dirty[v] = 0;
+          degsum += (unsigned long long)(ro[v + 1] - ro[v]);
+        } else {
+          remain++;
+          dmin = min(dmin, d);
+        }
       }
+      const uint64_t mask = ballot(sel);
+      if (lane < 2) {
+        const int64_t ww = (base + k * kCollectBlock + wid * kWave) / 32 + lane;
+        if (ww < words) front[ww] = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
+      }
+      if (sel) picked[cnt++] = (int32_t)v;
     }
-    const uint64_t mask = ballot(sel);
-    if (lane < 2) {
-      const int32_t ww = (int32_t)(2 * p + lane);
-      if (ww < words) front[ww] = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
+    int32_t total;
+    const int32_t off = block_exclusive_scan<kCollectBlock>(cnt, s_scan, &total);
+    if (total) {
+      if (threadIdx.x == 0) s_base = atomicAdd(qsize, total);
+      __syncthreads();
+      const int32_t b = s_base;
+      for (int k = 0; k < cnt; k++) queue[b + off + k] = picked[k];
+      __syncthreads();
     }
-    wave_append(sel, v, queue, qsize);
   }
-  remain = wave_sum(remain);
-  degsum = wave_sum(degsum);
+  block_counter_add<kCollectBlock>(&counters[16], remain);
+  block_counter_add<kCollectBlock>(&counters[18], degsum);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) dmin = min(dmin, __shfl_xor(dmin, off));
-  if (lane == 0) {
-    if (remain) atomicAdd(&counters[16], remain);
-    if (degsum) atomicAdd(&counters[18], degsum);
-    if (dmin != INT_MAX) atomicMin(&counters[17], (unsigned long long)dmin);
+  if (lane == 0) s_min[wid] = dmin;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t mn = INT_MAX;
+    for (int w = 0; w < kCollectBlock / kWave; w++) mn = min(mn, s_min[w]);
+    if (mn != INT_MAX) atomicMin(&counters[17], (unsigned long long)mn);
   }
 }
 
@@ -187,10 +209,13 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   GG_TRY(g->vtx[3].ensure((size_t)n));
   const int32_t *pparts = nullptr;
   const void *wcv = nullptr;
-  const int64_t pull_alpha = g->opt.pull_alpha;
+  int64_t pull_alpha = g->opt.pull_alpha;
   if (pull_alpha > 0) {
     GG_TRY(pull_partition(g, &pparts));
-    GG_TRY(csc_values(g, &wcv));
+    if (csc_values(g, &wcv) != GG_OK) {  // in-edge weights unknown: push only
+      clear_error();
+      pull_alpha = 0;
+    }
   }
   int32_t *dist;
   if (flags & GG_OUT_DEVICE) {
@@ -233,7 +258,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     GG_HIP(hipStreamSynchronize(s));
     reached_edges0 = h[1] - h[0];
   }
-  const int collect_grid = grid_for((int64_t)words * 32, 256, 8192);
+  const int collect_grid = grid_for(n, kCollectStep, 2048);
   while (iterations < g->opt.iter_num) {
     GG_HIP(fill_async<int32_t>(qsize, 0, 1, s));
     GG_HIP(fill_async<unsigned long long>(ctr + 16, 0ull, 3, s));

@@ -155,6 +155,37 @@ def test_error_paths(gpu_available):
         with pytest.raises(gg.GGError) as e:
             g.cc(hook=True)
         assert e.value.code == _abi.GG_ERR_UNSUPPORTED
+    # An undirected graph must be symmetric: 0->1 without 1->0 is rejected.
+    with pytest.raises(gg.GGError) as e:
+        gg.Graph(csr_of([0, 1, 1], [1]))
+    assert e.value.code == _abi.GG_ERR_INVALID
+
+
+def test_undirected_asymmetric_weights_and_unsorted_rows(gpu_available):
+    """Mirrored entries with different weights: the pull step must read the
+    in-edge's own weight.  Unsorted rows: symmetry unverifiable, push only."""
+    c = gg.gen_rmat(13, 16, undirected=True, seed=28)
+    rows = np.repeat(np.arange(c.n), c.degrees())
+    w = (1 + (rows * 7 + c.col_indices * 3) % 64).astype(np.int32)  # w(u,v) != w(v,u)
+    src = workloads.max_degree_vertex(c)
+    exp = oracle.sssp(c.row_offsets, c.col_indices, w, src)
+    with gg.Graph(gg.CSR(c.row_offsets, c.col_indices, w)) as g:
+        g.set_options(pull_alpha=1 << 20)
+        dist, st = g.sssp(src)
+        np.testing.assert_array_equal(dist, exp)
+        assert st["pull_iterations"] >= 1
+    perm_col = c.col_indices.copy()
+    perm_w = w.copy()
+    for u in range(c.n):  # reverse every row
+        b, e = c.row_offsets[u], c.row_offsets[u + 1]
+        perm_col[b:e] = perm_col[b:e][::-1]
+        perm_w[b:e] = perm_w[b:e][::-1]
+    with gg.Graph(gg.CSR(c.row_offsets, perm_col, perm_w)) as g:
+        g.set_options(pull_alpha=1 << 20)
+        dist, st = g.sssp(src)
+        np.testing.assert_array_equal(dist, exp)
+        d, _ = g.bfs(src)
+        np.testing.assert_array_equal(d, oracle.bfs(c.row_offsets, c.col_indices, src))
 
 
 @pytest.fixture(scope="module")
