@@ -198,9 +198,9 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
   uint32_t *visited = g->bits[0].as<uint32_t>();
   uint32_t *fbits[2] = {g->bits[1].as<uint32_t>(), g->bits[2].as<uint32_t>()};
   int32_t *queue[2] = {g->frontier[0].as<int32_t>(), g->frontier[1].as<int32_t>()};
-  // counters: [0..3] per level (see BfsPushOp), [8],[9] as int32: queue sizes
+  // counters: [0..3] per level (see BfsPushOp); slots 8, 9 hold the int32 queue sizes
   unsigned long long *ctr = g->counters.as<unsigned long long>();
-  int32_t *qsize = reinterpret_cast<int32_t *>(ctr + 8);
+  int32_t *qsz[2] = {reinterpret_cast<int32_t *>(ctr + 8), reinterpret_cast<int32_t *>(ctr + 9)};
   g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
   LbsPhaseTimer lbs_timer(g->prof, "bfs", "top_down");
 
@@ -213,7 +213,7 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
     GG_HIP(fill_async<int32_t>(depth, -1, n, s));
     GG_HIP(fill_async<uint32_t>(visited, 0u, words, s));
     GG_HIP(fill_async<unsigned long long>(ctr, 0ull, 16, s));
-    hipLaunchKernelGGL(k_bfs_init, dim3(1), dim3(1), 0, s, depth, visited, queue[0], qsize, g->dg.row_offsets, src,
+    hipLaunchKernelGGL(k_bfs_init, dim3(1), dim3(1), 0, s, depth, visited, queue[0], qsz[0], g->dg.row_offsets, src,
                        ctr);
     GG_HIP(hipGetLastError());
   }
@@ -235,21 +235,20 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
     if (!bottom_up && mf > mu / alpha && nf > prev_nf) {
       TimedRegion t(g->prof, "bfs.queue_to_bits");
       GG_HIP(fill_async<uint32_t>(fbits[cur], 0u, words, s));
-      hipLaunchKernelGGL(k_queue_to_bits, dim3(grid_for(nf, 256, 4096)), dim3(256), 0, s, queue[cur], qsize + cur,
+      hipLaunchKernelGGL(k_queue_to_bits, dim3(grid_for(nf, 256, 4096)), dim3(256), 0, s, queue[cur], qsz[cur],
                          fbits[cur]);
       GG_HIP(hipGetLastError());
       bottom_up = true;
     } else if (bottom_up && nf < n / beta && nf < prev_nf) {
       TimedRegion t(g->prof, "bfs.bits_to_queue");
-      GG_HIP(fill_async<int32_t>(qsize + cur, 0, 1, s));
+      GG_HIP(slot_resets_async(ctr, {{8 + cur, 0ull}}, s));
       hipLaunchKernelGGL(k_bits_to_queue, dim3(grid_for(words, 256, 4096)), dim3(256), 0, s, fbits[cur], words,
-                         queue[cur], qsize + cur);
+                         queue[cur], qsz[cur]);
       GG_HIP(hipGetLastError());
       bottom_up = false;
     }
     const int nxt = cur ^ 1;
-    GG_HIP(fill_async<unsigned long long>(ctr, 0ull, 4, s));
-    GG_HIP(fill_async<int32_t>(qsize + nxt, 0, 1, s));
+    GG_HIP(slot_resets_async(ctr, {{0, 0ull}, {1, 0ull}, {2, 0ull}, {3, 0ull}, {8 + nxt, 0ull}}, s));
     double lvl_bytes;
     if (bottom_up) {
       {
@@ -270,13 +269,13 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
       pull_levels++;
     } else {
       BfsPushOp op{g->dg.row_offsets, g->dg.column_indices, depth, visited, ctr, levels + 1, 0ull};
-      lbs_launch(op, false, queue[cur], qsize + cur, nf, (int64_t)n + g->m, lb, queue[nxt], qsize + nxt, s,
+      lbs_launch(op, false, queue[cur], qsz[cur], nf, (int64_t)n + g->m, lb, queue[nxt], qsz[nxt], s,
                  lbs_timer);
       GG_HIP(hipGetLastError());
       GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 10 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
       GG_HIP(hipStreamSynchronize(s));
       const int64_t E = g->h64()[2];
-      const int64_t N = reinterpret_cast<int32_t *>(g->h_pinned + 16)[nxt];
+      const int64_t N = g->h_pinned[(8 + nxt) * 2];
       lvl_bytes = 12.0 * nf + 8.0 * E + 8.0 * N;
       g->prof.add_bytes("bfs.lbs_count", 12.0 * nf);
       g->prof.add_bytes("bfs.top_down", 8.0 * E + 8.0 * N);

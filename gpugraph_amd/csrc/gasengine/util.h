@@ -7,6 +7,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <initializer_list>
+#include <utility>
+
 namespace GASengine {
 
 constexpr int kWave = 64;
@@ -145,6 +148,30 @@ template <class T>
 __global__ void k_sequence(T *p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = (T)i;
+}
+
+// Several 64-bit counter slots set in one launch (per-iteration resets).
+struct SlotResets {
+  int32_t count;
+  int32_t idx[8];
+  unsigned long long val[8];
+};
+
+template <int D = 0>
+__global__ void k_slot_resets(unsigned long long *c, SlotResets r) {
+  if ((int)threadIdx.x < r.count) c[r.idx[threadIdx.x]] = r.val[threadIdx.x];
+}
+
+inline hipError_t slot_resets_async(unsigned long long *c, std::initializer_list<std::pair<int, unsigned long long>> kv,
+                                    hipStream_t s) {
+  SlotResets r{};
+  for (const auto &p : kv) {
+    r.idx[r.count] = p.first;
+    r.val[r.count] = p.second;
+    r.count++;
+  }
+  hipLaunchKernelGGL(k_slot_resets<0>, dim3(1), dim3(64), 0, s, c, r);
+  return hipGetLastError();
 }
 
 inline int grid_for(int64_t n, int block, int max_grid = 65536) {

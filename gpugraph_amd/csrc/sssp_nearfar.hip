@@ -148,8 +148,7 @@ k_sssp_collect(uint8_t *__restrict__ dirty, const int32_t *__restrict__ dist, co
   unsigned long long remain = 0, degsum = 0;
   int32_t dmin = INT_MAX;
   for (int64_t base = (int64_t)blockIdx.x * kCollectStep; base < n; base += (int64_t)gridDim.x * kCollectStep) {
-    int32_t picked[4];
-    int32_t cnt = 0;
+    uint32_t selmask = 0;  // bit k: vertex of pass k selected (no dynamically indexed arrays)
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const int64_t v = base + k * kCollectBlock + threadIdx.x;
@@ -170,15 +169,17 @@ k_sssp_collect(uint8_t *__restrict__ dirty, const int32_t *__restrict__ dist, co
         const int64_t ww = (base + k * kCollectBlock + wid * kWave) / 32 + lane;
         if (ww < words) front[ww] = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
       }
-      if (sel) picked[cnt++] = (int32_t)v;
+      selmask |= sel ? (1u << k) : 0u;
     }
     int32_t total;
-    const int32_t off = block_exclusive_scan<kCollectBlock>(cnt, s_scan, &total);
+    const int32_t off = block_exclusive_scan<kCollectBlock>(__popc(selmask), s_scan, &total);
     if (total) {
       if (threadIdx.x == 0) s_base = atomicAdd(qsize, total);
       __syncthreads();
-      const int32_t b = s_base;
-      for (int k = 0; k < cnt; k++) queue[b + off + k] = picked[k];
+      int32_t at = s_base + off;
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (selmask & (1u << k)) queue[at++] = (int32_t)(base + k * kCollectBlock + threadIdx.x);
       __syncthreads();
     }
   }
@@ -260,9 +261,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   }
   const int collect_grid = grid_for(n, kCollectStep, 2048);
   while (iterations < g->opt.iter_num) {
-    GG_HIP(fill_async<int32_t>(qsize, 0, 1, s));
-    GG_HIP(fill_async<unsigned long long>(ctr + 16, 0ull, 3, s));
-    GG_HIP(fill_async<unsigned long long>(ctr + 17, (unsigned long long)INT_MAX, 1, s));
+    GG_HIP(slot_resets_async(ctr, {{8, 0ull}, {9, 0ull}, {16, 0ull}, {17, (unsigned long long)INT_MAX}, {18, 0ull}}, s));
     {
       TimedRegion t(g->prof, "sssp.collect");
       hipLaunchKernelGGL(k_sssp_collect, dim3(collect_grid), dim3(256), 0, s, dirty, dist, g->dg.row_offsets, n,
