@@ -127,6 +127,15 @@ def _ptr(a: np.ndarray):
     return C.c_void_p(a.ctypes.data)
 
 
+def _dev_ptr(x) -> int:
+    """Device address of a torch tensor (must be contiguous) or a raw int."""
+    if hasattr(x, "data_ptr"):
+        if not x.is_contiguous():
+            raise ValueError("device tensors must be contiguous")
+        return int(x.data_ptr())
+    return int(x)
+
+
 class Graph:
     """A graph resident on the current HIP device (gg_graph handle)."""
 
@@ -227,19 +236,32 @@ class Graph:
 
     def pagerank(self, damping: float = 0.85, teleport=None, tol_l1: float = 1e-6, max_iter: int = 1000,
                  flags: int = 0, out=None):
+        """FP32 PageRank (gg_pagerank).  With GG_OUT_DEVICE, `teleport` and
+        `out` are device pointers (ints) or torch tensors."""
         st = _abi.Stats()
         out, ptr = self._out(out, np.float32, flags)
         tp = None
         if teleport is not None:
-            teleport = np.ascontiguousarray(teleport, dtype=np.float32)
-            tp = _ptr(teleport)
-        _abi.check(self.lib.gg_pagerank(self.h, damping, tp, tol_l1, max_iter, ptr, flags, C.byref(st)),
-                   "gg_pagerank")
+            if flags & _abi.GG_OUT_DEVICE:
+                tp = C.c_void_p(_dev_ptr(teleport))
+            else:
+                teleport = np.ascontiguousarray(teleport, dtype=np.float32)
+                tp = _ptr(teleport)
+        rc = self.lib.gg_pagerank(self.h, damping, tp, tol_l1, max_iter, ptr, flags, C.byref(st))
+        _abi.check(rc, "gg_pagerank")
         return out, st.as_dict()
 
     def pagerank_adjoint(self, rank, c, damping: float = 0.85, tol_l1: float = 1e-7, max_iter: int = 1000,
-                         flags: int = 0):
+                         flags: int = 0, grad_w=None, grad_t=None):
+        """Adjoint of pagerank for L = c . r at the stored ranks `rank`.
+        Host arrays by default; with GG_OUT_DEVICE every array argument is a
+        device pointer / torch tensor and grad_w / grad_t must be given."""
         st = _abi.Stats()
+        if flags & _abi.GG_OUT_DEVICE:
+            args = [C.c_void_p(_dev_ptr(x)) if x is not None else None for x in (rank, c, grad_w, grad_t)]
+            _abi.check(self.lib.gg_pagerank_adjoint(self.h, damping, args[0], args[1], tol_l1, max_iter, args[2],
+                                                    args[3], flags, C.byref(st)), "gg_pagerank_adjoint")
+            return grad_w, grad_t, st.as_dict()
         rank = np.ascontiguousarray(rank, dtype=np.float32)
         c = np.ascontiguousarray(c, dtype=np.float32)
         gw = np.empty(self.m, np.float32)
@@ -247,6 +269,17 @@ class Graph:
         _abi.check(self.lib.gg_pagerank_adjoint(self.h, damping, _ptr(rank), _ptr(c), tol_l1, max_iter, _ptr(gw),
                                                 _ptr(gt), flags, C.byref(st)), "gg_pagerank_adjoint")
         return gw, gt, st.as_dict()
+
+    def update_values(self, values, device: bool = False):
+        """Replace the edge values (same dtype): host array, or a device
+        pointer / torch tensor with device=True."""
+        if device:
+            _abi.check(self.lib.gg_graph_update_values(self.h, C.c_void_p(_dev_ptr(values)), _abi.GG_OUT_DEVICE),
+                       "gg_graph_update_values")
+        else:
+            dt = np.int32 if self.value_dtype == _abi.GG_VALUES_I32 else np.float32
+            v = np.ascontiguousarray(values, dtype=dt)
+            _abi.check(self.lib.gg_graph_update_values(self.h, _ptr(v), 0), "gg_graph_update_values")
 
     def batch(self, sssp_sources=(), ppr_seeds=(), damping: float = 0.85, tol_l1: float = 1e-6,
               max_iter: int = 1000, flags: int = 0):
@@ -271,8 +304,8 @@ class Graph:
     def _out(self, out, dtype, flags):
         if flags & _abi.GG_OUT_DEVICE:
             if out is None:
-                raise ValueError("GG_OUT_DEVICE needs a device pointer (int) in out")
-            return out, C.c_void_p(int(out))
+                raise ValueError("GG_OUT_DEVICE needs a device pointer (int) or tensor in out")
+            return out, C.c_void_p(_dev_ptr(out))
         if out is None:
             out = np.empty(self.n, dtype)
         return out, _ptr(out)

@@ -286,14 +286,14 @@ int gg_device_count(int32_t *count) {
 int gg_default_options(gg_options *o) {
   GG_CHECK_ARG(o, "out is NULL");
   o->threshold = -1;  // per-algorithm default: BFS 0, SSSP/CC 10000, PR 1000
-  o->wave_degree = 32;
+  o->wave_degree = 128;
   o->block_degree = 1024;
   o->iter_num = INT_MAX;
   o->bfs_alpha = 14;
   o->bfs_beta = 24;
-  o->lbs_tile = 2048;
+  o->lbs_tile = GASengine::kLbsTile;
   o->sssp_delta = 0;
-  o->pull_alpha = 8;
+  o->pull_alpha = 0;
   return GG_OK;
 }
 
@@ -420,7 +420,7 @@ int gg_graph_set_options(gg_graph *g, const gg_options *o) {
   GG_CHECK_ARG(o->wave_degree >= 1 && o->block_degree >= o->wave_degree, "need 1 <= wave_degree <= block_degree");
   GG_CHECK_ARG(o->iter_num >= 1, "iter_num must be >= 1");
   GG_CHECK_ARG(o->bfs_alpha >= 1 && o->bfs_beta >= 1, "bfs_alpha/bfs_beta must be >= 1");
-  GG_CHECK_ARG(o->lbs_tile == 2048, "lbs_tile: only 2048 is compiled in");
+  GG_CHECK_ARG(o->lbs_tile == GASengine::kLbsTile, "lbs_tile: only %d is compiled in", GASengine::kLbsTile);
   GG_CHECK_ARG(o->sssp_delta >= 0, "sssp_delta must be >= 0");
   GG_CHECK_ARG(o->pull_alpha >= 0, "pull_alpha must be >= 0");
   g->opt = *o;

@@ -136,6 +136,12 @@ struct gg_graph {
   gg::DevBuf frontier[2], bits[4], vtx[4], scan_local, partials, parts, info, counters;
   gg::DevBuf pr[8];    // PageRank vectors / CSC-ordered transition weights
   gg::DevBuf batch[8]; // batched-instance state
+  // Degree bins of the vertices for row reductions: [0] over CSC in-edges,
+  // [1] over CSR out-edges; lists of small / mid / large rows.
+  gg::DevBuf bins[2][3];
+  int32_t bin_count[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  bool bins_ready[2] = {false, false};
+  int64_t pr_weights_version = -1;  // values version the PageRank transition weights were built for
   int32_t *h_pinned = nullptr;  // pinned readback area (64 x int64)
   bool pr_cache_valid = false;
 

@@ -231,7 +231,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   int32_t *queue = g->frontier[0].as<int32_t>();
   unsigned long long *ctr = g->counters.as<unsigned long long>();
   int32_t *qsize = reinterpret_cast<int32_t *>(ctr + 8);
-  const int64_t delta = g->opt.sssp_delta > 0 ? g->opt.sssp_delta : 32;
+  const int64_t delta = g->opt.sssp_delta > 0 ? g->opt.sssp_delta : 128;
   g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
   LbsPhaseTimer lbs_timer(g->prof, "sssp", "push");
 

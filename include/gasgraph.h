@@ -67,13 +67,14 @@ typedef struct gg_graph gg_graph;
  * a per-algorithm "threshold" parameter (Algorithms/SSSP/register.h:39). */
 typedef struct gg_options {
   int32_t threshold;     /* |frontier| above which the scan-based LBS expand is used */
-  int32_t wave_degree;   /* degree >= this: one wave64 per vertex (tiered expand) */
+  int32_t wave_degree;   /* degree >= this: one wave64 per vertex (tiered expand; also the
+                            longest in-edge list a single lane scans in bottom-up BFS) */
   int32_t block_degree;  /* degree >= this: whole workgroup per vertex (tiered expand) */
   int32_t iter_num;      /* maximum GAS iterations (reference "iter_num") */
   int32_t bfs_alpha;     /* top-down -> bottom-up when m_f > m_u / alpha */
   int32_t bfs_beta;      /* bottom-up -> top-down when n_f < n / beta */
-  int32_t lbs_tile;      /* merge-path positions per LBS workgroup tile (multiple of 256) */
-  int32_t sssp_delta;    /* SSSP bucket width (0 = auto: 32); INT32_MAX = one bucket */
+  int32_t lbs_tile;      /* merge-path positions per LBS workgroup tile (read-only: 1024) */
+  int32_t sssp_delta;    /* SSSP bucket width (0 = auto: 128); INT32_MAX = one bucket */
   int32_t pull_alpha;    /* SSSP: pull step when frontier edges > m / pull_alpha (0 = never) */
 } gg_options;
 
@@ -159,7 +160,9 @@ int gg_pagerank(gg_graph *g, float damping, const float *teleport,
 /* Adjoint of gg_pagerank for L = sum_v c_v r_v, given the stored forward
  * ranks: grad_w (m, CSR order) = dL/dw for the raw F32 values (or dL/dP_uv
  * when the graph has no F32 values); grad_t (n) = dL/dt.  Either output may
- * be NULL.  No reference counterpart. */
+ * be NULL.  The adjoint iteration stops when its L1 residual is at most
+ * tol_l1 * ||c||_1.  With GG_OUT_DEVICE every array argument (rank, c and
+ * the outputs) is a device pointer.  No reference counterpart. */
 int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank,
                         const float *c, double tol_l1, int32_t max_iter,
                         float *grad_w, float *grad_t, uint32_t flags,

@@ -32,7 +32,8 @@ def test_binding_covers_header():
 
 def test_default_options_and_version():
     o = gg.default_options()
-    assert (o.wave_degree, o.block_degree, o.bfs_alpha, o.bfs_beta) == (32, 1024, 14, 24)
+    assert (o.wave_degree, o.block_degree, o.bfs_alpha, o.bfs_beta) == (128, 1024, 14, 24)
+    assert (o.lbs_tile, o.sssp_delta, o.pull_alpha) == (1024, 0, 0)
     assert o.threshold == -1 and o.iter_num == 2**31 - 1
     assert gg.load_library().gg_version() == 10000
 

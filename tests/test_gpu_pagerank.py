@@ -1,0 +1,170 @@
+"""GPU PageRank: FP32 compensated forward against the FP64 oracle (L1
+tolerance 1e-6, stated per test), personalised teleports, the reference's own
+PageRank program, the adjoint against the FP64 CPU adjoint and central finite
+differences, and the torch.autograd wrapper."""
+import numpy as np
+import pytest
+
+import gpugraph_amd as gg
+import oracle
+from gpugraph_amd import _abi, workloads
+
+pytestmark = pytest.mark.gpu
+L1_TOL = 1e-6  # FP32 forward vs FP64 oracle, summed over all vertices
+
+
+def l1(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)).sum())
+
+
+def normalised_p32(csr):
+    """P_uv = w_uv / S_u in FP64, rounded to FP32 (the library's transition
+    weights), returned as FP64 for the oracle."""
+    deg = csr.degrees()
+    w = csr.values.astype(np.float64)
+    s = row_sums(csr, w)
+    return (w / np.repeat(np.where(deg > 0, s, 1.0), deg)).astype(np.float32).astype(np.float64)
+
+
+def row_sums(csr, w):
+    rows = np.repeat(np.arange(csr.n), csr.degrees())
+    return np.bincount(rows, weights=w, minlength=csr.n)
+
+
+def test_kats(gpu_available, kat_graphs):
+    for k in kat_graphs:
+        csr = gg.CSR(k["row_offsets"], k["col_indices"])
+        with gg.Graph(csr, directed=k["directed"]) as g:
+            r, st = g.pagerank(0.85, tol_l1=1e-7)
+            assert l1(r, k["pagerank_d085"]) <= L1_TOL, k["name"]
+            assert st["residual"] <= 1e-7
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_rmat_directed(gpu_available, weighted):
+    c = gg.gen_rmat(14, 16, undirected=False, seed=31)
+    if weighted:
+        c = c.with_weights("f32", symmetric=False, seed=32)
+    p = normalised_p32(c) if weighted else None
+    exp, _, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=p, tol=1e-14)
+    with gg.Graph(c, directed=True) as g:
+        r, st = g.pagerank(0.85, tol_l1=1e-7)
+        r2, _ = g.pagerank(0.85, tol_l1=1e-7)
+    assert l1(r, exp) <= L1_TOL
+    np.testing.assert_array_equal(r, r2)  # reproducible bit for bit
+    assert abs(float(np.sum(r, dtype=np.float64)) - 1.0) < 1e-5
+    assert st["iterations"] > 20
+
+
+def test_undirected_and_personalised(gpu_available):
+    c = workloads.config1_er()
+    rng = np.random.default_rng(33)
+    seeds = rng.choice(c.n, 3, replace=False)
+    with gg.Graph(c) as g:
+        r, _ = g.pagerank(0.85, tol_l1=1e-7)
+        assert l1(r, oracle.pagerank_f64(c.row_offsets, c.col_indices, tol=1e-14)[0]) <= L1_TOL
+        for s in seeds:
+            t = np.zeros(c.n)
+            t[s] = 1.0
+            r, _ = g.pagerank(0.85, teleport=t.astype(np.float32), tol_l1=1e-7)
+            assert l1(r, oracle.pagerank_f64(c.row_offsets, c.col_indices, t=t, tol=1e-14)[0]) <= L1_TOL
+            assert r[s] == r.max()
+
+
+def test_reference_pagerank_program(gpu_available):
+    """The reference's own PageRank (0.15 + 0.85 sum, per-vertex tolerance
+    0.01) checked the way its driver checks it: relative L2 against CPUPR
+    below tol (PageRank.cu:307-313, default tol 0.1); here also below 0.02."""
+    c = gg.gen_rmat(13, 16, undirected=False, seed=34)
+    exp, _ = oracle.pr_reference(c.row_offsets, c.col_indices)
+    with gg.Graph(c, directed=True) as g:
+        r, st = g.pagerank_reference()
+    rel = np.linalg.norm(r.astype(np.float64) - exp) / np.linalg.norm(exp)
+    assert rel < 0.02, rel
+    assert st["iterations"] >= 2
+
+
+def test_adjoint_vs_fp64_and_finite_differences(gpu_available):
+    c = gg.gen_rmat(12, 16, undirected=False, seed=35).with_weights("f32", symmetric=False, seed=36)
+    n, m = c.n, c.m
+    rng = np.random.default_rng(37)
+    cv = rng.standard_normal(n)
+    w64 = c.values.astype(np.float64)
+    p = normalised_p32(c)
+    with gg.Graph(c, directed=True) as g:
+        r, _ = g.pagerank(0.85, tol_l1=1e-7)
+        gw, gt, st = g.pagerank_adjoint(r, cv.astype(np.float32), tol_l1=1e-7)
+    # FP64 CPU adjoint at the FP64 ranks (same transition weights).
+    r64, _, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=p, tol=1e-15)
+    lam, gw64, gt64, _ = oracle.pagerank_adjoint_f64(c.row_offsets, c.col_indices, w64, r64, cv, tol=1e-15)
+    assert np.linalg.norm(gt - gt64) / np.linalg.norm(gt64) < 1e-4
+    assert np.linalg.norm(gw - gw64) / np.linalg.norm(gw64) < 1e-4
+    # Central differences on 256 random edges (FP64 forward, raw weights renormalised).
+    deg = c.degrees()
+    edges = rng.choice(m, 256, replace=False)
+
+    def loss(wv):
+        s = row_sums(c, wv)
+        pv = wv / np.repeat(np.where(deg > 0, s, 1.0), deg)
+        return float(cv @ oracle.pagerank_f64(c.row_offsets, c.col_indices, p=pv, tol=1e-15)[0])
+
+    fd = np.empty(len(edges))
+    for i, e in enumerate(edges):
+        h = 1e-3 * max(abs(w64[e]), 1e-3)
+        wp, wm = w64.copy(), w64.copy()
+        wp[e] += h
+        wm[e] -= h
+        fd[i] = (loss(wp) - loss(wm)) / (2 * h)
+    assert np.linalg.norm(gw[edges] - fd) / np.linalg.norm(fd) < 1e-4
+    big = np.abs(fd) > 1e-2 * np.abs(fd).max()
+    np.testing.assert_allclose(gw[edges][big], fd[big], rtol=1e-3)
+
+
+def test_autograd_function(gpu_available):
+    import torch
+
+    from gpugraph_amd.autograd import pagerank
+
+    c = gg.gen_rmat(11, 16, undirected=False, seed=38).with_weights("f32", symmetric=False, seed=39)
+    rng = np.random.default_rng(40)
+    cv = rng.standard_normal(c.n).astype(np.float32)
+    with gg.Graph(c, directed=True) as g:
+        w = torch.tensor(c.values, device="cuda", requires_grad=True)
+        t = torch.full((c.n,), 1.0 / c.n, device="cuda", requires_grad=True)
+        r = pagerank(g, w, t)
+        loss = (torch.tensor(cv, device="cuda") * r).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        r_ref, _ = g.pagerank(0.85, teleport=np.full(c.n, 1.0 / c.n, np.float32), tol_l1=1e-7)
+        gw, gt, _ = g.pagerank_adjoint(r_ref, cv, tol_l1=1e-7)
+    np.testing.assert_allclose(r.detach().cpu().numpy(), r_ref, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(w.grad.cpu().numpy(), gw, rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(t.grad.cpu().numpy(), gt, rtol=1e-5, atol=1e-9)
+
+
+def test_update_values_changes_ranks(gpu_available):
+    c = gg.gen_rmat(11, 8, undirected=False, seed=41).with_weights("f32", symmetric=False, seed=42)
+    with gg.Graph(c, directed=True) as g:
+        r1, _ = g.pagerank(0.85, tol_l1=1e-7)
+        w2 = c.values.copy()
+        w2[: c.m // 2] *= 5.0
+        g.update_values(w2)
+        r2, _ = g.pagerank(0.85, tol_l1=1e-7)
+    c2 = gg.CSR(c.row_offsets, c.col_indices, w2)
+    exp2, _, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=normalised_p32(c2), tol=1e-14)
+    assert l1(r2, exp2) <= L1_TOL
+    assert l1(r1, r2) > 1e-4
+
+
+def test_errors(gpu_available):
+    c = workloads.config1_er().with_weights("i32", 1, 64, symmetric=True, seed=1)
+    with gg.Graph(c) as g:
+        with pytest.raises(gg.GGError) as e:
+            g.pagerank()
+        assert e.value.code == _abi.GG_ERR_INVALID  # int32 weights
+    with gg.Graph(workloads.config1_er()) as g:
+        with pytest.raises(gg.GGError) as e:
+            g.pagerank(tol_l1=0.0, max_iter=3)
+        assert e.value.code == _abi.GG_ERR_NOT_CONVERGED
+        with pytest.raises(gg.GGError):
+            g.pagerank(damping=1.0)
