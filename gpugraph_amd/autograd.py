@@ -10,14 +10,26 @@ forward() loads the raw edge weights into the resident graph (the library
 normalises each source's out-weights), runs the compensated FP32 forward and
 saves the ranks; backward() hands the saved ranks and the incoming gradient
 to the adjoint kernel, which returns dL/dw (CSR order, through the
-normalisation) and dL/dt.  Both run on the graph's stream, which is pointed
-at torch's current stream so no extra synchronisation is needed.
+normalisation) and dL/dt.  Both run on the graph's stream, which is
+ordered after torch's pending work (see _join_torch_stream).
 """
 from __future__ import annotations
 
 import torch
 
 from . import Graph, _abi
+
+
+def _join_torch_stream(graph: Graph):
+    """Order the library's work after torch's pending work.  A real torch
+    stream is shared outright; on the legacy default stream (handle 0) the
+    library keeps its own non-blocking stream, so torch's work is drained
+    first (the library synchronises its own stream before returning)."""
+    handle = torch.cuda.current_stream().cuda_stream
+    if handle:
+        graph.set_stream(handle)
+    else:
+        torch.cuda.current_stream().synchronize()
 
 
 class PageRankFunction(torch.autograd.Function):
@@ -27,11 +39,11 @@ class PageRankFunction(torch.autograd.Function):
             raise ValueError("weights must be a float32 CUDA tensor with one entry per CSR edge")
         if graph.value_dtype != _abi.GG_VALUES_F32:
             raise ValueError("the graph must be created with float32 edge values")
-        graph.set_stream(torch.cuda.current_stream().cuda_stream)
         w = weights.detach().contiguous()
-        graph.update_values(w, device=True)
         rank = torch.empty(graph.n, dtype=torch.float32, device=weights.device)
         tp = None if teleport is None else teleport.detach().contiguous()
+        _join_torch_stream(graph)
+        graph.update_values(w, device=True)
         graph.pagerank(damping=damping, teleport=tp, tol_l1=tol_l1, max_iter=max_iter, flags=_abi.GG_OUT_DEVICE,
                        out=rank)
         ctx.graph = graph
@@ -46,10 +58,10 @@ class PageRankFunction(torch.autograd.Function):
     def backward(ctx, grad_rank):
         (rank,) = ctx.saved_tensors
         graph: Graph = ctx.graph
-        graph.set_stream(torch.cuda.current_stream().cuda_stream)
         c = grad_rank.detach().to(torch.float32).contiguous()
         gw = torch.empty(graph.m, dtype=torch.float32, device=rank.device)
         gt = torch.empty(graph.n, dtype=torch.float32, device=rank.device)
+        _join_torch_stream(graph)
         graph.pagerank_adjoint(rank, c, damping=ctx.damping, tol_l1=ctx.adj_tol, max_iter=ctx.max_iter,
                                flags=_abi.GG_OUT_DEVICE, grad_w=gw, grad_t=gt)
         return gw, (gt if ctx.has_teleport else None), None, None, None, None, None
