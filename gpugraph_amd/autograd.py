@@ -68,6 +68,6 @@ class PageRankFunction(torch.autograd.Function):
 
 
 def pagerank(graph: Graph, weights: torch.Tensor, teleport: torch.Tensor | None = None, damping: float = 0.85,
-             tol_l1: float = 1e-7, max_iter: int = 1000, adj_tol: float = 1e-7) -> torch.Tensor:
+             tol_l1: float = 2e-7, max_iter: int = 1000, adj_tol: float = 2e-7) -> torch.Tensor:
     """Ranks r(w, t) with gradients for the raw edge weights and teleport."""
     return PageRankFunction.apply(weights, teleport, graph, damping, tol_l1, max_iter, adj_tol)

@@ -35,9 +35,9 @@ def test_kats(gpu_available, kat_graphs):
     for k in kat_graphs:
         csr = gg.CSR(k["row_offsets"], k["col_indices"])
         with gg.Graph(csr, directed=k["directed"]) as g:
-            r, st = g.pagerank(0.85, tol_l1=1e-7)
+            r, st = g.pagerank(0.85, tol_l1=2e-7)
             assert l1(r, k["pagerank_d085"]) <= L1_TOL, k["name"]
-            assert st["residual"] <= 1e-7
+            assert st["residual"] <= 2e-7
 
 
 @pytest.mark.parametrize("weighted", [False, True])
@@ -48,12 +48,12 @@ def test_rmat_directed(gpu_available, weighted):
     p = normalised_p32(c) if weighted else None
     exp, _, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=p, tol=1e-14)
     with gg.Graph(c, directed=True) as g:
-        r, st = g.pagerank(0.85, tol_l1=1e-7)
-        r2, _ = g.pagerank(0.85, tol_l1=1e-7)
+        r, st = g.pagerank(0.85, tol_l1=2e-7)
+        r2, _ = g.pagerank(0.85, tol_l1=2e-7)
     assert l1(r, exp) <= L1_TOL
     np.testing.assert_array_equal(r, r2)  # reproducible bit for bit
     assert abs(float(np.sum(r, dtype=np.float64)) - 1.0) < 1e-5
-    assert st["iterations"] > 20
+    assert st["iterations"] > 5
 
 
 def test_undirected_and_personalised(gpu_available):
@@ -61,12 +61,12 @@ def test_undirected_and_personalised(gpu_available):
     rng = np.random.default_rng(33)
     seeds = rng.choice(c.n, 3, replace=False)
     with gg.Graph(c) as g:
-        r, _ = g.pagerank(0.85, tol_l1=1e-7)
+        r, _ = g.pagerank(0.85, tol_l1=2e-7)
         assert l1(r, oracle.pagerank_f64(c.row_offsets, c.col_indices, tol=1e-14)[0]) <= L1_TOL
         for s in seeds:
             t = np.zeros(c.n)
             t[s] = 1.0
-            r, _ = g.pagerank(0.85, teleport=t.astype(np.float32), tol_l1=1e-7)
+            r, _ = g.pagerank(0.85, teleport=t.astype(np.float32), tol_l1=2e-7)
             assert l1(r, oracle.pagerank_f64(c.row_offsets, c.col_indices, t=t, tol=1e-14)[0]) <= L1_TOL
             assert r[s] == r.max()
 
@@ -92,8 +92,8 @@ def test_adjoint_vs_fp64_and_finite_differences(gpu_available):
     w64 = c.values.astype(np.float64)
     p = normalised_p32(c)
     with gg.Graph(c, directed=True) as g:
-        r, _ = g.pagerank(0.85, tol_l1=1e-7)
-        gw, gt, st = g.pagerank_adjoint(r, cv.astype(np.float32), tol_l1=1e-7)
+        r, _ = g.pagerank(0.85, tol_l1=2e-7)
+        gw, gt, st = g.pagerank_adjoint(r, cv.astype(np.float32), tol_l1=2e-7)
     # FP64 CPU adjoint at the FP64 ranks (same transition weights).
     r64, _, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=p, tol=1e-15)
     lam, gw64, gt64, _ = oracle.pagerank_adjoint_f64(c.row_offsets, c.col_indices, w64, r64, cv, tol=1e-15)
@@ -131,12 +131,12 @@ def test_autograd_function(gpu_available):
     with gg.Graph(c, directed=True) as g:
         w = torch.tensor(c.values, device="cuda", requires_grad=True)
         t = torch.full((c.n,), 1.0 / c.n, device="cuda", requires_grad=True)
-        r = pagerank(g, w, t)
+        r = pagerank(g, w, t, tol_l1=2e-7, adj_tol=2e-7)
         loss = (torch.tensor(cv, device="cuda") * r).sum()
         loss.backward()
         torch.cuda.synchronize()
-        r_ref, _ = g.pagerank(0.85, teleport=np.full(c.n, 1.0 / c.n, np.float32), tol_l1=1e-7)
-        gw, gt, _ = g.pagerank_adjoint(r_ref, cv, tol_l1=1e-7)
+        r_ref, _ = g.pagerank(0.85, teleport=np.full(c.n, 1.0 / c.n, np.float32), tol_l1=2e-7)
+        gw, gt, _ = g.pagerank_adjoint(r_ref, cv, tol_l1=2e-7)
     np.testing.assert_allclose(r.detach().cpu().numpy(), r_ref, rtol=0, atol=1e-9)
     np.testing.assert_allclose(w.grad.cpu().numpy(), gw, rtol=1e-5, atol=1e-9)
     np.testing.assert_allclose(t.grad.cpu().numpy(), gt, rtol=1e-5, atol=1e-9)
@@ -145,11 +145,11 @@ def test_autograd_function(gpu_available):
 def test_update_values_changes_ranks(gpu_available):
     c = gg.gen_rmat(11, 8, undirected=False, seed=41).with_weights("f32", symmetric=False, seed=42)
     with gg.Graph(c, directed=True) as g:
-        r1, _ = g.pagerank(0.85, tol_l1=1e-7)
+        r1, _ = g.pagerank(0.85, tol_l1=2e-7)
         w2 = c.values.copy()
         w2[: c.m // 2] *= 5.0
         g.update_values(w2)
-        r2, _ = g.pagerank(0.85, tol_l1=1e-7)
+        r2, _ = g.pagerank(0.85, tol_l1=2e-7)
     c2 = gg.CSR(c.row_offsets, c.col_indices, w2)
     exp2, _, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=normalised_p32(c2), tol=1e-14)
     assert l1(r2, exp2) <= L1_TOL
