@@ -382,7 +382,6 @@ int gg_pagerank(gg_graph *g, float damping, const float *teleport, double tol_l1
   GG_CHECK_ARG(g, "graph is NULL");
   GG_CHECK_ARG(damping >= 0.f && damping < 1.f, "damping must be in [0, 1)");
   GG_CHECK_ARG(tol_l1 >= 0 && max_iter >= 1, "need tol_l1 >= 0 and max_iter >= 1");
-  GG_CHECK_ARG(g->value_dtype != GG_VALUES_I32, "PageRank takes F32 edge weights or none");
   GG_TRY(activate(g));
   const int32_t n = g->n;
   const int64_t m = g->m;
@@ -458,7 +457,7 @@ int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank, const flo
   GG_CHECK_ARG(g && rank && c, "graph/rank/c is NULL");
   GG_CHECK_ARG(damping >= 0.f && damping < 1.f, "damping must be in [0, 1)");
   GG_CHECK_ARG(tol_l1 >= 0 && max_iter >= 1, "need tol_l1 >= 0 and max_iter >= 1");
-  GG_CHECK_ARG(g->value_dtype != GG_VALUES_I32, "PageRank takes F32 edge weights or none");
+  // Int32 edge values (SSSP weights) do not enter PageRank: transitions are then 1/outdeg.
   GG_TRY(activate(g));
   const int32_t n = g->n;
   const int64_t m = g->m;
@@ -562,12 +561,6 @@ int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank, const flo
     return set_error(GG_ERR_NOT_CONVERGED, "adjoint: residual %.3e > tol %.3e x ||c||_1 after %d sweeps", resid, tol_l1,
                      iters);
   return GG_OK;
-}
-
-int gg_batch_run(gg_graph *g, const gg_batch_spec *, uint32_t, gg_stats *) {
-  gg::clear_error();
-  GG_CHECK_ARG(g, "graph is NULL");
-  return gg::set_error(GG_ERR_UNSUPPORTED, "gg_batch_run: not built yet");
 }
 
 }  // extern "C"

@@ -148,7 +148,8 @@ int gg_pagerank_reference(gg_graph *g, float *rank, uint32_t flags,
                           gg_stats *stats);
 
 /* PageRank r = d (P^T r + dangling/n) + (1-d) t, P_uv = w_uv / sum_v' w_uv'
- * (F32 values) or 1/outdeg(u) (no F32 values).  FP32 gather with Kahan
+ * (F32 values) or 1/outdeg(u) (no values, or int32 values, which are SSSP
+ * weights and do not enter PageRank).  FP32 gather with Kahan
  * compensation, FP64 dangling sum and L1 residual; stops when the L1
  * residual <= tol_l1 (GG_ERR_NOT_CONVERGED after max_iter sweeps).
  * teleport: n floats or NULL for uniform.  No reference counterpart beyond

@@ -1,0 +1,89 @@
+"""Batched instances (gg_batch_run): every SSSP instance bit-equal to the
+oracle and to its plain single-source run; every personalised PageRank within
+L1 1e-6 of the FP64 oracle with a one-hot teleport; passes beyond 128 / 64
+instances; argument errors."""
+import numpy as np
+import pytest
+
+import gpugraph_amd as gg
+import oracle
+from gpugraph_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def l1(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)).sum())
+
+
+@pytest.fixture(scope="module")
+def rmat_directed():
+    return workloads.config4_rmat(scale=13)
+
+
+def test_sssp_batch_matches_oracle_and_plain(gpu_available, rmat_directed):
+    c = rmat_directed
+    src, _ = workloads.config4_instances(c, 130, 0)  # two passes: 128 + 2
+    src[5] = src[4]  # a repeated source is two independent instances
+    with gg.Graph(c, directed=True) as g:
+        dist, _, st = g.batch(sssp_sources=src)
+        for i in (0, 4, 5, 63, 64, 127, 128, 129):
+            exp = oracle.sssp(c.row_offsets, c.col_indices, c.values, int(src[i]))
+            np.testing.assert_array_equal(dist[i], exp, err_msg=f"instance {i}")
+        for i in range(0, 130, 9):
+            plain, _ = g.sssp(int(src[i]))
+            np.testing.assert_array_equal(dist[i], plain)
+    assert st["iterations"] >= 2
+
+
+def test_ppr_batch(gpu_available, rmat_directed):
+    c = rmat_directed
+    _, seeds = workloads.config4_instances(c, 0, 64)
+    with gg.Graph(c, directed=True) as g:
+        _, rank, st = g.batch(ppr_seeds=seeds, damping=0.85, tol_l1=2e-7, max_iter=1000)
+        for i in (0, 1, 31, 32, 63):
+            t = np.zeros(c.n)
+            t[seeds[i]] = 1.0
+            exp, _, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, t=t, tol=1e-14)
+            assert l1(rank[i], exp) <= 1e-6, i
+            plain, _ = g.pagerank(0.85, teleport=t.astype(np.float32), tol_l1=2e-7)
+            assert l1(rank[i], plain) <= 1e-6, i
+    assert st["residual"] <= 2e-7
+
+
+def test_mixed_batch_and_weighted_ppr(gpu_available):
+    c = gg.gen_rmat(12, 16, undirected=False, seed=51)
+    ci = c.with_weights("i32", 1, 64, symmetric=False, seed=52)
+    rng = np.random.default_rng(53)
+    cand = np.flatnonzero(c.degrees() > 0)
+    src = rng.choice(cand, 40, replace=False).astype(np.int32)
+    seeds = rng.choice(cand, 70, replace=False).astype(np.int32)  # two PPR passes: 64 + 6
+    with gg.Graph(ci, directed=True) as g:
+        dist, rank, _ = g.batch(sssp_sources=src, ppr_seeds=seeds, tol_l1=2e-7)
+    for i in (0, 39):
+        np.testing.assert_array_equal(dist[i], oracle.sssp(ci.row_offsets, ci.col_indices, ci.values, int(src[i])))
+    for i in (0, 63, 64, 69):
+        t = np.zeros(c.n)
+        t[seeds[i]] = 1.0
+        assert l1(rank[i], oracle.pagerank_f64(c.row_offsets, c.col_indices, t=t, tol=1e-14)[0]) <= 1e-6
+    # F32-weighted transitions in the batched PPR
+    cf = c.with_weights("f32", symmetric=False, seed=54)
+    deg = cf.degrees()
+    rows = np.repeat(np.arange(cf.n), deg)
+    s = np.bincount(rows, weights=cf.values.astype(np.float64), minlength=cf.n)
+    p = (cf.values / np.repeat(np.where(deg > 0, s, 1.0), deg)).astype(np.float32).astype(np.float64)
+    with gg.Graph(cf, directed=True) as g:
+        _, rank, _ = g.batch(ppr_seeds=seeds[:8], tol_l1=2e-7)
+    for i in (0, 7):
+        t = np.zeros(c.n)
+        t[seeds[i]] = 1.0
+        assert l1(rank[i], oracle.pagerank_f64(cf.row_offsets, cf.col_indices, p=p, t=t, tol=1e-14)[0]) <= 1e-6
+
+
+def test_batch_errors(gpu_available):
+    c = workloads.config1_er()
+    with gg.Graph(c) as g:
+        with pytest.raises(gg.GGError):
+            g.batch(sssp_sources=[0])  # no int32 weights
+        with pytest.raises(gg.GGError):
+            g.batch(ppr_seeds=[c.n])  # seed out of range

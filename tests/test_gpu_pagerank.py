@@ -156,12 +156,11 @@ def test_update_values_changes_ranks(gpu_available):
     assert l1(r1, r2) > 1e-4
 
 
-def test_errors(gpu_available):
-    c = workloads.config1_er().with_weights("i32", 1, 64, symmetric=True, seed=1)
-    with gg.Graph(c) as g:
-        with pytest.raises(gg.GGError) as e:
-            g.pagerank()
-        assert e.value.code == _abi.GG_ERR_INVALID  # int32 weights
+def test_errors_and_int_weights(gpu_available):
+    c = workloads.config1_er()
+    with gg.Graph(c.with_weights("i32", 1, 64, symmetric=True, seed=1)) as g:
+        r, _ = g.pagerank(tol_l1=2e-7)  # int32 values are SSSP weights: PageRank is unweighted
+        assert l1(r, oracle.pagerank_f64(c.row_offsets, c.col_indices, tol=1e-14)[0]) <= L1_TOL
     with gg.Graph(workloads.config1_er()) as g:
         with pytest.raises(gg.GGError) as e:
             g.pagerank(tol_l1=0.0, max_iter=3)
