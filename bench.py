@@ -1,15 +1,22 @@
-"""Benchmark: plain BFS + SSSP on R-MAT scale 22 (undirected, edge factor 16,
-int32 weights in [1, 64], source = highest-degree vertex), one MI355X.
+"""Benchmark on one MI355X.
 
-A step is one BFS and one SSSP from the source, graph resident in HBM and
-outputs kept on the device.  value = GTEPS over the timed steps, counting
-sum of CSR out-degrees of the reached vertices per traversal (the
-reference's counting, Algorithms/BFS/bfs.cu:963-980); the undirected-edge
-count is half of it.  Kernel timings come from HIP events recorded on the
-library's stream inside the timed region (GG_INSTRUMENT); the roofline is
-that of the dominant kernel class.  The CPU baseline is the oracle (a
-single-threaded C restatement of the reference's CPUBFS and, for SSSP, a
-heap Dijkstra with CPUSSSP's output) on the same graph and source.
+Headline (value): plain BFS + SSSP on R-MAT scale 22 (undirected, edge factor
+16, int32 weights in [1, 64], source = highest-degree vertex).  A step is one
+BFS and one SSSP from the source, graph resident in HBM and outputs kept on
+the device; value = GTEPS over the timed steps, counting the sum of CSR
+out-degrees of the reached vertices per traversal (the reference's counting,
+Algorithms/BFS/bfs.cu:963-980); the undirected-edge count is half of it.
+Kernel timings come from HIP events recorded on the library's stream inside
+the timed region (GG_INSTRUMENT); `roofline` is that of the dominant kernel
+class.  `cpu_baseline` is the oracle (single-threaded C restatement of the
+reference's CPUBFS and, for SSSP, a heap Dijkstra with CPUSSSP's output) on
+the same graph and source.
+
+Extras (same JSON line, `extras`): CC on the 2048^2 percolated lattice
+(label propagation and hooking); batched 128 SSSP + 64 PPR on R-MAT scale 20
+directed against 128 + 64 plain launches; PageRank and its adjoint on R-MAT
+scale 22 directed with FP32 weights.  --full-checks adds the FP64 oracle's
+PageRank L1 error at scale 22 (tens of seconds of CPU time).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -29,26 +36,19 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--scale", type=int, default=22)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-extras", action="store_true", help="skip the lattice CC / ER lines")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path is not built.",
-              file=sys.stderr)
-        sys.exit(2)
-
+def median_ms(fn, reps):
     import numpy as np
-    import torch
 
-    import gpugraph_amd as gg
-    from gpugraph_amd import _abi, workloads
+    fn()
+    ts = []
+    st = None
+    for _ in range(reps):
+        _, st = fn()[:2]
+        ts.append(st["gpu_ms"])
+    return float(np.median(ts)), st
 
+
+def headline(args, gg, _abi, workloads, torch, np):
     t0 = time.perf_counter()
     csr = workloads.config2_rmat(args.scale)
     src = workloads.max_degree_vertex(csr)
@@ -64,8 +64,8 @@ def main():
 
     def step(instrument):
         f = flags | (_abi.GG_INSTRUMENT if instrument else 0)
-        _, sb = g.bfs(src, flags=f, out=out_bfs.data_ptr())
-        _, ss = g.sssp(src, flags=f, out=out_sssp.data_ptr())
+        _, sb = g.bfs(src, flags=f, out=out_bfs)
+        _, ss = g.sssp(src, flags=f, out=out_sssp)
         return sb, ss
 
     for _ in range(args.warmup):
@@ -83,16 +83,14 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     prof = g.profile()
+    g.close()
     re_bfs = sb_list[-1]["reached_edges"]
     re_sssp = ss_list[-1]["reached_edges"]
-    total_edges = args.steps * (re_bfs + re_sssp)
-    value = total_edges / wall / 1e9
+    value = args.steps * (re_bfs + re_sssp) / wall / 1e9
     bfs_ms = float(np.median([s["gpu_ms"] for s in sb_list]))
     sssp_ms = float(np.median([s["gpu_ms"] for s in ss_list]))
     log(f"timed {args.steps} steps in {wall * 1e3:.1f} ms: {value:.2f} GTEPS "
         f"(bfs {bfs_ms:.3f} ms, sssp {sssp_ms:.3f} ms median)")
-
-    # Dominant kernel class by event time over the timed region.
     name, ent = max(prof.items(), key=lambda kv: kv[1]["total_ms"])
     achieved = ent["alg_bytes"] / (ent["total_ms"] * 1e-3) / 1e9 if ent["total_ms"] > 0 else 0.0
     roofline = {
@@ -104,7 +102,6 @@ def main():
     kernels = {k: {"launches": v["launches"], "ms": round(v["total_ms"], 4),
                    "GBps": round(v["alg_bytes"] / (v["total_ms"] * 1e-3) / 1e9, 1) if v["total_ms"] > 0 else None}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["total_ms"])}
-
     per_alg = {
         "bfs": {"gteps": re_bfs / (bfs_ms * 1e-3) / 1e9, "gteps_undirected": re_bfs / 2 / (bfs_ms * 1e-3) / 1e9,
                 "ms": bfs_ms, "levels": sb_list[-1]["iterations"], "pull_levels": sb_list[-1]["pull_iterations"],
@@ -114,23 +111,6 @@ def main():
                  "edges_inspected": ss_list[-1]["edges_inspected"],
                  "hbm_frac": ss_list[-1]["alg_bytes"] / (sssp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
-
-    extras = {}
-    if not args.no_extras:
-        lat = workloads.config3_lattice(2048)
-        with gg.Graph(lat) as gl:
-            for hook in (False, True):
-                gl.cc(hook=hook)
-                ts = []
-                for _ in range(3):
-                    _, st = gl.cc(hook=hook)
-                    ts.append(st["gpu_ms"])
-                ms = float(np.median(ts))
-                extras["cc_lattice_" + ("hook" if hook else "labelprop")] = {
-                    "ms": ms, "gteps": st["reached_edges"] / (ms * 1e-3) / 1e9, "iterations": st["iterations"]}
-        log(f"lattice CC: {extras}")
-    g.close()
-
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -145,15 +125,137 @@ def main():
                "sample": f"one BFS ({tb:.2f} s) + one heap-Dijkstra SSSP ({ts:.2f} s) from the same source on "
                          f"the same R-MAT s{args.scale} graph, 1 thread"}
         log(f"cpu baseline: {cpu}")
+    config = {"workload": f"rmat{args.scale}_undirected_bfs+sssp", "n": csr.n, "m_csr": csr.m, "edge_factor": 16,
+              "weights": "int32 uniform [1,64]", "source": src, "step": "1 BFS + 1 SSSP",
+              "teps_counting": "sum of CSR degree over reached vertices"}
+    return value, wall, roofline, kernels, per_alg, cpu, config
 
+
+def extra_lattice(gg, workloads, np):
+    lat = workloads.config3_lattice(2048)
+    out = {}
+    with gg.Graph(lat) as gl:
+        for hook in (False, True):
+            ms, st = median_ms(lambda: gl.cc(hook=hook), 3)
+            out["cc_lattice_" + ("hook" if hook else "labelprop")] = {
+                "ms": ms, "gteps": st["reached_edges"] / (ms * 1e-3) / 1e9, "iterations": st["iterations"]}
+    return out
+
+
+def extra_batch(gg, workloads, torch, np, scale):
+    c = workloads.config4_rmat(scale)
+    src, seeds = workloads.config4_instances(c, 128, 64)
+    out = {"graph": f"rmat{scale} directed, n={c.n}, m={c.m}"}
+    with gg.Graph(c, directed=True) as g:
+        g.batch(sssp_sources=src[:8], ppr_seeds=seeds[:4], tol_l1=1e-6)  # warm-up (allocations)
+        g.sssp(int(src[0]))
+        g.pagerank(teleport=np.eye(1, c.n, int(seeds[0]), dtype=np.float32)[0], tol_l1=1e-6)
+        res = {}
+        for label, s_list, p_list in (("sssp128", src, ()), ("ppr64", (), seeds), ("both", src, seeds)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, _, st = g.batch(sssp_sources=s_list, ppr_seeds=p_list, tol_l1=1e-6)
+            res[label] = (time.perf_counter() - t0) * 1e3
+        dev_i = torch.empty(c.n, dtype=torch.int32, device="cuda")
+        dev_f = torch.empty(c.n, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in src:
+            g.sssp(int(s), flags=gg._abi.GG_OUT_DEVICE, out=dev_i)
+        plain_sssp = (time.perf_counter() - t0) * 1e3
+        tele = torch.zeros(c.n, dtype=torch.float32, device="cuda")
+        t0 = time.perf_counter()
+        for s in seeds:
+            tele.zero_()
+            tele[int(s)] = 1.0
+            torch.cuda.synchronize()  # torch's default stream -> the library's stream
+            g.pagerank(teleport=tele, tol_l1=1e-6, flags=gg._abi.GG_OUT_DEVICE, out=dev_f)
+        plain_ppr = (time.perf_counter() - t0) * 1e3
+    out.update({"batched_ms": res, "plain_ms": {"sssp128": plain_sssp, "ppr64": plain_ppr,
+                                                "both": plain_sssp + plain_ppr},
+                "speedup": {"sssp128": plain_sssp / res["sssp128"], "ppr64": plain_ppr / res["ppr64"],
+                            "both": (plain_sssp + plain_ppr) / res["both"]}})
+    return out
+
+
+def extra_pagerank(gg, workloads, np, scale, full_checks):
+    c = workloads.config5_rmat(scale)
+    out = {"graph": f"rmat{scale} directed f32, n={c.n}, m={c.m}"}
+    cvec = workloads.config5_loss_weights(c.n).astype(np.float32)
+    with gg.Graph(c, directed=True) as g:
+        g.pagerank(tol_l1=2e-7)
+        ms, st = median_ms(lambda: g.pagerank(tol_l1=2e-7), 3)
+        r, _ = g.pagerank(tol_l1=2e-7)
+        out["forward"] = {"ms": ms, "sweeps": st["iterations"], "residual_l1": st["residual"],
+                          "hbm_frac": st["alg_bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        ams, ast = median_ms(lambda: g.pagerank_adjoint(r, cvec, tol_l1=2e-7)[1:], 3)
+        out["adjoint"] = {"ms": ams, "sweeps": ast["iterations"],
+                          "hbm_frac": ast["alg_bytes"] / (ams * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if full_checks:
+        import oracle
+
+        deg = c.degrees()
+        rows = np.repeat(np.arange(c.n), deg)
+        s = np.bincount(rows, weights=c.values.astype(np.float64), minlength=c.n)
+        p = (c.values / np.repeat(np.where(deg > 0, s, 1.0), deg)).astype(np.float32).astype(np.float64)
+        t0 = time.perf_counter()
+        exp, it, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=p, tol=1e-12)
+        out["forward"]["l1_error_vs_fp64"] = float(np.abs(r.astype(np.float64) - exp).sum())
+        out["forward"]["fp64_oracle_s"] = time.perf_counter() - t0
+    return out
+
+
+def copy_probe(torch):
+    """A plain device copy of known size, for calibrating FETCH_SIZE / WRITE_SIZE."""
+    n = 256 << 20  # 1 GiB of float32
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": 4 * n}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true", help="headline workload only")
+    ap.add_argument("--full-checks", action="store_true", help="FP64 PageRank error at full size")
+    ap.add_argument("--copy-probe", action="store_true", help="run only a 1 GiB device copy (PMC calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    if args.copy_probe:
+        copy_probe(torch)
+        return
+
+    import gpugraph_amd as gg
+    from gpugraph_amd import _abi, workloads
+
+    value, wall, roofline, kernels, per_alg, cpu, config = headline(args, gg, _abi, workloads, torch, np)
+    extras = {}
+    if not args.no_extras:
+        extras.update(extra_lattice(gg, workloads, np))
+        log(f"lattice CC: {extras}")
+        extras["batched"] = extra_batch(gg, workloads, torch, np, 20)
+        log(f"batched: {extras['batched']}")
+        extras["pagerank"] = extra_pagerank(gg, workloads, np, 22, args.full_checks)
+        log(f"pagerank: {extras['pagerank']}")
     line = {
         "metric": "GTEPS (BFS/SSSP/CC, R-MAT s22) + % HBM roofline; batched vs plain speedup",
         "value": value, "unit": "GTEPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "int32", "data": "synthetic (seeded R-MAT generator)",
-        "config": {"workload": f"rmat{args.scale}_undirected_bfs+sssp", "n": csr.n, "m_csr": csr.m,
-                   "edge_factor": 16, "weights": "int32 uniform [1,64]", "source": src,
-                   "step": "1 BFS + 1 SSSP", "teps_counting": "sum of CSR degree over reached vertices"},
+        "dtype": "int32", "data": "synthetic (seeded R-MAT generator)", "config": config,
         "roofline": roofline, "cpu_baseline": cpu, "per_algorithm": per_alg, "kernels": kernels, "extras": extras,
     }
     print(json.dumps(line))
