@@ -151,11 +151,19 @@ def extra_batch(gg, workloads, torch, np, scale):
         g.sssp(int(src[0]))
         g.pagerank(teleport=np.eye(1, c.n, int(seeds[0]), dtype=np.float32)[0], tol_l1=1e-6)
         res = {}
+        # Results stay on the device for both the batched and the plain runs.
+        dist_dev = torch.empty((len(src), c.n), dtype=torch.int32, device="cuda")
+        rank_dev = torch.empty((len(seeds), c.n), dtype=torch.float32, device="cuda")
         for label, s_list, p_list in (("sssp128", src, ()), ("ppr64", (), seeds), ("both", src, seeds)):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            _, _, st = g.batch(sssp_sources=s_list, ppr_seeds=p_list, tol_l1=1e-6)
+            _, _, st = g.batch(sssp_sources=s_list, ppr_seeds=p_list, tol_l1=1e-6, flags=gg._abi.GG_OUT_DEVICE,
+                               dist_out=dist_dev, rank_out=rank_dev)
             res[label] = (time.perf_counter() - t0) * 1e3
+            if label == "sssp128":
+                out["sssp128_union_edges"] = st["edges_inspected"]
+                out["sssp128_iterations"] = st["iterations"]
+        del dist_dev, rank_dev
         dev_i = torch.empty(c.n, dtype=torch.int32, device="cuda")
         dev_f = torch.empty(c.n, dtype=torch.float32, device="cuda")
         torch.cuda.synchronize()

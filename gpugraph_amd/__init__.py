@@ -282,18 +282,30 @@ class Graph:
             _abi.check(self.lib.gg_graph_update_values(self.h, _ptr(v), 0), "gg_graph_update_values")
 
     def batch(self, sssp_sources=(), ppr_seeds=(), damping: float = 0.85, tol_l1: float = 1e-6,
-              max_iter: int = 1000, flags: int = 0):
+              max_iter: int = 1000, flags: int = 0, dist_out=None, rank_out=None):
+        """Batched SSSP / personalised PageRank (gg_batch_run).  Results are
+        instance-major: dist[i][v], rank[j][v].  With GG_OUT_DEVICE in flags,
+        dist_out / rank_out are device buffers (contiguous tensors or raw
+        addresses) of len(sources) x n int32 / len(seeds) x n float32, and are
+        returned in place of host arrays."""
         src = np.ascontiguousarray(sssp_sources, dtype=np.int32)
         seeds = np.ascontiguousarray(ppr_seeds, dtype=np.int32)
-        dist = np.empty((len(src), self.n), np.int32)
-        rank = np.empty((len(seeds), self.n), np.float32)
         spec = _abi.BatchSpec()
+        if flags & _abi.GG_OUT_DEVICE:
+            if (len(src) and dist_out is None) or (len(seeds) and rank_out is None):
+                raise ValueError("GG_OUT_DEVICE needs dist_out / rank_out device buffers")
+            dist, rank = dist_out, rank_out
+            spec.sssp_dist = C.cast(C.c_void_p(_dev_ptr(dist_out) if len(src) else 0), _abi._I32P)
+            spec.ppr_rank = C.cast(C.c_void_p(_dev_ptr(rank_out) if len(seeds) else 0), _abi._F32P)
+        else:
+            dist = np.empty((len(src), self.n), np.int32)
+            rank = np.empty((len(seeds), self.n), np.float32)
+            spec.sssp_dist = dist.ctypes.data_as(_abi._I32P)
+            spec.ppr_rank = rank.ctypes.data_as(_abi._F32P)
         spec.num_sssp = len(src)
         spec.sssp_sources = src.ctypes.data_as(_abi._I32P)
-        spec.sssp_dist = dist.ctypes.data_as(_abi._I32P)
         spec.num_ppr = len(seeds)
         spec.ppr_seeds = seeds.ctypes.data_as(_abi._I32P)
-        spec.ppr_rank = rank.ctypes.data_as(_abi._F32P)
         spec.damping = damping
         spec.tol_l1 = tol_l1
         spec.max_iter = max_iter

@@ -13,12 +13,13 @@
 //     k_bs_count   degrees of the queue (workgroup scan; clears the queue's
 //                  bits in the union bitmap)
 //     k_lbs_scan   partials -> offsets, total edges
-//     k_bs_relax   one wave per chunk of kChunk edges (walks the queue's
-//                  offsets, so high-degree vertices spread over many waves):
+//     k_bs_partition  merge-path tile boundaries (items + edges)
+//     k_bs_relax   per tile, each wave relaxes a contiguous quarter of its
+//                  edges for all instances, four edges' row loads in flight:
 //                  atomicMin on dist[v][i] after a plain pre-check, improved
-//                  instances OR-ed into the next active mask (two atomics per
-//                  wave per edge at most), new vertices staged in LDS and
-//                  appended with one atomic per workgroup step
+//                  instances OR-ed into the next active mask (at most four
+//                  atomics per wave per edge), new vertices staged in LDS and
+//                  appended with one atomic per tile
 //     k_bs_clear   clears the active masks just consumed
 //     k_bs_swap    one thread: swaps queue / bitmap / mask buffers, counts,
 //                  sets done when the next queue is empty
@@ -51,10 +52,8 @@ using namespace GASengine;
 constexpr int32_t kInf = 100000000;
 constexpr int kSsspMax = 128;
 constexpr int kPprMax = 64;
-constexpr int kChunk = 16;        // edges per wave task in k_bs_relax
 constexpr int kLongRow = 512;     // PPR rows longer than this are chunked
 constexpr int kIters = 8;         // iterations per graph replay
-constexpr int kRelaxBlock = 256;
 
 struct BatchState {
   // SSSP part
@@ -142,91 +141,165 @@ k_bs_scan(BatchState *st, int32_t *__restrict__ partials, LbsInfo *__restrict__ 
   }
 }
 
-__global__ void __launch_bounds__(kRelaxBlock)
-k_bs_relax(BatchState *st, const int32_t *__restrict__ ro, const int32_t *__restrict__ col,
-           const int32_t *__restrict__ wt, int32_t *__restrict__ dist, const int32_t *__restrict__ scan_local,
-           const int32_t *__restrict__ partials, const LbsInfo *__restrict__ info) {
-  constexpr int W = kRelaxBlock / kWave;
-  __shared__ int32_t s_q[W * kChunk];
-  __shared__ int32_t s_n, s_base;
+// Merge-path tile boundaries of the current queue (k_lbs_partition with the
+// queue size taken from the state block).
+__global__ void __launch_bounds__(256)
+k_bs_partition(BatchState *st, const int32_t *__restrict__ scan_local, const int32_t *__restrict__ partials,
+               const LbsInfo *__restrict__ info, int32_t *__restrict__ parts) {
   if (st->sssp_done) return;
   const int32_t nf = st->cur_size;
-  const int32_t E = info->edges;
-  const int32_t *q = st->q_cur;
+  const int32_t ne = info->edges;
+  const int64_t path = info->path;
+  const int64_t ntiles = (path + kLbsTile - 1) / kLbsTile;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t <= ntiles; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t d = t * kLbsTile < path ? t * kLbsTile : path;
+    parts[t] = lbs_path_search(scan_local, partials, d, nf, ne);
+  }
+}
+
+// Relax one edge (u -> v, w) for every instance: lane l owns instances l and
+// l + 64.  m0/m1: u's active words for this lane; du0/du1: u's distances;
+// dv0/dv1: v's distances as pre-read.  Returns true when v must join the
+// next frontier (some instance improved); ORs the improved instances into
+// v's next mask (two words per improved half, one lane each).
+__device__ __forceinline__ bool relax_row(int32_t v, int32_t w, uint32_t m0, uint32_t m1, int32_t du0, int32_t du1,
+                                          int32_t dv0, int32_t dv1, int32_t *dist, uint32_t *act_nxt) {
+  const int lane = lane_id();
+  const int64_t vrow = (int64_t)v * kSsspMax;
+  bool imp0 = false, imp1 = false;
+  if ((m0 >> (lane & 31)) & 1u) {
+    const int64_t nd = (int64_t)du0 + w;
+    if (nd < kInf && nd < dv0) imp0 = (int32_t)nd < atomicMin(&dist[vrow + lane], (int32_t)nd);
+  }
+  if ((m1 >> (lane & 31)) & 1u) {
+    const int64_t nd = (int64_t)du1 + w;
+    if (nd < kInf && nd < dv1) imp1 = (int32_t)nd < atomicMin(&dist[vrow + 64 + lane], (int32_t)nd);
+  }
+  const uint64_t b0 = ballot(imp0), b1 = ballot(imp1);
+  if (!(b0 | b1)) return false;
+  const uint32_t word = lane == 0 ? (uint32_t)b0 : lane == 1 ? (uint32_t)(b0 >> 32)
+                      : lane == 2 ? (uint32_t)b1 : lane == 3 ? (uint32_t)(b1 >> 32) : 0u;
+  if (lane < 4 && word) atomicOr(&act_nxt[(int64_t)v * 4 + lane], word);
+  return true;
+}
+
+// Tile-based relax: the merge path of (queue items, edges) in tiles of
+// kLbsTile (same partition as k_lbs_expand); each wave takes a contiguous
+// quarter of the tile's edges and relaxes them for all instances, four edges'
+// loads in flight at a time.  New frontier vertices are staged in LDS and
+// appended with one atomic per tile.
+__global__ void __launch_bounds__(kExpandBlock)
+k_bs_relax(BatchState *st, const int32_t *__restrict__ ro, const int32_t *__restrict__ col,
+           const int32_t *__restrict__ wt, int32_t *__restrict__ dist, const int32_t *__restrict__ scan_local,
+           const int32_t *__restrict__ partials, const LbsInfo *__restrict__ info, const int32_t *__restrict__ parts) {
+  constexpr int TILE = kLbsTile;
+  constexpr int VT = kExpandVt;
+  constexpr int W = kExpandBlock / kWave;
+  __shared__ int32_t s_offs[TILE + 1];
+  __shared__ int32_t s_vert[TILE + 1];
+  __shared__ int32_t s_begin[TILE + 1];
+  __shared__ int32_t s_slot[TILE];
+  __shared__ int32_t s_out[TILE];
+  __shared__ int32_t s_nout, s_obase;
+  if (st->sssp_done) return;
+  const int32_t *queue = st->q_cur;
   const uint32_t *act = st->act_cur;
   uint32_t *act_nxt = st->act_nxt;
   uint32_t *ub_nxt = st->ub_nxt;
   const int lane = lane_id(), wid = threadIdx.x / kWave;
-  const int64_t tasks = ((int64_t)E + kChunk - 1) / kChunk;
-  for (int64_t t0 = (int64_t)blockIdx.x * W; t0 < tasks; t0 += (int64_t)gridDim.x * W) {
-    if (threadIdx.x == 0) s_n = 0;
-    __syncthreads();
-    const int64_t task = t0 + wid;
-    if (task < tasks) {
-      const int32_t e0 = (int32_t)(task * kChunk);
-      const int32_t e1 = e0 + kChunk < E ? e0 + kChunk : E;
-      // last queue item i with offset(i) <= e0
-      int32_t lo = 0, hi = nf;
-      while (hi - lo > 1) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (lbs_offset(scan_local, partials, mid) <= e0)
-          lo = mid;
-        else
-          hi = mid;
+  const int64_t path = info->path;
+  for (int64_t t = blockIdx.x; t * TILE < path; t += gridDim.x) {
+    const int64_t d0 = t * TILE;
+    const int64_t d1 = d0 + TILE < path ? d0 + TILE : path;
+    const int32_t i0 = parts[t], i1 = parts[t + 1];
+    const int32_t j0 = (int32_t)(d0 - i0), j1 = (int32_t)(d1 - i1);
+    const int32_t ni = i1 - i0, ne = j1 - j0;
+    for (int k = threadIdx.x; k <= ni; k += kExpandBlock) {
+      const int32_t i = i0 - 1 + k;
+      if (i >= 0) {
+        const int32_t v = queue[i];
+        s_vert[k] = v;
+        s_offs[k] = lbs_offset(scan_local, partials, i);
+        s_begin[k] = ro[v];
+      } else {
+        s_vert[k] = 0;
+        s_offs[k] = 0;
+        s_begin[k] = 0;
       }
-      int32_t i = lo;
-      int32_t next_off = i + 1 < nf ? lbs_offset(scan_local, partials, i + 1) : E;
-      int32_t u = -1, ubase = 0, uoff = 0;
-      int32_t du0 = kInf, du1 = kInf;
-      bool a0 = false, a1 = false;
-      for (int32_t e = e0; e < e1; e++) {
-        while (e >= next_off) {  // skip to the item owning edge e (zero-degree items included)
-          i++;
-          next_off = i + 1 < nf ? lbs_offset(scan_local, partials, i + 1) : E;
-          u = -1;
+    }
+    if (threadIdx.x == 0) s_nout = 0;
+    __syncthreads();
+    {
+      const int32_t dl = (int32_t)threadIdx.x * VT;
+      if (dl < ni + ne) {
+        int32_t lo = dl - ne > 0 ? dl - ne : 0, hi = dl < ni ? dl : ni;
+        while (lo < hi) {
+          int32_t mid = (lo + hi) >> 1;
+          if (s_offs[mid + 1] <= j0 + (dl - mid) - 1)
+            lo = mid + 1;
+          else
+            hi = mid;
         }
-        if (u < 0) {
-          u = q[i];
-          ubase = ro[u];
-          uoff = lbs_offset(scan_local, partials, i);
-          const int64_t row = (int64_t)u * kSsspMax;
-          const uint32_t m0 = act[(int64_t)u * 4 + (lane >> 5)], m1 = act[(int64_t)u * 4 + 2 + (lane >> 5)];
-          a0 = (m0 >> (lane & 31)) & 1u;
-          a1 = (m1 >> (lane & 31)) & 1u;
-          du0 = a0 ? dist[row + lane] : kInf;
-          du1 = a1 ? dist[row + 64 + lane] : kInf;
-        }
-        const int32_t slot = ubase + (e - uoff);
-        const int32_t v = col[slot];
-        const int32_t w = wt[slot];
-        const int64_t vrow = (int64_t)v * kSsspMax;
-        bool imp0 = false, imp1 = false;
-        if (a0) {
-          const int64_t nd = (int64_t)du0 + w;
-          if (nd < kInf && nd < __atomic_load_n(&dist[vrow + lane], __ATOMIC_RELAXED))
-            imp0 = (int32_t)nd < atomicMin(&dist[vrow + lane], (int32_t)nd);
-        }
-        if (a1) {
-          const int64_t nd = (int64_t)du1 + w;
-          if (nd < kInf && nd < __atomic_load_n(&dist[vrow + 64 + lane], __ATOMIC_RELAXED))
-            imp1 = (int32_t)nd < atomicMin(&dist[vrow + 64 + lane], (int32_t)nd);
-        }
-        const uint64_t b0 = ballot(imp0), b1 = ballot(imp1);
-        if (b0 | b1) {
-          if (lane == 0 && (uint32_t)b0) atomicOr(&act_nxt[v * 4LL + 0], (uint32_t)b0);
-          if (lane == 1 && (uint32_t)(b0 >> 32)) atomicOr(&act_nxt[v * 4LL + 1], (uint32_t)(b0 >> 32));
-          if (lane == 2 && (uint32_t)b1) atomicOr(&act_nxt[v * 4LL + 2], (uint32_t)b1);
-          if (lane == 3 && (uint32_t)(b1 >> 32)) atomicOr(&act_nxt[v * 4LL + 3], (uint32_t)(b1 >> 32));
-          if (lane == 0 && bit_test_and_set(ub_nxt, v)) s_q[atomicAdd(&s_n, 1)] = v;
+        int32_t li = lo, lj = dl - lo;
+#pragma unroll
+        for (int k = 0; k < VT; k++) {
+          if (li + lj >= ni + ne) break;
+          if (li < ni && s_offs[li + 1] <= j0 + lj) {
+            li++;
+          } else {
+            s_slot[lj] = li;
+            lj++;
+          }
         }
       }
     }
     __syncthreads();
-    const int32_t cnt = s_n;
-    if (cnt) {
-      if (threadIdx.x == 0) s_base = atomicAdd(&st->nxt_size, cnt);
+    const int32_t per = (ne + W - 1) / W;
+    const int32_t kb = wid * per, ke = min(ne, kb + per);
+    for (int32_t k0 = kb; k0 < ke; k0 += 4) {
+      int32_t vv[4], ww[4], uu[4], du0[4], du1[4], dv0[4], dv1[4];
+      uint32_t m0[4], m1[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int32_t k = min(k0 + q, ke - 1);  // tail: repeat the last edge, masked below
+        const int32_t s = s_slot[k];
+        const int32_t u = s_vert[s];
+        const int32_t slot = s_begin[s] + (j0 + k - s_offs[s]);
+        vv[q] = col[slot];
+        ww[q] = wt[slot];
+        uu[q] = u;
+        m0[q] = act[(int64_t)u * 4 + (lane >> 5)];
+        m1[q] = act[(int64_t)u * 4 + 2 + (lane >> 5)];
+      }
+      // Row loads only for active instances: a sparse mask touches a few
+      // cache lines instead of both 256 B halves of two rows.
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int64_t urow = (int64_t)uu[q] * kSsspMax, vrow = (int64_t)vv[q] * kSsspMax;
+        du0[q] = dv0[q] = du1[q] = dv1[q] = kInf;
+        if ((m0[q] >> (lane & 31)) & 1u) {
+          du0[q] = dist[urow + lane];
+          dv0[q] = __atomic_load_n(&dist[vrow + lane], __ATOMIC_RELAXED);
+        }
+        if ((m1[q] >> (lane & 31)) & 1u) {
+          du1[q] = dist[urow + 64 + lane];
+          dv1[q] = __atomic_load_n(&dist[vrow + 64 + lane], __ATOMIC_RELAXED);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (k0 + q >= ke) break;
+        if (relax_row(vv[q], ww[q], m0[q], m1[q], du0[q], du1[q], dv0[q], dv1[q], dist, act_nxt) && lane == 0 &&
+            bit_test_and_set(ub_nxt, vv[q]))
+          s_out[atomicAdd(&s_nout, 1)] = vv[q];
+      }
+    }
+    __syncthreads();
+    const int32_t nout = s_nout;
+    if (nout) {
+      if (threadIdx.x == 0) s_obase = atomicAdd(&st->nxt_size, nout);
       __syncthreads();
-      for (int k = threadIdx.x; k < cnt; k += kRelaxBlock) st->q_nxt[s_base + k] = s_q[k];
+      for (int32_t k = threadIdx.x; k < nout; k += kExpandBlock) st->q_nxt[s_obase + k] = s_out[k];
     }
     __syncthreads();
   }
@@ -263,12 +336,22 @@ __global__ void k_bs_swap(BatchState *st, const LbsInfo *info, int32_t max_iter)
   if (st->cur_size == 0 || st->sssp_iters >= max_iter) st->sssp_done = 1;
 }
 
-// dist[v][i] -> out[i][v] for i < k.
-__global__ void k_bs_transpose(const int32_t *__restrict__ dist, int32_t n, int32_t k, int32_t *__restrict__ out) {
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)n * k;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = idx / n, v = idx % n;
-    out[idx] = dist[v * kSsspMax + i];
+// Vertex-major rows -> instance-major output: in[v][KW] -> out[i][v] for
+// i < k.  A workgroup moves 64 vertices through LDS (row-contiguous reads,
+// 256 B per instance writes; the +1 pad keeps both LDS passes conflict-free).
+template <typename T, int KW>
+__global__ void __launch_bounds__(256)
+k_rows_to_instances(const T *__restrict__ in, int32_t n, int32_t k, T *__restrict__ out) {
+  __shared__ T tile[64][KW + 1];
+  for (int64_t v0 = (int64_t)blockIdx.x * 64; v0 < n; v0 += (int64_t)gridDim.x * 64) {
+    const int nv = n - v0 < 64 ? (int)(n - v0) : 64;
+    for (int idx = threadIdx.x; idx < nv * KW; idx += 256) tile[idx / KW][idx % KW] = in[v0 * KW + idx];
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < k * 64; idx += 256) {
+      const int i = idx / 64, v = idx % 64;
+      if (v < nv) out[(int64_t)i * n + v0 + v] = tile[v][i];
+    }
+    __syncthreads();
   }
 }
 
@@ -425,14 +508,6 @@ __global__ void k_ppr_init(PprArgs a, BatchState *st, float *r0) {
   }
 }
 
-__global__ void k_ppr_transpose(const float *__restrict__ r, int32_t n, int32_t k, float *__restrict__ out) {
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)n * k;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = idx / n, v = idx % n;
-    out[idx] = r[v * kPprMax + i];
-  }
-}
-
 __global__ void k_bs_state_init(BatchState *st, int32_t *q0, int32_t *q1, uint32_t *ub0, uint32_t *ub1,
                                 uint32_t *a0, uint32_t *a1, float *r0, float *r1, int32_t sssp_on, int32_t ppr_on) {
   st->q_cur = q0;
@@ -561,8 +636,11 @@ static int batch_pass(gg_graph *g, const int32_t *h_src, int32_t ns, int32_t *di
       hipLaunchKernelGGL(k_bs_count, dim3(grid_for(n, kCountTile, 2048)), dim3(kCountBlock), 0, s, st,
                          g->dg.row_offsets, lb.scan_local, lb.partials);
       hipLaunchKernelGGL(k_bs_scan, dim3(1), dim3(kScanBlock), 0, s, st, lb.partials, lb.info);
-      hipLaunchKernelGGL(k_bs_relax, dim3(2048), dim3(kRelaxBlock), 0, s, st, g->dg.row_offsets, g->dg.column_indices,
-                         static_cast<const int32_t *>(g->dg.edge_values), dist, lb.scan_local, lb.partials, lb.info);
+      hipLaunchKernelGGL(k_bs_partition, dim3(grid_for(((int64_t)n + m) / kLbsTile + 2, 256, 4096)), dim3(256), 0, s,
+                         st, lb.scan_local, lb.partials, lb.info, lb.parts);
+      hipLaunchKernelGGL(k_bs_relax, dim3(lb.expand_grid), dim3(kExpandBlock), 0, s, st, g->dg.row_offsets,
+                         g->dg.column_indices, static_cast<const int32_t *>(g->dg.edge_values), dist, lb.scan_local,
+                         lb.partials, lb.info, lb.parts);
       hipLaunchKernelGGL(k_bs_clear, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, st);
       hipLaunchKernelGGL(k_bs_swap, dim3(1), dim3(1), 0, s, st, lb.info, INT_MAX);  // SSSP runs to quiescence
     }
@@ -614,7 +692,8 @@ static int batch_pass(gg_graph *g, const int32_t *h_src, int32_t ns, int32_t *di
         GG_TRY(g->vtx[2].ensure(4 * (size_t)n * ns));
         o = g->vtx[2].as<int32_t>();
       }
-      hipLaunchKernelGGL(k_bs_transpose, dim3(grid_for((int64_t)n * ns, 256, 16384)), dim3(256), 0, s, dist, n, ns, o);
+      hipLaunchKernelGGL((k_rows_to_instances<int32_t, kSsspMax>), dim3(grid_for(n, 64, 16384)), dim3(256), 0, s,
+                         dist, n, ns, o);
       GG_HIP(hipGetLastError());
       if (!(flags & GG_OUT_DEVICE))
         GG_HIP(hipMemcpyAsync(dist_out, o, 4 * (size_t)n * ns, hipMemcpyDeviceToHost, s));
@@ -626,7 +705,8 @@ static int batch_pass(gg_graph *g, const int32_t *h_src, int32_t ns, int32_t *di
         GG_TRY(g->vtx[1].ensure(4 * (size_t)n * np));
         o = g->vtx[1].as<float>();
       }
-      hipLaunchKernelGGL(k_ppr_transpose, dim3(grid_for((int64_t)n * np, 256, 16384)), dim3(256), 0, s, Rf, n, np, o);
+      hipLaunchKernelGGL((k_rows_to_instances<float, kPprMax>), dim3(grid_for(n, 64, 16384)), dim3(256), 0, s, Rf,
+                         n, np, o);
       GG_HIP(hipGetLastError());
       if (!(flags & GG_OUT_DEVICE))
         GG_HIP(hipMemcpyAsync(rank_out, o, 4 * (size_t)n * np, hipMemcpyDeviceToHost, s));

@@ -80,6 +80,35 @@ def test_mixed_batch_and_weighted_ppr(gpu_available):
         assert l1(rank[i], oracle.pagerank_f64(cf.row_offsets, cf.col_indices, p=p, t=t, tol=1e-14)[0]) <= 1e-6
 
 
+def test_batch_device_outputs(gpu_available, rmat_directed):
+    """GG_OUT_DEVICE: instance-major results written into device buffers equal
+    the host-output run (n not a multiple of the 64-vertex transpose tile)."""
+    import torch
+    c = gg.gen_rmat(11, 8, undirected=False, seed=61).with_weights("i32", 1, 64, symmetric=False, seed=62)
+    c = gg.CSR(*_trim(c, c.n - 5))
+    src = np.array([0, 3, 17, c.n - 1], np.int32)
+    seeds = np.array([2, 5, c.n - 2], np.int32)
+    with gg.Graph(c, directed=True) as g:
+        dh, rh, _ = g.batch(sssp_sources=src, ppr_seeds=seeds, tol_l1=2e-7)
+        dd = torch.full((len(src), c.n), -7, dtype=torch.int32, device="cuda")
+        rd = torch.full((len(seeds), c.n), -7.0, dtype=torch.float32, device="cuda")
+        g.batch(sssp_sources=src, ppr_seeds=seeds, tol_l1=2e-7, flags=gg._abi.GG_OUT_DEVICE, dist_out=dd, rank_out=rd)
+        g.synchronize()
+    np.testing.assert_array_equal(dd.cpu().numpy(), dh)
+    np.testing.assert_array_equal(rd.cpu().numpy(), rh)
+    np.testing.assert_array_equal(dh[3], oracle.sssp(c.row_offsets, c.col_indices, c.values, int(src[3])))
+
+
+def _trim(c, n):
+    """Induced subgraph on vertices [0, n): (row_offsets, col_indices, values)."""
+    keep = c.col_indices < n
+    rows = np.repeat(np.arange(c.n), np.diff(c.row_offsets))
+    sel = keep & (rows < n)
+    deg = np.bincount(rows[sel], minlength=n)[:n]
+    ro = np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
+    return ro, c.col_indices[sel].astype(np.int32), c.values[sel]
+
+
 def test_batch_errors(gpu_available):
     c = workloads.config1_er()
     with gg.Graph(c) as g:
