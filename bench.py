@@ -6,9 +6,12 @@ BFS and one SSSP from the source, graph resident in HBM and outputs kept on
 the device; value = GTEPS over the timed steps, counting the sum of CSR
 out-degrees of the reached vertices per traversal (the reference's counting,
 Algorithms/BFS/bfs.cu:963-980); the undirected-edge count is half of it.
-Kernel timings come from HIP events recorded on the library's stream inside
-the timed region (GG_INSTRUMENT); `roofline` is that of the dominant kernel
-class.  `cpu_baseline` is the oracle (single-threaded C restatement of the
+The graph's options are first picked by gg_autotune (BFS + SSSP, outside
+the timed region; --no-tune keeps the defaults).  `value` comes from a timed
+region without per-kernel events; a second timed pass of the same steps
+records HIP events around every kernel on the library's stream
+(GG_INSTRUMENT), and `roofline` is that of the dominant kernel class, with
+its frontier-heavy launches (>= 1% of m edges) broken out.  `cpu_baseline` is the oracle (single-threaded C restatement of the
 reference's CPUBFS and, for SSSP, a heap Dijkstra with CPUSSSP's output) on
 the same graph and source.
 
@@ -61,6 +64,15 @@ def headline(args, gg, _abi, workloads, torch, np):
     g = gg.Graph(csr)
     log(f"graph resident on device in {time.perf_counter() - t0:.1f}s (H2D + validation)")
     flags = _abi.GG_OUT_DEVICE
+    tuned = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        best, trials = g.autotune(programs=_abi.GG_TUNE_BFS | _abi.GG_TUNE_SSSP, src=src, reps=3)
+        defaults = gg.default_options().as_dict()
+        tuned = {"seconds": round(time.perf_counter() - t0, 2), "trials": len(trials),
+                 "baseline_ms": round(trials[0][3], 3), "tuned_ms": None,
+                 "changed": {k: v for k, v in best.items() if v != defaults[k]}}
+        log(f"autotune: {tuned}")
 
     def step(instrument):
         f = flags | (_abi.GG_INSTRUMENT if instrument else 0)
@@ -70,19 +82,30 @@ def headline(args, gg, _abi, workloads, torch, np):
 
     for _ in range(args.warmup):
         step(False)
-    torch.cuda.synchronize()
-    g.profile_reset()
+    # Timed region for `value`: no per-kernel events (they add a marker per
+    # launch, which costs BFS about a quarter of its time).
     sb_list, ss_list = [], []
+    g.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        sb, ss = step(True)
+        sb, ss = step(False)
         sb_list.append(sb)
         ss_list.append(ss)
     g.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # Second timed pass of the same steps with HIP events around every kernel
+    # on the library's stream: the per-kernel durations behind `roofline`.
+    g.profile_reset()
+    g.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    g.synchronize()
+    wall_instrumented = time.perf_counter() - t1
     prof = g.profile()
+    launches = g.profile_launches()
     g.close()
     re_bfs = sb_list[-1]["reached_edges"]
     re_sssp = ss_list[-1]["reached_edges"]
@@ -91,18 +114,35 @@ def headline(args, gg, _abi, workloads, torch, np):
     sssp_ms = float(np.median([s["gpu_ms"] for s in ss_list]))
     log(f"timed {args.steps} steps in {wall * 1e3:.1f} ms: {value:.2f} GTEPS "
         f"(bfs {bfs_ms:.3f} ms, sssp {sssp_ms:.3f} ms median)")
+    if tuned is not None:
+        tuned["tuned_ms"] = round(bfs_ms + sssp_ms, 3)
     name, ent = max(prof.items(), key=lambda kv: kv[1]["total_ms"])
     achieved = ent["alg_bytes"] / (ent["total_ms"] * 1e-3) / 1e9 if ent["total_ms"] > 0 else 0.0
     roofline = {
         "bound": "hbm", "kernel": name, "launches": ent["launches"],
         "avg_launch_us": ent["total_ms"] * 1e3 / max(ent["launches"], 1),
         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": None, "pass": f"second timed pass of {args.steps} steps with per-kernel HIP events "
+                                   f"({wall_instrumented * 1e3:.1f} ms vs {wall * 1e3:.1f} ms without)",
     }
+    # Frontier-heavy launches: those expanding >= 1% of m edges (by each
+    # kernel's per-edge byte model: push 12 B, top-down 8 B, bottom-up 4 B).
+    per_edge = {"sssp.push": 12.0, "bfs.top_down": 8.0, "bfs.bottom_up": 4.0}
+    heavy = {}
+    for kname, bpe in per_edge.items():
+        ls = [(ms, b) for nm, ms, b in launches if nm == kname and b >= 0.01 * csr.m * bpe and ms > 0]
+        if ls:
+            tb, tm = sum(b for _, b in ls), sum(ms for ms, _ in ls)
+            best = max(b / (ms * 1e-3) / 1e9 for ms, b in ls)
+            heavy[kname] = {"launches": len(ls), "achieved_GBps": round(tb / (tm * 1e-3) / 1e9, 1),
+                            "frac": round(tb / (tm * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "best_launch_GBps": round(best, 1)}
+    if name in heavy:
+        roofline["frontier_heavy"] = heavy[name]
     kernels = {k: {"launches": v["launches"], "ms": round(v["total_ms"], 4),
                    "GBps": round(v["alg_bytes"] / (v["total_ms"] * 1e-3) / 1e9, 1) if v["total_ms"] > 0 else None}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["total_ms"])}
     per_alg = {
+        "frontier_heavy": heavy,
         "bfs": {"gteps": re_bfs / (bfs_ms * 1e-3) / 1e9, "gteps_undirected": re_bfs / 2 / (bfs_ms * 1e-3) / 1e9,
                 "ms": bfs_ms, "levels": sb_list[-1]["iterations"], "pull_levels": sb_list[-1]["pull_iterations"],
                 "edges_inspected": sb_list[-1]["edges_inspected"],
@@ -127,7 +167,10 @@ def headline(args, gg, _abi, workloads, torch, np):
         log(f"cpu baseline: {cpu}")
     config = {"workload": f"rmat{args.scale}_undirected_bfs+sssp", "n": csr.n, "m_csr": csr.m, "edge_factor": 16,
               "weights": "int32 uniform [1,64]", "source": src, "step": "1 BFS + 1 SSSP",
-              "teps_counting": "sum of CSR degree over reached vertices"}
+              "teps_counting": "sum of CSR degree over reached vertices",
+              "options": "autotuned (gg_autotune, BFS + SSSP) before timing" if tuned else "defaults"}
+    if tuned is not None:
+        per_alg["autotune"] = tuned
     return value, wall, roofline, kernels, per_alg, cpu, config
 
 
@@ -199,6 +242,11 @@ def extra_pagerank(gg, workloads, np, scale, full_checks):
         ams, ast = median_ms(lambda: g.pagerank_adjoint(r, cvec, tol_l1=2e-7)[1:], 3)
         out["adjoint"] = {"ms": ams, "sweeps": ast["iterations"],
                           "hbm_frac": ast["alg_bytes"] / (ams * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        # The same solves with XCD-sliced sweeps (opt-in: slower on this graph).
+        g.set_options(xcd_slices=1)
+        ms1, st1 = median_ms(lambda: g.pagerank(tol_l1=2e-7), 3)
+        ams1, _ = median_ms(lambda: g.pagerank_adjoint(r, cvec, tol_l1=2e-7)[1:], 3)
+        out["xcd_sliced"] = {"forward_ms": ms1, "forward_sweeps": st1["iterations"], "adjoint_ms": ams1}
     if full_checks:
         import oracle
 
@@ -232,6 +280,7 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
+    ap.add_argument("--no-tune", action="store_true", help="time the default options (no gg_autotune)")
     ap.add_argument("--full-checks", action="store_true", help="FP64 PageRank error at full size")
     ap.add_argument("--copy-probe", action="store_true", help="run only a 1 GiB device copy (PMC calibration)")
     args = ap.parse_args()

@@ -189,9 +189,39 @@ class Graph:
             setattr(o, k, v)
         _abi.check(self.lib.gg_graph_set_options(self.h, C.byref(o)), "gg_graph_set_options")
 
+    def autotune(self, programs: int | None = None, src: int = 0, reps: int = 3, max_trials: int = 256):
+        """Search the schedule options on this graph (gg_autotune) and install
+        the best.  programs: GG_TUNE_* bits (default: every program this
+        graph supports).  Returns (best options dict, trials) where trials is
+        a list of (programs, option name or "baseline", value, ms)."""
+        if programs is None:
+            programs = _abi.GG_TUNE_BFS | _abi.GG_TUNE_GAS | _abi.GG_TUNE_PAGERANK
+            if self.value_dtype == _abi.GG_VALUES_I32:
+                programs |= _abi.GG_TUNE_SSSP
+            if not self.directed:
+                programs |= _abi.GG_TUNE_CC
+        best = _abi.Options()
+        arr = (_abi.TuneTrial * max(max_trials, 1))()
+        cnt = C.c_int32()
+        _abi.check(self.lib.gg_autotune(self.h, programs, src, reps, C.byref(best), arr, max_trials, C.byref(cnt)),
+                   "gg_autotune")
+        names = [f for f, _ in _abi.Options._fields_]
+        label = {-1: "baseline", -2: "final"}
+        trials = [(t.programs, label[t.option] if t.option < 0 else names[t.option], t.value, float(t.ms))
+                  for t in arr[:cnt.value]]
+        return best.as_dict(), trials
+
     def set_stream(self, stream_ptr: int | None):
         _abi.check(self.lib.gg_graph_set_stream(self.h, C.c_void_p(stream_ptr) if stream_ptr else None),
                    "gg_graph_set_stream")
+
+    def profile_launches(self) -> list:
+        """Per-launch (name, ms, alg_bytes) of instrumented runs since the last reset."""
+        cnt = C.c_int32()
+        _abi.check(self.lib.gg_profile_launches(self.h, None, 0, C.byref(cnt)), "gg_profile_launches")
+        arr = (_abi.LaunchStat * max(cnt.value, 1))()
+        _abi.check(self.lib.gg_profile_launches(self.h, arr, cnt.value, C.byref(cnt)), "gg_profile_launches")
+        return [(a.name.decode(), a.ms, a.alg_bytes) for a in arr[:cnt.value]]
 
     def profile(self) -> dict:
         cnt = C.c_int32()
@@ -214,6 +244,16 @@ class Graph:
         out, ptr = self._out(out, np.int32, f)
         _abi.check(self.lib.gg_bfs(self.h, src, ptr, f, C.byref(st)), "gg_bfs")
         return out, st.as_dict()
+
+    def bfs_tree(self, src: int, push_only: bool = False, flags: int = 0, depth_out=None, pred_out=None):
+        """BFS depths and parents (gg_bfs_tree): pred[src] = src, pred[v] =
+        the smallest in-neighbour one level up, -1 where unreached."""
+        st = _abi.Stats()
+        f = flags | (_abi.GG_BFS_PUSH_ONLY if push_only else 0)
+        depth, dptr = self._out(depth_out, np.int32, f)
+        pred, pptr = self._out(pred_out, np.int32, f)
+        _abi.check(self.lib.gg_bfs_tree(self.h, src, dptr, pptr, f, C.byref(st)), "gg_bfs_tree")
+        return depth, pred, st.as_dict()
 
     def sssp(self, src: int, flags: int = 0, out=None):
         st = _abi.Stats()

@@ -21,6 +21,7 @@ GG_ERR_NOT_CONVERGED = 5
 GG_ERR_NO_DEVICE = 6
 GG_ERR_IO = 7
 GG_ERR_UNSUPPORTED = 8
+GG_ERR_INTERNAL = 9
 
 GG_VALUES_NONE = 0
 GG_VALUES_I32 = 1
@@ -35,6 +36,12 @@ GG_BFS_PUSH_ONLY = 4
 GG_CC_HOOK = 8
 GG_SSSP_GAS = 16
 
+GG_TUNE_BFS = 1
+GG_TUNE_SSSP = 2
+GG_TUNE_CC = 4
+GG_TUNE_GAS = 8
+GG_TUNE_PAGERANK = 16
+
 
 class Options(C.Structure):
     _fields_ = [
@@ -47,6 +54,24 @@ class Options(C.Structure):
         ("lbs_tile", C.c_int32),
         ("sssp_delta", C.c_int32),
         ("pull_alpha", C.c_int32),
+        ("expand_grid", C.c_int32),
+        ("xcd_slices", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class LaunchStat(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("ms", C.c_double), ("alg_bytes", C.c_double)]
+
+
+class TuneTrial(C.Structure):
+    _fields_ = [
+        ("programs", C.c_uint32),
+        ("option", C.c_int32),
+        ("value", C.c_int32),
+        ("ms", C.c_float),
     ]
 
 
@@ -118,8 +143,10 @@ PROTOTYPES = {
     "gg_graph_update_values": (C.c_int, [_P, _P, C.c_uint32]),
     "gg_graph_info": (C.c_int, [_P, _I32P, C.POINTER(C.c_int64), _I32P, _I32P]),
     "gg_graph_device_csr": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
+    "gg_graph_device_csc": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "gg_graph_synchronize": (C.c_int, [_P]),
     "gg_bfs": (C.c_int, [_P, C.c_int32, _P, C.c_uint32, C.POINTER(Stats)]),
+    "gg_bfs_tree": (C.c_int, [_P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(Stats)]),
     "gg_sssp": (C.c_int, [_P, C.c_int32, _P, C.c_uint32, C.POINTER(Stats)]),
     "gg_cc": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(Stats)]),
     "gg_pagerank_reference": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(Stats)]),
@@ -127,8 +154,11 @@ PROTOTYPES = {
     "gg_pagerank_adjoint": (C.c_int, [_P, C.c_float, _P, _P, C.c_double, C.c_int32, _P, _P, C.c_uint32,
                                       C.POINTER(Stats)]),
     "gg_batch_run": (C.c_int, [_P, C.POINTER(BatchSpec), C.c_uint32, C.POINTER(Stats)]),
+    "gg_autotune": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int32, C.POINTER(Options), C.POINTER(TuneTrial),
+                              C.c_int32, _I32P]),
     "gg_profile_reset": (C.c_int, [_P]),
     "gg_profile_get": (C.c_int, [_P, C.POINTER(KernelStat), C.c_int32, _I32P]),
+    "gg_profile_launches": (C.c_int, [_P, C.POINTER(LaunchStat), C.c_int32, _I32P]),
     "gg_csr_free": (None, [C.POINTER(Csr)]),
     "gg_read_mtx": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(Csr)]),
     "gg_gen_erdos_renyi": (C.c_int, [C.c_int32, C.c_int64, C.c_uint64, C.POINTER(Csr)]),

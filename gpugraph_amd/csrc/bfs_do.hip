@@ -262,7 +262,7 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
       GG_HIP(hipStreamSynchronize(s));
       const int64_t N = g->h64()[0], E = g->h64()[2], U = g->h64()[3];
       lvl_bytes = 8.0 * U + 4.0 * E + n / 8.0 + 8.0 * N;
-      g->prof.add_bytes("bfs.bottom_up", lvl_bytes);
+      g->prof.add_launch_bytes("bfs.bottom_up", lvl_bytes);
       examined_total += E;
       prev_nf = nf;
       nf = N;
@@ -277,8 +277,8 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
       const int64_t E = g->h64()[2];
       const int64_t N = g->h_pinned[(8 + nxt) * 2];
       lvl_bytes = 12.0 * nf + 8.0 * E + 8.0 * N;
-      g->prof.add_bytes("bfs.lbs_count", 12.0 * nf);
-      g->prof.add_bytes("bfs.top_down", 8.0 * E + 8.0 * N);
+      g->prof.add_launch_bytes("bfs.lbs_count", 12.0 * nf);
+      g->prof.add_launch_bytes("bfs.top_down", 8.0 * E + 8.0 * N);
       examined_total += E;
       prev_nf = nf;
       nf = N;

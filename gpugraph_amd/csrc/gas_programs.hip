@@ -105,6 +105,8 @@ int run_program(gg_graph *g, int num_srcs, int *srcs, typename P::DataType *out,
   GASengine::EnactorVertexCentric<Problem<P>, P> enactor;
   enactor.config.wave_degree = g->opt.wave_degree;
   enactor.config.block_degree = g->opt.block_degree;
+  enactor.config.expand_grid = g->opt.expand_grid;
+  enactor.config.expand_vt = g->opt.lbs_tile / GASengine::kExpandBlock;
   g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
   PrefixTimer timer(g->prof, Slot<P>::name);
   if (g->prof.enabled) enactor.timer = &timer;
@@ -150,6 +152,42 @@ int run_program(gg_graph *g, int num_srcs, int *srcs, typename P::DataType *out,
 
 }  // namespace gg
 
+namespace gg {
+namespace {
+
+// BFS tree parents from the depths: one wave per vertex scans its in-edges
+// (CSC, sources ascending) 64 at a time; the first source one level up is the
+// parent (the smallest such id).  pred[src] = src, -1 where unreached.
+__global__ void __launch_bounds__(256)
+k_bfs_parents(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, const int32_t *__restrict__ depth,
+              int32_t n, int32_t src, int32_t *__restrict__ pred) {
+  const int64_t waves = (int64_t)gridDim.x * (256 / GASengine::kWave);
+  const int lane = GASengine::lane_id();
+  for (int64_t v = (int64_t)blockIdx.x * (256 / GASengine::kWave) + threadIdx.x / GASengine::kWave; v < n;
+       v += waves) {
+    const int32_t dv = depth[v];
+    int32_t p = -1;
+    if (v == src) {
+      p = src;
+    } else if (dv > 0) {
+      const int32_t b = co[v], e = co[v + 1];
+      for (int32_t k0 = b; k0 < e; k0 += GASengine::kWave) {
+        const int32_t k = k0 + lane;
+        const int32_t u = k < e ? ri[k] : -1;
+        const uint64_t hit = GASengine::ballot(k < e && depth[u] == dv - 1);
+        if (hit) {
+          p = __shfl(u, __builtin_ctzll(hit));
+          break;
+        }
+      }
+    }
+    if (lane == 0) pred[v] = p;
+  }
+}
+
+}  // namespace
+}  // namespace gg
+
 // Declared in bfs_do.hip / cc_hook.hip.
 namespace gg {
 int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth, uint32_t flags, gg_stats *stats);
@@ -166,6 +204,37 @@ int gg_bfs(gg_graph *g, int32_t src, int32_t *depth, uint32_t flags, gg_stats *s
   if (!(flags & GG_BFS_PUSH_ONLY)) return gg::bfs_direction_optimizing(g, src, depth, flags, stats);
   int s = src;
   return gg::run_program<bfs>(g, 1, &s, depth, flags, stats, -1, true);
+}
+
+int gg_bfs_tree(gg_graph *g, int32_t src, int32_t *depth, int32_t *pred, uint32_t flags, gg_stats *stats) {
+  using namespace gg;
+  gg::clear_error();
+  GG_CHECK_ARG(g && pred, "graph/pred is NULL");
+  GG_CHECK_ARG(src >= 0 && src < g->n, "src %d out of range [0, %d)", src, g->n);
+  GG_TRY(activate(g));
+  const bool dev = flags & GG_OUT_DEVICE;
+  const size_t bytes = 4 * (size_t)g->n;
+  int32_t *d_depth = depth;
+  if (!dev || !depth) {
+    GG_TRY(g->tree[0].ensure(bytes));
+    d_depth = g->tree[0].as<int32_t>();
+  }
+  GG_TRY(gg_bfs(g, src, d_depth, flags | GG_OUT_DEVICE, stats));
+  int32_t *d_pred = pred;
+  if (!dev) {
+    GG_TRY(g->tree[1].ensure(bytes));
+    d_pred = g->tree[1].as<int32_t>();
+  }
+  {
+    TimedRegion t(g->prof, "bfs.parents");
+    hipLaunchKernelGGL(k_bfs_parents, dim3(GASengine::grid_for((int64_t)g->n * GASengine::kWave, 256, 16384)),
+                       dim3(256), 0, g->stream, g->dg.column_offsets, g->dg.row_indices, d_depth, g->n, src, d_pred);
+    GG_HIP(hipGetLastError());
+  }
+  if (!dev && depth) GG_TRY(copy_out(g, depth, d_depth, bytes, flags));
+  GG_TRY(copy_out(g, pred, d_pred, bytes, flags));
+  if (flags & GG_INSTRUMENT) finish_profile(g);
+  return GG_OK;
 }
 
 int gg_sssp(gg_graph *g, int32_t src, int32_t *dist, uint32_t flags, gg_stats *stats) {

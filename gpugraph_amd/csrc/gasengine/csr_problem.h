@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "Enums.h"
+#include "csc_build.h"
 #include "lbs.h"
 #include "program_ops.h"
 
@@ -116,24 +117,6 @@ struct Workspace {
   }
 };
 
-// Host CSC of a CSR by a stable counting sort on the column: within each
-// column the rows stay ascending (the reference's stable sort by column of
-// the (row, col)-sorted COO, b40c/graph/csr_graph.cuh:173-192), plus the
-// CSC->CSR edge-id permutation (util.cuh:55-126 builds it with thrust).
-inline void host_csr_to_csc(int32_t n, int64_t m, const int32_t *ro, const int32_t *col, int32_t *co, int32_t *ri,
-                            int32_t *eid) {
-  std::vector<int64_t> cnt((size_t)n + 1, 0);
-  for (int64_t e = 0; e < m; e++) cnt[(size_t)col[e] + 1]++;
-  for (int32_t v = 0; v < n; v++) cnt[(size_t)v + 1] += cnt[(size_t)v];
-  for (int32_t v = 0; v <= n; v++) co[v] = (int32_t)cnt[(size_t)v];
-  for (int32_t u = 0; u < n; u++)
-    for (int32_t e = ro[u]; e < ro[u + 1]; e++) {
-      int64_t pos = cnt[(size_t)col[e]]++;
-      ri[pos] = u;
-      if (eid) eid[pos] = e;
-    }
-}
-
 template <class Program, typename VertexId = int, typename SizeT = int,
           typename Value = typename Program::DataType, bool MARK_PREDECESSORS = false, bool WITH_VALUE = true>
 struct CsrProblem {
@@ -163,8 +146,9 @@ struct CsrProblem {
   }
 
   // Reference form (csr_problem.cuh:512-524, upstream 10-argument overload):
-  // copy a host CSR (and, for directed graphs, its CSC when given; built on
-  // the host otherwise) to the current device.
+  // copy a host CSR to the current device; for directed graphs the CSC and
+  // its edge-id permutation are built there (device_csr_to_csc).  A CSC the
+  // caller passes is not needed and not read.
   hipError_t FromHostProblem(bool /*stream_from_host*/, SizeT nodes, SizeT edges, VertexId *h_column_indices,
                              SizeT *h_row_offsets, Value *h_edge_values, VertexId *h_row_indices,
                              SizeT *h_column_offsets, int /*num_gpus*/, int directed) {
@@ -187,17 +171,23 @@ struct CsrProblem {
       GAS_RETURN_IF(up(h_edge_values, sizeof(Value) * (size_t)edges, &p));
       g.edge_values = p;
     }
+    (void)h_row_indices;
+    (void)h_column_offsets;
     if (directed) {
-      std::vector<int32_t> co((size_t)nodes + 1), ri((size_t)edges), eid((size_t)edges);
-      host_csr_to_csc(nodes, edges, h_row_offsets, h_column_indices, co.data(), ri.data(), eid.data());
-      (void)h_row_indices;
-      (void)h_column_offsets;
-      GAS_RETURN_IF(up(co.data(), 4 * co.size(), &p));
-      g.column_offsets = (const int32_t *)p;
-      GAS_RETURN_IF(up(ri.data(), 4 * ri.size(), &p));
-      g.row_indices = (const int32_t *)p;
-      GAS_RETURN_IF(up(eid.data(), 4 * eid.size(), &p));
-      g.csc_edge_ids = (const int32_t *)p;
+      auto alloc = [&](size_t bytes, void **dst) -> hipError_t {
+        GAS_RETURN_IF(hipMalloc(dst, bytes > 0 ? bytes : 4));
+        owned.push_back(*dst);
+        return hipSuccess;
+      };
+      void *co = nullptr, *ri = nullptr, *eid = nullptr;
+      GAS_RETURN_IF(alloc(4 * ((size_t)nodes + 1), &co));
+      GAS_RETURN_IF(alloc(4 * (size_t)edges, &ri));
+      GAS_RETURN_IF(alloc(4 * (size_t)edges, &eid));
+      GAS_RETURN_IF(device_csr_to_csc(nodes, edges, g.row_offsets, g.column_indices, (int32_t *)co, (int32_t *)ri,
+                                      (int32_t *)eid, stream));
+      g.column_offsets = (const int32_t *)co;
+      g.row_indices = (const int32_t *)ri;
+      g.csc_edge_ids = (const int32_t *)eid;
     } else {
       g.column_offsets = g.row_offsets;
       g.row_indices = g.column_indices;

@@ -43,6 +43,7 @@ struct EnactorConfig {
   int32_t wave_degree = 64;
   int32_t block_degree = 4096;
   int32_t expand_grid = 2048;  // persistent workgroups for k_lbs_expand
+  int32_t expand_vt = kExpandVt;  // LBS tile = kExpandBlock * expand_vt positions (2, 4 or 8)
 };
 
 // Per-iteration record: frontier size in, edges expanded, frontier size out.
@@ -143,7 +144,8 @@ class EnactorVertexCentric {
         const bool first = p == 0;
         if (lbs) {
           static const char *names[4] = {"lbs_count", "lbs_scan", "lbs_partition", "lbs_expand"};
-          LbsBuffers b{ws.scan_local, ws.partials, ws.info, ws.parts, ws.counters, config.expand_grid};
+          LbsBuffers b{ws.scan_local, ws.partials, ws.info, ws.parts, ws.counters, config.expand_grid,
+                       config.expand_vt};
           lbs_launch(op, first, f_in, sz_in, nf, (int64_t)n + g.edges, b, f_out, sz_out, s, [&](int phase) {
             if (phase > 0) te(names[phase - 1]);
             if (phase < 4) tb(names[phase]);

@@ -10,7 +10,8 @@
 //                       exclusive scan, one partial per 1024 items
 //      k_lbs_scan       one workgroup scans the partials -> global offsets
 //      k_lbs_partition  the merge path of (items + edges) is cut into tiles
-//                       of kLbsTile positions; one thread per tile boundary
+//                       of 512 / 1024 / 2048 positions (run-time choice);
+//                       one thread per tile boundary
 //                       finds its (item, edge) split by binary search
 //      k_lbs_expand     each workgroup stages its tile's items in LDS, builds
 //                       an edge->item map with a per-thread serial merge,
@@ -45,8 +46,13 @@ constexpr int kCountIpt = 4;
 constexpr int kCountTile = kCountBlock * kCountIpt;  // items per scan partial
 constexpr int kScanBlock = 1024;
 constexpr int kExpandBlock = 256;
-constexpr int kExpandVt = 4;  // 1024-position tiles: ~21 KB LDS, 7 workgroups per CU
-constexpr int kLbsTile = kExpandBlock * kExpandVt;  // merge-path positions per tile
+constexpr int kExpandVt = 4;  // default 1024-position tiles: ~21 KB LDS, 7 workgroups per CU
+constexpr int kLbsTile = kExpandBlock * kExpandVt;  // default merge-path positions per tile
+// Compiled tile shapes (positions per workgroup tile = kExpandBlock * VT);
+// lbs_launch picks one at run time (the tuner's `lbs_tile`).
+constexpr int kExpandVtMin = 2, kExpandVtMax = 8;
+constexpr int kLbsTileMin = kExpandBlock * kExpandVtMin;
+inline bool lbs_tile_supported(int tile) { return tile == 512 || tile == 1024 || tile == 2048; }
 
 // Device-side counters of one expansion (written by the kernels).
 struct LbsInfo {
@@ -55,8 +61,9 @@ struct LbsInfo {
   int64_t path;   // merge path length = items + edges
 };
 
-// Partition-array entries needed for a frontier of at most n items over m edges.
-inline size_t lbs_partition_entries(int64_t n, int64_t m) { return (size_t)((n + m) / kLbsTile + 2); }
+// Partition-array entries needed for a frontier of at most n items over m
+// edges, for every compiled tile shape.
+inline size_t lbs_partition_entries(int64_t n, int64_t m) { return (size_t)((n + m) / kLbsTileMin + 2); }
 
 template <class Op, bool PROLOGUE>
 __global__ void __launch_bounds__(kCountBlock)
@@ -133,29 +140,28 @@ __device__ __forceinline__ int32_t lbs_path_search(const int32_t *scan_local, co
   return (int32_t)lo;
 }
 
-// parts[t] = items consumed at diagonal min(t * kLbsTile, path), t = 0..ntiles.
-template <int DUMMY = 0>
+// parts[t] = items consumed at diagonal min(t * TILE, path), t = 0..ntiles.
+template <int TILE = kLbsTile>
 __global__ void __launch_bounds__(256)
 k_lbs_partition(const int32_t *__restrict__ fsize, const int32_t *__restrict__ scan_local,
                 const int32_t *__restrict__ partials, const LbsInfo *__restrict__ info, int32_t *__restrict__ parts) {
   const int32_t nf = *fsize;
   const int32_t ne = info->edges;
   const int64_t path = info->path;
-  const int64_t ntiles = (path + kLbsTile - 1) / kLbsTile;
+  const int64_t ntiles = (path + TILE - 1) / TILE;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t <= ntiles; t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t d = t * kLbsTile < path ? t * kLbsTile : path;
+    const int64_t d = t * TILE < path ? t * TILE : path;
     parts[t] = lbs_path_search(scan_local, partials, d, nf, ne);
   }
 }
 
-template <class Op>
+template <class Op, int VT = kExpandVt>
 __global__ void __launch_bounds__(kExpandBlock)
 k_lbs_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__restrict__ fsize,
              const int32_t *__restrict__ scan_local, const int32_t *__restrict__ partials,
              const LbsInfo *__restrict__ info, const int32_t *__restrict__ parts, int32_t *__restrict__ out_queue,
              int32_t *__restrict__ out_size) {
-  constexpr int TILE = kLbsTile;
-  constexpr int VT = kExpandVt;
+  constexpr int TILE = kExpandBlock * VT;
   // slot k <-> frontier item i0 - 1 + k (slot 0: the item owning the tile's first edges)
   __shared__ int32_t s_offs[TILE + 1];
   __shared__ int32_t s_vert[TILE + 1];
@@ -460,8 +466,21 @@ struct LbsBuffers {
   LbsInfo *info;
   int32_t *parts;
   unsigned long long *edge_counter;
-  int expand_grid;
+  int expand_grid;        // persistent workgroups of k_lbs_expand
+  int vt = kExpandVt;     // positions per thread: tile = kExpandBlock * vt (2, 4 or 8)
 };
+
+template <int VT, class Op, class Timer>
+inline void lbs_partition_expand(const Op &op, const int32_t *frontier, const int32_t *fsize, int64_t max_path,
+                                 const LbsBuffers &b, int32_t *out_queue, int32_t *out_size, hipStream_t s,
+                                 Timer &timer) {
+  constexpr int TILE = kExpandBlock * VT;
+  hipLaunchKernelGGL(k_lbs_partition<TILE>, dim3(grid_for(max_path / TILE + 2, 256, 4096)), dim3(256), 0, s, fsize,
+                     b.scan_local, b.partials, b.info, b.parts);
+  timer(3);
+  hipLaunchKernelGGL((k_lbs_expand<Op, VT>), dim3(b.expand_grid), dim3(kExpandBlock), 0, s, op, frontier, fsize,
+                     b.scan_local, b.partials, b.info, b.parts, out_queue, out_size);
+}
 
 template <class Op, class Timer>
 inline void lbs_launch(const Op &op, bool prologue, const int32_t *frontier, const int32_t *fsize, int64_t nf_host,
@@ -478,11 +497,12 @@ inline void lbs_launch(const Op &op, bool prologue, const int32_t *frontier, con
   hipLaunchKernelGGL(k_lbs_scan<kScanBlock>, dim3(1), dim3(kScanBlock), 0, s, fsize, b.partials, b.info,
                      b.edge_counter);
   timer(2);
-  hipLaunchKernelGGL(k_lbs_partition<0>, dim3(grid_for(max_path / kLbsTile + 2, 256, 4096)), dim3(256), 0, s, fsize,
-                     b.scan_local, b.partials, b.info, b.parts);
-  timer(3);
-  hipLaunchKernelGGL(k_lbs_expand<Op>, dim3(b.expand_grid), dim3(kExpandBlock), 0, s, op, frontier, fsize,
-                     b.scan_local, b.partials, b.info, b.parts, out_queue, out_size);
+  if (b.vt == 2)
+    lbs_partition_expand<2>(op, frontier, fsize, max_path, b, out_queue, out_size, s, timer);
+  else if (b.vt == 8)
+    lbs_partition_expand<8>(op, frontier, fsize, max_path, b, out_queue, out_size, s, timer);
+  else
+    lbs_partition_expand<4>(op, frontier, fsize, max_path, b, out_queue, out_size, s, timer);
   timer(4);
 }
 

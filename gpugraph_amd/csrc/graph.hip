@@ -211,7 +211,8 @@ int lbs_buffers(gg_graph *g, GASengine::LbsBuffers *b) {
   b->info = g->info.as<GASengine::LbsInfo>();
   b->parts = g->parts.as<int32_t>();
   b->edge_counter = g->counters.as<unsigned long long>() + 2;
-  b->expand_grid = 2048;
+  b->expand_grid = g->opt.expand_grid;
+  b->vt = g->opt.lbs_tile / GASengine::kExpandBlock;
   return GG_OK;
 }
 
@@ -239,8 +240,18 @@ void EventProfiler::end(const char *name) {
   if (!enabled || !open_) return;
   hipEvent_t b = take();
   (void)hipEventRecord(b, stream);
-  pending_.push_back({name, open_, b});
+  pending_.push_back({name, open_, b, 0.0});
   open_ = nullptr;
+}
+
+void EventProfiler::add_launch_bytes(const char *name, double bytes) {
+  if (!enabled) return;
+  table[name].bytes += bytes;
+  for (auto it = pending_.rbegin(); it != pending_.rend(); ++it)
+    if (it->name == name) {
+      it->bytes += bytes;
+      return;
+    }
 }
 
 void EventProfiler::flush() {
@@ -251,6 +262,7 @@ void EventProfiler::flush() {
     Entry &e = table[p.name];
     e.launches++;
     e.ms += ms;
+    if (launches.size() < kMaxLaunches) launches.push_back({p.name, (double)ms, p.bytes});
     pool_.push_back(p.a);
     pool_.push_back(p.b);
   }
@@ -260,6 +272,7 @@ void EventProfiler::flush() {
 void EventProfiler::reset() {
   flush();
   table.clear();
+  launches.clear();
   for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
   pool_.clear();
 }
@@ -294,6 +307,8 @@ int gg_default_options(gg_options *o) {
   o->lbs_tile = GASengine::kLbsTile;
   o->sssp_delta = 0;
   o->pull_alpha = 0;
+  o->expand_grid = 2048;
+  o->xcd_slices = -1;
   return GG_OK;
 }
 
@@ -312,23 +327,6 @@ int gg_graph_create(int32_t n, int64_t m, const int32_t *row_offsets, const int3
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return gg::set_error(GG_ERR_NO_DEVICE, "no HIP device");
 
   const bool dev_in = flags & GG_GRAPH_DEVICE_PTRS;
-  std::vector<int32_t> h_ro, h_col;
-  const int32_t *hro = row_offsets, *hcol = col_indices;
-  if (dev_in) {
-    h_ro.resize((size_t)n + 1);
-    h_col.resize((size_t)std::max<int64_t>(m, 1));
-    GG_HIP(hipMemcpy(h_ro.data(), row_offsets, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost));
-    if (m) GG_HIP(hipMemcpy(h_col.data(), col_indices, 4 * (size_t)m, hipMemcpyDeviceToHost));
-    hro = h_ro.data();
-    hcol = h_col.data();
-  }
-  // Validate the CSR on the host: the kernels index with it unchecked.
-  GG_CHECK_ARG(hro[0] == 0 && hro[n] == m, "row_offsets must start at 0 and end at m");
-  for (int32_t v = 0; v < n; v++)
-    GG_CHECK_ARG(hro[v] <= hro[v + 1], "row_offsets not monotone at %d", v);
-  for (int64_t e = 0; e < m; e++)
-    GG_CHECK_ARG(hcol[e] >= 0 && hcol[e] < n, "column index %d at %lld out of range", hcol[e], (long long)e);
-
   gg_graph *g = new (std::nothrow) gg_graph();
   if (!g) return gg::set_error(GG_ERR_OOM, "host allocation failed");
   std::unique_ptr<gg_graph, int (*)(gg_graph *)> guard(g, gg_graph_destroy);
@@ -352,6 +350,21 @@ int gg_graph_create(int32_t n, int64_t m, const int32_t *row_offsets, const int3
     GG_TRY(g->val.ensure(4 * (size_t)std::max<int64_t>(m, 1)));
     if (m) GG_HIP(hipMemcpy(g->val.p, values, 4 * (size_t)m, kind));
   }
+  // Validate the CSR on the device before any kernel indexes with it.
+  {
+    GG_TRY(g->counters.ensure(64 * sizeof(unsigned long long)));
+    int32_t *bad = reinterpret_cast<int32_t *>(g->counters.as<unsigned long long>());
+    const int32_t init[4] = {0, INT32_MAX, 0, INT32_MAX};
+    GG_HIP(hipMemcpyAsync(bad, init, sizeof(init), hipMemcpyHostToDevice, g->stream));
+    hipLaunchKernelGGL(GASengine::k_validate_csr<0>, dim3(GASengine::grid_for(std::max<int64_t>(m, n + 1), 256, 16384)),
+                       dim3(256), 0, g->stream, g->ro.as<int32_t>(), g->col.as<int32_t>(), n, m, bad);
+    GG_HIP(hipGetLastError());
+    int32_t h[4];
+    GG_HIP(hipMemcpyAsync(h, bad, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+    GG_HIP(hipStreamSynchronize(g->stream));
+    GG_CHECK_ARG(h[0] == 0, "row_offsets must start at 0, not decrease and end at m (first violation at %d)", h[1]);
+    GG_CHECK_ARG(h[2] == 0, "%d column indices out of range (first at slot %d)", h[2], h[3]);
+  }
   GASengine::DeviceGraph &d = g->dg;
   d.nodes = n;
   d.edges = m;
@@ -360,14 +373,11 @@ int gg_graph_create(int32_t n, int64_t m, const int32_t *row_offsets, const int3
   d.column_indices = g->col.as<int32_t>();
   d.edge_values = g->val.p;
   if (g->directed) {
-    std::vector<int32_t> co((size_t)n + 1), ri((size_t)std::max<int64_t>(m, 1)), eid((size_t)std::max<int64_t>(m, 1));
-    GASengine::host_csr_to_csc(n, m, hro, hcol, co.data(), ri.data(), eid.data());
-    GG_TRY(g->co.ensure(4 * co.size()));
-    GG_TRY(g->ri.ensure(4 * ri.size()));
-    GG_TRY(g->eid.ensure(4 * eid.size()));
-    GG_HIP(hipMemcpy(g->co.p, co.data(), 4 * co.size(), hipMemcpyHostToDevice));
-    GG_HIP(hipMemcpy(g->ri.p, ri.data(), 4 * ri.size(), hipMemcpyHostToDevice));
-    GG_HIP(hipMemcpy(g->eid.p, eid.data(), 4 * eid.size(), hipMemcpyHostToDevice));
+    GG_TRY(g->co.ensure(4 * ((size_t)n + 1)));
+    GG_TRY(g->ri.ensure(4 * (size_t)std::max<int64_t>(m, 1)));
+    GG_TRY(g->eid.ensure(4 * (size_t)std::max<int64_t>(m, 1)));
+    GG_HIP(GASengine::device_csr_to_csc(n, m, g->ro.as<int32_t>(), g->col.as<int32_t>(), g->co.as<int32_t>(),
+                                        g->ri.as<int32_t>(), g->eid.as<int32_t>(), g->stream));
     d.column_offsets = g->co.as<int32_t>();
     d.row_indices = g->ri.as<int32_t>();
     d.csc_edge_ids = g->eid.as<int32_t>();
@@ -420,10 +430,22 @@ int gg_graph_set_options(gg_graph *g, const gg_options *o) {
   GG_CHECK_ARG(o->wave_degree >= 1 && o->block_degree >= o->wave_degree, "need 1 <= wave_degree <= block_degree");
   GG_CHECK_ARG(o->iter_num >= 1, "iter_num must be >= 1");
   GG_CHECK_ARG(o->bfs_alpha >= 1 && o->bfs_beta >= 1, "bfs_alpha/bfs_beta must be >= 1");
-  GG_CHECK_ARG(o->lbs_tile == GASengine::kLbsTile, "lbs_tile: only %d is compiled in", GASengine::kLbsTile);
+  GG_CHECK_ARG(GASengine::lbs_tile_supported(o->lbs_tile), "lbs_tile must be 512, 1024 or 2048");
+  GG_CHECK_ARG(o->expand_grid >= 64 && o->expand_grid <= 65536, "expand_grid must be in [64, 65536]");
+  GG_CHECK_ARG(o->xcd_slices >= -1 && o->xcd_slices <= 1, "xcd_slices must be -1, 0 or 1");
   GG_CHECK_ARG(o->sssp_delta >= 0, "sssp_delta must be >= 0");
   GG_CHECK_ARG(o->pull_alpha >= 0, "pull_alpha must be >= 0");
   g->opt = *o;
+  return GG_OK;
+}
+
+int gg_graph_device_csc(const gg_graph *g, const int32_t **column_offsets, const int32_t **row_indices,
+                        const int32_t **csc_edge_ids) {
+  gg::clear_error();
+  GG_CHECK_ARG(g && column_offsets && row_indices && csc_edge_ids, "NULL argument");
+  *column_offsets = g->dg.column_offsets;
+  *row_indices = g->dg.row_indices;
+  *csc_edge_ids = g->directed ? g->dg.csc_edge_ids : nullptr;
   return GG_OK;
 }
 
@@ -491,6 +513,24 @@ int gg_profile_get(gg_graph *g, gg_kernel_stat *out, int32_t max_entries, int32_
       out[i].launches = kv.second.launches;
       out[i].total_ms = kv.second.ms;
       out[i].alg_bytes = kv.second.bytes;
+    }
+    i++;
+  }
+  *count = i;
+  return GG_OK;
+}
+
+int gg_profile_launches(gg_graph *g, gg_launch_stat *out, int32_t max_entries, int32_t *count) {
+  gg::clear_error();
+  GG_CHECK_ARG(g && count, "NULL argument");
+  g->prof.flush();
+  int32_t i = 0;
+  for (auto &l : g->prof.launches) {
+    if (out && i < max_entries) {
+      std::memset(&out[i], 0, sizeof(gg_launch_stat));
+      std::strncpy(out[i].name, l.name.c_str(), sizeof(out[i].name) - 1);
+      out[i].ms = l.ms;
+      out[i].alg_bytes = l.bytes;
     }
     i++;
   }

@@ -71,11 +71,20 @@ class EventProfiler : public GASengine::KernelTimer {
   hipStream_t stream = nullptr;
   bool enabled = false;
   std::map<std::string, Entry> table;
+  // Per-launch record (resolved launches, oldest first, capped at kMaxLaunches).
+  struct Launch {
+    std::string name;
+    double ms, bytes;
+  };
+  static constexpr size_t kMaxLaunches = 1 << 16;
+  std::vector<Launch> launches;
 
   ~EventProfiler() override;
   void begin(const char *name) override;
   void end(const char *name) override;
   void add_bytes(const char *name, double bytes) { if (enabled) table[name].bytes += bytes; }
+  // Same, and attribute the bytes to the latest launch of that class.
+  void add_launch_bytes(const char *name, double bytes);
   // Resolve recorded event pairs (stream must be synchronised).
   void flush();
   void reset();
@@ -84,6 +93,7 @@ class EventProfiler : public GASengine::KernelTimer {
   struct Pending {
     std::string name;
     hipEvent_t a, b;
+    double bytes;
   };
   std::vector<Pending> pending_;
   std::vector<hipEvent_t> pool_;
@@ -136,12 +146,20 @@ struct gg_graph {
   gg::DevBuf frontier[2], bits[4], vtx[4], scan_local, partials, parts, info, counters;
   gg::DevBuf pr[8];    // PageRank vectors / CSC-ordered transition weights
   gg::DevBuf batch[8]; // batched-instance state
+  gg::DevBuf tree[2];  // gg_bfs_tree: device depth and parent arrays
   // Degree bins of the vertices for row reductions: [0] over CSC in-edges,
   // [1] over CSR out-edges; lists of small / mid / large rows.
   gg::DevBuf bins[2][3];
   int32_t bin_count[2][3] = {{0, 0, 0}, {0, 0, 0}};
   bool bins_ready[2] = {false, false};
   int64_t pr_weights_version = -1;  // values version the PageRank transition weights were built for
+  // XCD-sliced row structure per direction ([0] CSC, [1] CSR): slice offsets
+  // (8n + 1), gathered ids and original slots in slice order, long-row lists,
+  // slice-ordered transition weights (F32 graphs) and the partials buffer.
+  gg::DevBuf sl_off[2], sl_idx[2], sl_perm[2], sl_long[2], sl_val[2], sl_part;
+  int32_t sl_mid_base[2][9] = {}, sl_large_base[2][9] = {};
+  bool sl_ready[2] = {false, false};
+  int64_t sl_val_version[2] = {-1, -1};
   int32_t *h_pinned = nullptr;  // pinned readback area (64 x int64)
   bool pr_cache_valid = false;
 

@@ -34,7 +34,10 @@
 #include <cmath>
 #include <cstring>
 
+#include <hipcub/hipcub.hpp>
+
 #include "gasengine/rowreduce.h"
+#include "gasengine/xcd_slices.h"
 #include "internal.h"
 
 namespace gg {
@@ -319,6 +322,92 @@ int ensure_transition(gg_graph *g) {
   return GG_OK;
 }
 
+// Whether PageRank sweeps on this graph use the XCD-sliced rows.  Auto (-1)
+// means off: on R-MAT scale 22 (average in-degree 16) the eight sub-rows per
+// vertex cost more in offsets and partials than the L2-resident gathers save
+// (36.6 ms vs 30.0 ms per solve); gg_autotune tries both.
+bool use_slices(const gg_graph *g) { return g->opt.xcd_slices == 1; }
+
+// Slice structure of direction dir (0: CSC in-edges, 1: CSR out-edges),
+// built once per graph; *L describes it for the sweep kernels.
+int ensure_slices(gg_graph *g, int dir, SliceLists *L) {
+  const int32_t n = g->n;
+  const int64_t m = g->m;
+  hipStream_t s = g->stream;
+  const int32_t *off = dir == 0 ? g->dg.column_offsets : g->dg.row_offsets;
+  const int32_t *nbr = dir == 0 ? g->dg.row_indices : g->dg.column_indices;
+  const int64_t cells = (int64_t)kXcdSlices * n;
+  if (!g->sl_ready[dir]) {
+    DevBuf cnt, tmp;
+    GG_TRY(cnt.ensure(4 * (size_t)(cells + 1)));
+    GG_TRY(g->sl_off[dir].ensure(4 * (size_t)(cells + 1)));
+    GG_TRY(g->sl_idx[dir].ensure(4 * (size_t)std::max<int64_t>(m, 1)));
+    GG_TRY(g->sl_perm[dir].ensure(4 * (size_t)std::max<int64_t>(m, 1)));
+    GG_HIP(fill_async<int32_t>(cnt.as<int32_t>() + cells, 0, 1, s));
+    hipLaunchKernelGGL(k_slice_count<0>, dim3(grid_for((int64_t)n * kWave, 256, 16384)), dim3(256), 0, s, off, nbr, n,
+                       cnt.as<int32_t>());
+    GG_HIP(hipGetLastError());
+    size_t tmp_bytes = 0;
+    GG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt.as<int32_t>(), g->sl_off[dir].as<int32_t>(),
+                                            (int)(cells + 1), s));
+    GG_TRY(tmp.ensure(tmp_bytes));
+    GG_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, cnt.as<int32_t>(), g->sl_off[dir].as<int32_t>(),
+                                            (int)(cells + 1), s));
+    hipLaunchKernelGGL(k_slice_fill<0>, dim3(grid_for((int64_t)n * kWave, 256, 16384)), dim3(256), 0, s, off, nbr, n,
+                       g->sl_off[dir].as<int32_t>(), g->sl_idx[dir].as<int32_t>(), g->sl_perm[dir].as<int32_t>());
+    GG_HIP(hipGetLastError());
+    // Long sub-rows: count, lay out per slice, fill.
+    int32_t *ctr = cnt.as<int32_t>();  // reused: 16 counters + 18 bases
+    GG_HIP(fill_async<int32_t>(ctr, 0, 2 * kXcdSlices, s));
+    hipLaunchKernelGGL(k_slice_long<0>, dim3(grid_for(cells, 256, 16384)), dim3(256), 0, s,
+                       g->sl_off[dir].as<int32_t>(), n, ctr, (const int32_t *)nullptr, (int32_t *)nullptr,
+                       (int32_t *)nullptr);
+    GG_HIP(hipGetLastError());
+    int32_t h[2 * kXcdSlices];
+    GG_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, s));
+    GG_HIP(hipStreamSynchronize(s));
+    int32_t base[2 * (kXcdSlices + 1)];
+    for (int c = 0; c < 2; c++) {
+      base[c * (kXcdSlices + 1)] = 0;
+      for (int t = 0; t < kXcdSlices; t++)
+        base[c * (kXcdSlices + 1) + t + 1] = base[c * (kXcdSlices + 1) + t] + h[c * kXcdSlices + t];
+    }
+    const int32_t nmid = base[kXcdSlices], nlarge = base[2 * kXcdSlices + 1];
+    GG_TRY(g->sl_long[dir].ensure(4 * (size_t)(nmid + nlarge + 1)));
+    std::memcpy(g->sl_mid_base[dir], base, sizeof(int32_t) * (kXcdSlices + 1));
+    std::memcpy(g->sl_large_base[dir], base + kXcdSlices + 1, sizeof(int32_t) * (kXcdSlices + 1));
+    if (nmid + nlarge) {
+      int32_t *dbase = ctr + 2 * kXcdSlices;
+      GG_HIP(hipMemcpyAsync(dbase, base, sizeof(base), hipMemcpyHostToDevice, s));
+      GG_HIP(fill_async<int32_t>(ctr, 0, 2 * kXcdSlices, s));
+      hipLaunchKernelGGL(k_slice_long<1>, dim3(grid_for(cells, 256, 16384)), dim3(256), 0, s,
+                         g->sl_off[dir].as<int32_t>(), n, ctr, (const int32_t *)dbase, g->sl_long[dir].as<int32_t>(),
+                         g->sl_long[dir].as<int32_t>() + nmid);
+      GG_HIP(hipGetLastError());
+    }
+    GG_HIP(hipStreamSynchronize(s));  // cnt / tmp are freed on return
+    g->sl_ready[dir] = true;
+    g->sl_val_version[dir] = -1;
+  }
+  // Slice-ordered transition weights (F32 graphs).
+  if (g->value_dtype == GG_VALUES_F32 && g->sl_val_version[dir] != g->pr_weights_version) {
+    GG_TRY(g->sl_val[dir].ensure(4 * (size_t)std::max<int64_t>(m, 1)));
+    const float *p = g->pr[dir == 0 ? 3 : 2].as<float>();
+    hipLaunchKernelGGL(k_gather_perm<float>, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, p,
+                       g->sl_perm[dir].as<int32_t>(), m, g->sl_val[dir].as<float>());
+    GG_HIP(hipGetLastError());
+    g->sl_val_version[dir] = g->pr_weights_version;
+  }
+  GG_TRY(g->sl_part.ensure(8 * (size_t)cells));
+  L->off = g->sl_off[dir].as<int32_t>();
+  L->mid = g->sl_long[dir].as<int32_t>();
+  L->large = L->mid + g->sl_mid_base[dir][kXcdSlices];
+  L->n = n;
+  std::memcpy(L->mid_base, g->sl_mid_base[dir], sizeof(L->mid_base));
+  std::memcpy(L->large_base, g->sl_large_base[dir], sizeof(L->large_base));
+  return GG_OK;
+}
+
 namespace {
 
 // Device input: copy a host array into buf unless the flags say device.
@@ -337,13 +426,16 @@ int stage_input(gg_graph *g, const float *p, uint32_t flags, gg::DevBuf &buf, co
   return GG_OK;
 }
 
-// Run sweeps (forward or adjoint op) to convergence; returns iterations.
+// Run sweeps (forward or adjoint op) to convergence; returns iterations.  The
+// sweep is either the degree-binned row reduction or, with slices, the
+// XCD-sliced partials + combine.
 template <class Op>
-int iterate(gg_graph *g, Op op, int dir, PrState *st, double tol, int32_t max_iter, const char *kname,
-            double bytes_per_sweep, int32_t *iters_out, double *resid_out) {
+int iterate(gg_graph *g, Op op, int dir, const SliceLists *slices, PrState *st, double tol, int32_t max_iter,
+            const char *kname, double bytes_per_sweep, int32_t *iters_out, double *resid_out) {
   hipStream_t s = g->stream;
   int32_t *lists[3] = {g->bins[dir][0].as<int32_t>(), g->bins[dir][1].as<int32_t>(), g->bins[dir][2].as<int32_t>()};
   const int32_t *done = &st->done;
+  float2 *part = slices ? g->sl_part.as<float2>() : nullptr;
   struct H {
     double resid_last;
     int32_t iters, done;
@@ -352,7 +444,10 @@ int iterate(gg_graph *g, Op op, int dir, PrState *st, double tol, int32_t max_it
     for (int j = 0; j < kItersPerCheck; j++) {
       {
         TimedRegion t(g->prof, kname);
-        rows_launch(op, lists, g->bin_count[dir], done, s);
+        if (slices)
+          slice_rows_launch(op, *slices, part, done, s);
+        else
+          rows_launch(op, lists, g->bin_count[dir], done, s);
       }
       hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(1), 0, s, st, tol, max_iter);
     }
@@ -423,9 +518,16 @@ int gg_pagerank(gg_graph *g, float damping, const float *teleport, double tol_l1
   op.inv_n = 1.0 / (double)n;
   op.st = st;
   const double per_sweep = (weighted ? 20.0 * n + 12.0 * m : 24.0 * n + 8.0 * m) + (tele ? 4.0 * n : 0.0);
+  SliceLists L{};
+  const bool sliced = use_slices(g);
+  if (sliced) {
+    GG_TRY(ensure_slices(g, 0, &L));
+    op.src = g->sl_idx[0].as<int32_t>();
+    op.pcsc = weighted ? g->sl_val[0].as<float>() : nullptr;
+  }
   int32_t iters = 0;
   double resid = 0;
-  GG_TRY(iterate(g, op, 0, st, tol_l1, max_iter, "pagerank.sweep", per_sweep, &iters, &resid));
+  GG_TRY(iterate(g, op, 0, sliced ? &L : nullptr, st, tol_l1, max_iter, "pagerank.sweep", per_sweep, &iters, &resid));
   GG_HIP(hipEventRecord(e1, s));
   GG_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -497,9 +599,17 @@ int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank, const flo
   op.inv_n = 1.0 / (double)n;
   op.st = st;
   const double per_sweep = weighted ? 20.0 * n + 12.0 * m : 24.0 * n + 8.0 * m;
+  SliceLists L{};
+  const bool sliced = use_slices(g);
+  if (sliced) {
+    GG_TRY(ensure_slices(g, 1, &L));
+    op.dst = g->sl_idx[1].as<int32_t>();
+    op.pcsr = weighted ? g->sl_val[1].as<float>() : nullptr;
+  }
   int32_t iters = 0;
   double resid = 0;
-  GG_TRY(iterate(g, op, 1, st, tol_l1, max_iter, "pagerank.adjoint_sweep", per_sweep, &iters, &resid));
+  GG_TRY(iterate(g, op, 1, sliced ? &L : nullptr, st, tol_l1, max_iter, "pagerank.adjoint_sweep", per_sweep, &iters,
+                 &resid));
   const float *lam = (iters & 1) ? l1 : l0;
   // Gradients.
   float *gw = nullptr, *gt = nullptr;

@@ -274,7 +274,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     const int64_t remain = g->h64()[16], dmin = g->h64()[17], degsum = g->h64()[18];
     const int64_t improved_so_far = g->h64()[4];
     const double cb = n + n / 8.0 + 4.0 * (nf + remain) + 4.0 * nf;
-    g->prof.add_bytes("sssp.collect", cb);
+    g->prof.add_launch_bytes("sssp.collect", cb);
     bytes += cb;
     if (nf == 0) {
       if (remain == 0) break;
@@ -293,7 +293,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
         GG_HIP(hipGetLastError());
       }
       const double pb = 4.0 * n + 4.0 * m + n / 8.0 + 8.0 * degsum;
-      g->prof.add_bytes("sssp.pull", pb);
+      g->prof.add_launch_bytes("sssp.pull", pb);
       bytes += pb;
       edges += m;
       pulls++;
@@ -305,8 +305,8 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
                  lbs_timer);
       GG_HIP(hipGetLastError());
       const double pb = 16.0 * nf + 12.0 * degsum;
-      g->prof.add_bytes("sssp.lbs_count", 16.0 * nf);
-      g->prof.add_bytes("sssp.push", 12.0 * degsum);
+      g->prof.add_launch_bytes("sssp.lbs_count", 16.0 * nf);
+      g->prof.add_launch_bytes("sssp.push", 12.0 * degsum);
       bytes += pb;
       edges += degsum;
     }

@@ -41,6 +41,7 @@ extern "C" {
 #define GG_ERR_NO_DEVICE 6      /* no HIP device visible */
 #define GG_ERR_IO 7             /* file open / parse failure */
 #define GG_ERR_UNSUPPORTED 8    /* option not supported for this graph */
+#define GG_ERR_INTERNAL 9       /* an internal consistency check failed (a library bug) */
 
 /* ---- edge value dtypes ------------------------------------------------- */
 #define GG_VALUES_NONE 0
@@ -73,9 +74,12 @@ typedef struct gg_options {
   int32_t iter_num;      /* maximum GAS iterations (reference "iter_num") */
   int32_t bfs_alpha;     /* top-down -> bottom-up when m_f > m_u / alpha */
   int32_t bfs_beta;      /* bottom-up -> top-down when n_f < n / beta */
-  int32_t lbs_tile;      /* merge-path positions per LBS workgroup tile (read-only: 1024) */
+  int32_t lbs_tile;      /* merge-path positions per LBS workgroup tile: 512, 1024 or 2048 */
   int32_t sssp_delta;    /* SSSP bucket width (0 = auto: 128); INT32_MAX = one bucket */
   int32_t pull_alpha;    /* SSSP: pull step when frontier edges > m / pull_alpha (0 = never) */
+  int32_t expand_grid;   /* persistent workgroups of the LBS expand kernel (64 .. 65536) */
+  int32_t xcd_slices;    /* PageRank sweeps split by gathered-vertex slice, one slice per
+                            XCD L2: -1 = auto (currently off), 0 = off, 1 = on */
 } gg_options;
 
 /* Per-run statistics (the reference prints these with INSTRUMENT/verbose). */
@@ -118,6 +122,13 @@ int gg_graph_info(const gg_graph *g, int32_t *n, int64_t *m, int32_t *directed,
 /* Device pointers of the resident CSR (for zero-copy callers). */
 int gg_graph_device_csr(const gg_graph *g, const int32_t **row_offsets,
                         const int32_t **col_indices, const void **values);
+/* Device pointers of the in-edge view: for directed graphs the CSC built on
+ * the device at creation and its CSC->CSR edge-id permutation (reference:
+ * d_column_offsets / d_row_indices / d_edge_id of GraphSlice, built by
+ * util.cuh:55-126); for undirected graphs the CSR itself and NULL ids.
+ * Within a column, source rows ascend. */
+int gg_graph_device_csc(const gg_graph *g, const int32_t **column_offsets,
+                        const int32_t **row_indices, const int32_t **csc_edge_ids);
 /* Wait for all work issued on the graph's stream. */
 int gg_graph_synchronize(gg_graph *g);
 
@@ -129,6 +140,16 @@ int gg_graph_synchronize(gg_graph *g);
  * generic GAS program path. */
 int gg_bfs(gg_graph *g, int32_t src, int32_t *depth, uint32_t flags,
            gg_stats *stats);
+
+/* BFS depths and a BFS tree.  Replaces the reference's predecessor marking
+ * (CsrProblem<..., MARK_PREDECESSORS> csr_problem.cuh:113-117, extraction
+ * Algorithms/BFS/predextract.cuh:246+): after the BFS of gg_bfs (same flags),
+ * one pass picks for every reached vertex v != src its smallest in-neighbour
+ * one level up, so the tree is unique; pred[src] = src, pred = -1 where
+ * unreached.  depth may be NULL.  The tree passes the reference's validation
+ * rules (Algorithms/BFS/bfs.cu:984-1012). */
+int gg_bfs_tree(gg_graph *g, int32_t src, int32_t *depth, int32_t *pred,
+                uint32_t flags, gg_stats *stats);
 
 /* SSSP distances with int32 edge weights (unreached = 100000000).  Replaces
  * EnactIterativeSearch with Program = sssp (Algorithms/SSSP/sssp.h:34-393;
@@ -190,6 +211,39 @@ typedef struct gg_batch_spec {
 int gg_batch_run(gg_graph *g, const gg_batch_spec *spec, uint32_t flags,
                  gg_stats *stats);
 
+/* ---- autotuner ---------------------------------------------------------- */
+/* Replaces the reference's fixed schedule constants (degree tiers 32/512,
+ * enactor_vertex_centric.cuh:3949-3950) and hand-set "threshold"
+ * (Algorithms/SSSP/register.h:39): coordinate descent over the gg_options
+ * knobs on this graph, timing the selected programs from `src` (fastest of
+ * `reps` runs, each candidate bracketed by runs of the current options) and
+ * keeping a change when the programs it affects get >= 3% faster; a final
+ * bracketed run against the starting options keeps the result only if it
+ * still wins.  Every trial's outputs must equal the first run's (device
+ * checksum) or the call fails with GG_ERR_INTERNAL.  On success the best
+ * options are installed on the graph and copied to *best; trials[] (up to
+ * max_trials, count in *num_trials) logs each timing: first the untuned
+ * baseline (option -1), last the final comparison (option -2) when any knob
+ * changed.  On failure the graph keeps the options it had. */
+#define GG_TUNE_BFS 1u   /* gg_bfs, direction-optimising */
+#define GG_TUNE_SSSP 2u  /* gg_sssp, near-far (int32 values only) */
+#define GG_TUNE_CC 4u    /* gg_cc, label propagation (undirected only) */
+#define GG_TUNE_GAS 8u   /* reference-form programs: gg_bfs GG_BFS_PUSH_ONLY, gg_sssp GG_SSSP_GAS */
+#define GG_TUNE_PAGERANK 16u /* gg_pagerank, damping 0.85, uniform teleport, tol 1e-6 (FP32
+                                results: a trial must stay within L1 1e-5 of the baseline) */
+
+typedef struct gg_tune_trial {
+  uint32_t programs;  /* GG_TUNE_* programs timed in this trial */
+  int32_t option;     /* index of the int32 field of gg_options varied (0 = threshold ...
+                         10 = xcd_slices); -1 = baseline */
+  int32_t value;      /* the value tried */
+  float ms;           /* summed median device time of those programs */
+} gg_tune_trial;
+
+int gg_autotune(gg_graph *g, uint32_t programs, int32_t src, int32_t reps,
+                gg_options *best, gg_tune_trial *trials, int32_t max_trials,
+                int32_t *num_trials);
+
 /* ---- instrumentation ---------------------------------------------------- */
 /* Per-kernel-class event timings collected by runs with GG_INSTRUMENT
  * (the reference's INSTRUMENT template flag, enactor_vertex_centric.cuh:96). */
@@ -203,6 +257,19 @@ typedef struct gg_kernel_stat {
 int gg_profile_reset(gg_graph *g);
 int gg_profile_get(gg_graph *g, gg_kernel_stat *out, int32_t max_entries,
                    int32_t *count);
+
+/* The same timings launch by launch (oldest first, at most 65536 kept until
+ * gg_profile_reset), for per-iteration bandwidth: alg_bytes is the launch's
+ * own byte-model share where the run attributes it per launch (BFS and SSSP
+ * frontier kernels), else 0. */
+typedef struct gg_launch_stat {
+  char name[48];
+  double ms;
+  double alg_bytes;
+} gg_launch_stat;
+
+int gg_profile_launches(gg_graph *g, gg_launch_stat *out, int32_t max_entries,
+                        int32_t *count);
 
 /* ---- graph construction (host) ----------------------------------------- */
 /* Host CSR produced by the builders below; free with gg_csr_free. */

@@ -32,6 +32,9 @@ def lib():
             build()
         L = C.CDLL(LIB_PATH)
         L.oracle_bfs.argtypes = [C.c_int32, _I, _I, C.c_int32, _I]
+        L.oracle_bfs_parents.argtypes = [C.c_int32, _I, _I, _I, C.c_int32, _I]
+        L.oracle_bfs_validate.argtypes = [C.c_int32, _I, _I, C.c_int32, _I, _I]
+        L.oracle_bfs_validate.restype = C.c_int32
         L.oracle_sssp_selection.argtypes = [C.c_int32, _I, _I, _I, C.c_int32, _I]
         L.oracle_sssp_dijkstra.argtypes = [C.c_int32, _I, _I, _I, C.c_int32, _I]
         L.oracle_cc.argtypes = [C.c_int32, _I, _I, _I]
@@ -60,6 +63,18 @@ def bfs(ro, col, src):
     out = np.empty(len(ro) - 1, np.int32)
     lib().oracle_bfs(len(ro) - 1, _i(ro), _i(col), int(src), out.ctypes.data_as(_I))
     return out
+
+
+def bfs_parents(ro, col, depth, src):
+    ro, col, depth = (np.ascontiguousarray(x, np.int32) for x in (ro, col, depth))
+    out = np.empty(len(ro) - 1, np.int32)
+    lib().oracle_bfs_parents(len(ro) - 1, _i(ro), _i(col), _i(depth), int(src), out.ctypes.data_as(_I))
+    return out
+
+
+def bfs_validate(ro, col, src, depth, pred):
+    ro, col, depth, pred = (np.ascontiguousarray(x, np.int32) for x in (ro, col, depth, pred))
+    return int(lib().oracle_bfs_validate(len(ro) - 1, _i(ro), _i(col), int(src), _i(depth), _i(pred)))
 
 
 def sssp(ro, col, w, src, selection=False):

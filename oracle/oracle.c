@@ -36,6 +36,66 @@ void oracle_bfs(int32_t n, const int32_t *ro, const int32_t *col, int32_t src,
   free(queue);
 }
 
+/* BFS tree parents in the form the library returns: pred[src] = src,
+ * pred[v] = the smallest u with an edge u->v and depth[u] = depth[v] - 1
+ * for every other reached v, -1 for unreached v.  (The reference marks the
+ * discovering vertex, MARK_PREDECESSORS of csr_problem.cuh:113-117; any
+ * valid parent passes its validation, this rule makes the choice unique.) */
+void oracle_bfs_parents(int32_t n, const int32_t *ro, const int32_t *col,
+                        const int32_t *depth, int32_t src, int32_t *pred) {
+  for (int32_t v = 0; v < n; v++) pred[v] = -1;
+  if (src < 0 || src >= n) return;
+  pred[src] = src;
+  for (int32_t u = 0; u < n; u++) {
+    if (depth[u] < 0) continue;
+    for (int32_t e = ro[u]; e < ro[u + 1]; e++) {
+      const int32_t v = col[e];
+      if (v != src && pred[v] == -1 && depth[v] == depth[u] + 1) pred[v] = u;
+    }
+  }
+}
+
+/* The reference's BFS validation (the five Graph500 rules listed at
+ * Algorithms/BFS/bfs.cu:984-989, directed form of its edge rule at
+ * bfs.cu:994-1012).  Returns 0 when (depth, pred) pass, else the first rule
+ * broken:
+ *   1 the root: depth[src] = 0, pred[src] = src, no other depth-0 vertex
+ *     (with rule 2 this makes the tree acyclic: depth drops by one per step)
+ *   2 each tree edge joins levels that differ by exactly one
+ *   3 every edge u->v with u reached has v reached and depth[v] <= depth[u]+1
+ *   4 the tree spans the reached set: pred >= 0 exactly where depth >= 0
+ *   5 (pred[v], v) is an edge of the graph */
+int32_t oracle_bfs_validate(int32_t n, const int32_t *ro, const int32_t *col,
+                            int32_t src, const int32_t *depth,
+                            const int32_t *pred) {
+  if (src < 0 || src >= n || depth[src] != 0 || pred[src] != src) return 1;
+  for (int32_t v = 0; v < n; v++)
+    if (v != src && depth[v] == 0) return 1;
+  for (int32_t v = 0; v < n; v++) {
+    if (v == src || depth[v] < 0) continue;
+    const int32_t p = pred[v];
+    if (p < 0 || p >= n || depth[p] < 0 || depth[v] != depth[p] + 1) return 2;
+  }
+  for (int32_t u = 0; u < n; u++) {
+    if (depth[u] < 0) continue;
+    for (int32_t e = ro[u]; e < ro[u + 1]; e++) {
+      const int32_t v = col[e];
+      if (depth[v] < 0 || depth[v] > depth[u] + 1) return 3;
+    }
+  }
+  for (int32_t v = 0; v < n; v++)
+    if ((depth[v] >= 0) != (pred[v] >= 0)) return 4;
+  unsigned char *found = (unsigned char *)calloc((size_t)(n > 0 ? n : 1), 1);
+  for (int32_t u = 0; u < n; u++)
+    for (int32_t e = ro[u]; e < ro[u + 1]; e++)
+      if (pred[col[e]] == u) found[col[e]] = 1;
+  int32_t rc = 0;
+  for (int32_t v = 0; v < n && !rc; v++)
+    if (v != src && depth[v] >= 0 && !found[v]) rc = 5;
+  free(found);
+  return rc;
+}
+
 /* Algorithms/SSSP/sssp.cu:126-183 -- selection Dijkstra exactly as written:
  * pick the unvisited vertex with the smallest dist < 100000000 (lowest id on
  * ties), relax its out-edges with 64-bit sums. */

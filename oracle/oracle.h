@@ -12,6 +12,10 @@
  * independent source -- hand-derived KATs, scipy.sparse.csgraph, dense FP64
  * linear solves, finite differences -- never against a retyped formula):
  *   oracle_bfs                 pinned: KATs + scipy BFS + |d[u]-d[v]|<=1
+ *   oracle_bfs_parents         pinned: hand-derived KAT parents + accepted by
+ *                              oracle_bfs_validate + brute-force min rule
+ *   oracle_bfs_validate        pinned: accepts scipy's BFS tree (any valid
+ *                              parents) and rejects one planted error per rule
  *   oracle_sssp_selection      pinned: KATs + scipy dijkstra
  *   oracle_sssp_dijkstra       pinned: equal to oracle_sssp_selection + scipy
  *   oracle_cc                  pinned: scipy connected_components (min id)
@@ -29,6 +33,17 @@ extern "C" {
 /* Algorithms/BFS/bfs.cu:57-111 (CPUBFS). depth[v] = -1 if unreached. */
 void oracle_bfs(int32_t n, const int32_t *ro, const int32_t *col, int32_t src,
                 int32_t *depth);
+
+/* BFS tree parents: pred[src] = src, pred[v] = smallest in-neighbour one
+ * level up, -1 unreached.  Same rule as gg_bfs_tree. */
+void oracle_bfs_parents(int32_t n, const int32_t *ro, const int32_t *col,
+                        const int32_t *depth, int32_t src, int32_t *pred);
+
+/* Graph500 BFS validation (Algorithms/BFS/bfs.cu:984-1012): 0 = valid, else
+ * the number of the first rule broken (see oracle.c). */
+int32_t oracle_bfs_validate(int32_t n, const int32_t *ro, const int32_t *col,
+                            int32_t src, const int32_t *depth,
+                            const int32_t *pred);
 
 /* Algorithms/SSSP/sssp.cu:126-183 (CPUSSSP) as written: O(n^2) selection,
  * sentinel 100000000, sums formed in 64-bit ("long"). */

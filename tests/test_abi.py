@@ -33,7 +33,7 @@ def test_binding_covers_header():
 def test_default_options_and_version():
     o = gg.default_options()
     assert (o.wave_degree, o.block_degree, o.bfs_alpha, o.bfs_beta) == (128, 1024, 14, 24)
-    assert (o.lbs_tile, o.sssp_delta, o.pull_alpha) == (1024, 0, 0)
+    assert (o.lbs_tile, o.sssp_delta, o.pull_alpha, o.expand_grid, o.xcd_slices) == (1024, 0, 0, 2048, -1)
     assert o.threshold == -1 and o.iter_num == 2**31 - 1
     assert gg.load_library().gg_version() == 10000
 
@@ -45,6 +45,7 @@ def test_null_arguments_are_rejected():
     assert lib.gg_default_options(None) == _abi.GG_ERR_INVALID
     assert lib.gg_bfs(None, 0, None, 0, None) == _abi.GG_ERR_INVALID
     assert lib.gg_graph_destroy(None) == _abi.GG_OK
+    assert lib.gg_autotune(None, _abi.GG_TUNE_BFS, 0, 3, None, None, 0, None) == _abi.GG_ERR_INVALID
 
 
 def test_graph_create_validation_without_device():

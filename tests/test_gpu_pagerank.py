@@ -167,3 +167,60 @@ def test_errors_and_int_weights(gpu_available):
         assert e.value.code == _abi.GG_ERR_NOT_CONVERGED
         with pytest.raises(gg.GGError):
             g.pagerank(damping=1.0)
+
+
+def star_ring(n):
+    """Every vertex points to vertex 0 and to its ring successor: vertex 0 has
+    n - 1 in-edges (more than 4096 per XCD slice for n > 32768)."""
+    src = np.concatenate([np.arange(1, n), np.arange(n)])
+    dst = np.concatenate([np.zeros(n - 1, np.int64), (np.arange(n) + 1) % n])
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    ro = np.zeros(n + 1, np.int64)
+    np.add.at(ro, src + 1, 1)
+    return gg.CSR(np.cumsum(ro).astype(np.int32), dst.astype(np.int32))
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_xcd_sliced_sweeps(gpu_available, weighted):
+    """Sliced sweeps (xcd_slices = 1: per-slice partials, short / wave / workgroup
+    sub-rows, combine) against the FP64 oracle and the unsliced sweeps."""
+    graphs = [gg.gen_rmat(14, 16, undirected=False, seed=43), star_ring(40000)]
+    for c in graphs:
+        if weighted:
+            c = c.with_weights("f32", symmetric=False, seed=44)
+        p = normalised_p32(c) if weighted else None
+        exp, _, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=p, tol=1e-14)
+        with gg.Graph(c, directed=True) as g:
+            g.set_options(xcd_slices=1)
+            r1, st1 = g.pagerank(0.85, tol_l1=2e-7)
+            r1b, _ = g.pagerank(0.85, tol_l1=2e-7)
+            g.set_options(xcd_slices=0)
+            r0, _ = g.pagerank(0.85, tol_l1=2e-7)
+        assert l1(r1, exp) <= L1_TOL and l1(r0, exp) <= L1_TOL
+        np.testing.assert_array_equal(r1, r1b)
+        assert st1["residual"] <= 2e-7
+
+
+def test_xcd_sliced_adjoint_and_update(gpu_available):
+    c = gg.gen_rmat(12, 16, undirected=False, seed=45).with_weights("f32", symmetric=False, seed=46)
+    rng = np.random.default_rng(47)
+    cv = rng.standard_normal(c.n)
+    p = normalised_p32(c)
+    r64, _, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=p, tol=1e-15)
+    _, gw64, gt64, _ = oracle.pagerank_adjoint_f64(c.row_offsets, c.col_indices, c.values.astype(np.float64), r64, cv,
+                                                   tol=1e-15)
+    with gg.Graph(c, directed=True) as g:
+        g.set_options(xcd_slices=1)
+        r, _ = g.pagerank(0.85, tol_l1=2e-7)
+        gw, gt, _ = g.pagerank_adjoint(r, cv.astype(np.float32), tol_l1=2e-7)
+        assert np.linalg.norm(gt - gt64) / np.linalg.norm(gt64) < 1e-4
+        assert np.linalg.norm(gw - gw64) / np.linalg.norm(gw64) < 1e-4
+        # new weights reach the slice-ordered transition weights
+        w2 = c.values.copy()
+        w2[: c.m // 2] *= 5.0
+        g.update_values(w2)
+        r2, _ = g.pagerank(0.85, tol_l1=2e-7)
+    exp2, _, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=normalised_p32(gg.CSR(c.row_offsets, c.col_indices, w2)),
+                                     tol=1e-14)
+    assert l1(r2, exp2) <= L1_TOL

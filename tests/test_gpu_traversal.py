@@ -219,3 +219,43 @@ def test_config2_full_size(gpu_available, rmat22):
     reached[src] = False
     assert np.all(tight[reached])  # every reached vertex has a shortest-path parent
     assert np.array_equal(dist < INF, d >= 0)  # same reachable set as BFS (undirected)
+
+
+def test_bfs_tree_full_size(gpu_available, rmat22):
+    """BFS tree at R-MAT scale 22: parents equal the oracle's smallest-parent
+    rule and pass the reference's validation rules."""
+    c = rmat22
+    src = workloads.max_degree_vertex(c)
+    with gg.Graph(c) as g:
+        d, p, st = g.bfs_tree(src)
+    assert oracle.bfs_validate(c.row_offsets, c.col_indices, src, d, p) == 0
+    np.testing.assert_array_equal(p, oracle.bfs_parents(c.row_offsets, c.col_indices, d, src))
+
+
+def test_bfs_tree(gpu_available, kat_graphs):
+    """Parents (smallest in-neighbour one level up) equal the oracle's on the
+    KATs and on directed / undirected R-MAT, through both BFS paths and with
+    device outputs; every tree passes the reference's validation rules."""
+    import torch
+
+    for k in kat_graphs:
+        with gg.Graph(csr_of(k["row_offsets"], k["col_indices"]), directed=k["directed"]) as g:
+            for s in k["bfs"]:
+                for push in (False, True):
+                    d, p, _ = g.bfs_tree(int(s), push_only=push)
+                    assert d.tolist() == k["bfs"][s]
+                    np.testing.assert_array_equal(p, oracle.bfs_parents(k["row_offsets"], k["col_indices"], d, int(s)))
+    for undirected in (True, False):
+        c = gg.gen_rmat(14, 16, undirected=undirected, seed=74)
+        srcs = (workloads.max_degree_vertex(c), 5)
+        with gg.Graph(c, directed=not undirected) as g:
+            for src in srcs:
+                d, p, _ = g.bfs_tree(src)
+                assert oracle.bfs_validate(c.row_offsets, c.col_indices, src, d, p) == 0
+                np.testing.assert_array_equal(p, oracle.bfs_parents(c.row_offsets, c.col_indices, d, src))
+                dd = torch.empty(c.n, dtype=torch.int32, device="cuda")
+                pd = torch.empty(c.n, dtype=torch.int32, device="cuda")
+                g.bfs_tree(src, push_only=True, flags=_abi.GG_OUT_DEVICE, depth_out=dd, pred_out=pd)
+                g.synchronize()
+                np.testing.assert_array_equal(dd.cpu().numpy(), d)
+                np.testing.assert_array_equal(pd.cpu().numpy(), p)

@@ -212,3 +212,75 @@ def test_mutation_is_caught(kat_graphs):
     d = oracle.bfs(g["row_offsets"], g["col_indices"], 0)
     d[3] += 1
     assert d.tolist() != g["bfs"]["0"]
+
+
+def test_bfs_parents_kats(kat_graphs):
+    """Hand-derived parents (smallest in-neighbour one level up)."""
+    exp = {("small", 0): [0, 0, 0, 0, 1, 2, 3], ("small", 3): [-1, -1, -1, 3, -1, 3, 3],
+           ("trivial", 0): [0, 0, 1, 2, 3], ("trivial", 4): [-1, -1, -1, 4, 4]}
+    by_name = {g["name"]: g for g in kat_graphs}
+    for (name, src), want in exp.items():
+        g = by_name[name]
+        d = oracle.bfs(g["row_offsets"], g["col_indices"], src)
+        p = oracle.bfs_parents(g["row_offsets"], g["col_indices"], d, src)
+        assert p.tolist() == want, (name, src)
+        assert oracle.bfs_validate(g["row_offsets"], g["col_indices"], src, d, p) == 0
+
+
+@pytest.mark.parametrize("seed", [6, 7])
+def test_bfs_parents_and_validation_vs_scipy(seed):
+    ro, col, _ = random_csr(3000, 9000, seed)
+    rows = np.repeat(np.arange(3000), np.diff(ro))
+    for src in (0, 1234):
+        d = oracle.bfs(ro, col, src)
+        # scipy's BFS tree (parent = discovering vertex) is another valid tree
+        _, sp_pred = csg.breadth_first_order(to_scipy(ro, col), src, directed=True, return_predecessors=True)
+        sp_pred = np.where(sp_pred < 0, -1, sp_pred).astype(np.int32)
+        sp_pred[src] = src
+        assert oracle.bfs_validate(ro, col, src, d, sp_pred) == 0
+        p = oracle.bfs_parents(ro, col, d, src)
+        assert oracle.bfs_validate(ro, col, src, d, p) == 0
+        # brute force of the smallest-parent rule
+        ok = (d[rows] >= 0) & (d[col] == d[rows] + 1)
+        want = np.full(3000, -1, np.int64)
+        for u, v in zip(rows[ok][::-1], col[ok][::-1]):  # ascending u wins: write descending, last write = min
+            want[v] = u
+        want[src] = src
+        np.testing.assert_array_equal(p, want)
+
+
+def test_bfs_validate_rejects_each_rule():
+    ro, col, _ = random_csr(500, 2000, 8)
+    src = 0
+    d = oracle.bfs(ro, col, src)
+    p = oracle.bfs_parents(ro, col, d, src)
+    rows = np.repeat(np.arange(500), np.diff(ro))
+    assert oracle.bfs_validate(ro, col, src, d, p) == 0
+    reached = np.flatnonzero((d >= 2))
+    v = int(reached[0])
+    # 1: root not at depth 0
+    d1 = d.copy(); d1[src] = 1
+    assert oracle.bfs_validate(ro, col, src, d1, p) == 1
+    # 2: a parent two levels up
+    p2 = p.copy(); p2[v] = src
+    assert oracle.bfs_validate(ro, col, src, d, p2) == 2
+    # 2: two vertices of one level pointing at each other (a cycle)
+    lvl = np.flatnonzero(d == d[v])
+    a, b = int(lvl[0]), int(lvl[1])
+    p3 = p.copy(); p3[a] = b; p3[b] = a
+    assert oracle.bfs_validate(ro, col, src, d, p3) == 2
+    # 3 and 4 on a hand graph: path 0->1->2->3, shortcut 0->3, isolated vertex 4
+    hro = np.array([0, 2, 3, 4, 4, 4], np.int32)
+    hcol = np.array([1, 3, 2, 3], np.int32)
+    hd = oracle.bfs(hro, hcol, 0)
+    hp = oracle.bfs_parents(hro, hcol, hd, 0)
+    assert hd.tolist() == [0, 1, 2, 1, -1] and hp.tolist() == [0, 0, 1, 0, -1]
+    assert oracle.bfs_validate(hro, hcol, 0, hd, hp) == 0
+    # a consistent tree along the path, but the shortcut skips two levels
+    assert oracle.bfs_validate(hro, hcol, 0, np.array([0, 1, 2, 3, -1]), np.array([0, 0, 1, 2, -1])) == 3
+    # the isolated vertex given a parent
+    assert oracle.bfs_validate(hro, hcol, 0, hd, np.array([0, 0, 1, 0, 0])) == 4
+    # 5: a parent at the right level that is not an in-neighbour
+    cand = [int(u) for u in np.flatnonzero(d == d[v] - 1) if u not in set(rows[col == v].tolist())]
+    p6 = p.copy(); p6[v] = cand[0]
+    assert oracle.bfs_validate(ro, col, src, d, p6) == 5
