@@ -19,7 +19,10 @@ Extras (same JSON line, `extras`): CC on the 2048^2 percolated lattice
 (label propagation and hooking); batched 128 SSSP + 64 PPR on R-MAT scale 20
 directed against 128 + 64 plain launches; PageRank and its adjoint on R-MAT
 scale 22 directed with FP32 weights.  --full-checks adds the FP64 oracle's
-PageRank L1 error at scale 22 (tens of seconds of CPU time).
+PageRank L1 error at scale 22 (tens of seconds of CPU time).  The R-MAT graphs
+are stored renumbered by degree (GG_GRAPH_REORDER; ids in and out stay the
+generator's; --no-reorder keeps the generated order), and the headline's
+BFS / SSSP times on the graph in generated order are reported beside it.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -61,8 +64,9 @@ def headline(args, gg, _abi, workloads, torch, np):
     out_bfs = torch.empty(csr.n, dtype=torch.int32, device=dev)
     out_sssp = torch.empty(csr.n, dtype=torch.int32, device=dev)
     t0 = time.perf_counter()
-    g = gg.Graph(csr)
-    log(f"graph resident on device in {time.perf_counter() - t0:.1f}s (H2D + validation)")
+    g = gg.Graph(csr, reorder=not args.no_reorder)
+    log(f"graph resident on device in {time.perf_counter() - t0:.1f}s (H2D + validation"
+        f"{'' if args.no_reorder else ' + renumbering by degree'})")
     flags = _abi.GG_OUT_DEVICE
     tuned = None
     if not args.no_tune:
@@ -168,7 +172,15 @@ def headline(args, gg, _abi, workloads, torch, np):
     config = {"workload": f"rmat{args.scale}_undirected_bfs+sssp", "n": csr.n, "m_csr": csr.m, "edge_factor": 16,
               "weights": "int32 uniform [1,64]", "source": src, "step": "1 BFS + 1 SSSP",
               "teps_counting": "sum of CSR degree over reached vertices",
-              "options": "autotuned (gg_autotune, BFS + SSSP) before timing" if tuned else "defaults"}
+              "options": "autotuned (gg_autotune, BFS + SSSP) before timing" if tuned else "defaults",
+              "vertex_order": "caller ids, stored renumbered by degree (GG_GRAPH_REORDER)" if not args.no_reorder
+              else "as generated"}
+    if not args.no_reorder:
+        # The same BFS / SSSP on the graph stored in generated order.
+        with gg.Graph(csr) as g0:
+            b0, _ = median_ms(lambda: g0.bfs(src, flags=flags, out=out_bfs), 5)
+            s0, _ = median_ms(lambda: g0.sssp(src, flags=flags, out=out_sssp), 5)
+        per_alg["generated_order"] = {"bfs_ms": b0, "sssp_ms": s0}
     if tuned is not None:
         per_alg["autotune"] = tuned
     return value, wall, roofline, kernels, per_alg, cpu, config
@@ -185,11 +197,11 @@ def extra_lattice(gg, workloads, np):
     return out
 
 
-def extra_batch(gg, workloads, torch, np, scale):
+def extra_batch(gg, workloads, torch, np, scale, reorder):
     c = workloads.config4_rmat(scale)
     src, seeds = workloads.config4_instances(c, 128, 64)
-    out = {"graph": f"rmat{scale} directed, n={c.n}, m={c.m}"}
-    with gg.Graph(c, directed=True) as g:
+    out = {"graph": f"rmat{scale} directed, n={c.n}, m={c.m}", "vertex_order": "renumbered" if reorder else "generated"}
+    with gg.Graph(c, directed=True, reorder=reorder) as g:
         g.batch(sssp_sources=src[:8], ppr_seeds=seeds[:4], tol_l1=1e-6)  # warm-up (allocations)
         g.sssp(int(src[0]))
         g.pagerank(teleport=np.eye(1, c.n, int(seeds[0]), dtype=np.float32)[0], tol_l1=1e-6)
@@ -229,11 +241,11 @@ def extra_batch(gg, workloads, torch, np, scale):
     return out
 
 
-def extra_pagerank(gg, workloads, np, scale, full_checks):
+def extra_pagerank(gg, workloads, np, scale, full_checks, reorder):
     c = workloads.config5_rmat(scale)
-    out = {"graph": f"rmat{scale} directed f32, n={c.n}, m={c.m}"}
+    out = {"graph": f"rmat{scale} directed f32, n={c.n}, m={c.m}", "vertex_order": "renumbered" if reorder else "generated"}
     cvec = workloads.config5_loss_weights(c.n).astype(np.float32)
-    with gg.Graph(c, directed=True) as g:
+    with gg.Graph(c, directed=True, reorder=reorder) as g:
         g.pagerank(tol_l1=2e-7)
         ms, st = median_ms(lambda: g.pagerank(tol_l1=2e-7), 3)
         r, _ = g.pagerank(tol_l1=2e-7)
@@ -281,6 +293,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
     ap.add_argument("--no-tune", action="store_true", help="time the default options (no gg_autotune)")
+    ap.add_argument("--no-reorder", action="store_true", help="keep the R-MAT graphs in generated vertex order")
     ap.add_argument("--full-checks", action="store_true", help="FP64 PageRank error at full size")
     ap.add_argument("--copy-probe", action="store_true", help="run only a 1 GiB device copy (PMC calibration)")
     args = ap.parse_args()
@@ -304,9 +317,9 @@ def main():
     if not args.no_extras:
         extras.update(extra_lattice(gg, workloads, np))
         log(f"lattice CC: {extras}")
-        extras["batched"] = extra_batch(gg, workloads, torch, np, 20)
+        extras["batched"] = extra_batch(gg, workloads, torch, np, 20, not args.no_reorder)
         log(f"batched: {extras['batched']}")
-        extras["pagerank"] = extra_pagerank(gg, workloads, np, 22, args.full_checks)
+        extras["pagerank"] = extra_pagerank(gg, workloads, np, 22, args.full_checks, not args.no_reorder)
         log(f"pagerank: {extras['pagerank']}")
     line = {
         "metric": "GTEPS (BFS/SSSP/CC, R-MAT s22) + % HBM roofline; batched vs plain speedup",

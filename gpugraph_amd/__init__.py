@@ -139,7 +139,7 @@ def _dev_ptr(x) -> int:
 class Graph:
     """A graph resident on the current HIP device (gg_graph handle)."""
 
-    def __init__(self, csr: CSR, directed: bool = False):
+    def __init__(self, csr: CSR, directed: bool = False, reorder: bool = False):
         self.lib = load_library()
         self.csr = csr
         self.n, self.m = csr.n, csr.m
@@ -155,7 +155,8 @@ class Graph:
         self.value_dtype = dt
         h = C.c_void_p()
         _abi.check(self.lib.gg_graph_create(self.n, self.m, _ptr(ro), _ptr(col), _ptr(vals) if vals is not None else None,
-                                            dt, _abi.GG_GRAPH_DIRECTED if directed else 0, C.byref(h)),
+                                            dt, (_abi.GG_GRAPH_DIRECTED if directed else 0)
+                                            | (_abi.GG_GRAPH_REORDER if reorder else 0), C.byref(h)),
                    "gg_graph_create")
         self.h = h
 

@@ -29,6 +29,7 @@ GG_VALUES_F32 = 2
 
 GG_GRAPH_DIRECTED = 1
 GG_GRAPH_DEVICE_PTRS = 2
+GG_GRAPH_REORDER = 4
 
 GG_OUT_DEVICE = 1
 GG_INSTRUMENT = 2
@@ -144,6 +145,7 @@ PROTOTYPES = {
     "gg_graph_info": (C.c_int, [_P, _I32P, C.POINTER(C.c_int64), _I32P, _I32P]),
     "gg_graph_device_csr": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "gg_graph_device_csc": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
+    "gg_graph_vertex_order": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P)]),
     "gg_graph_synchronize": (C.c_int, [_P]),
     "gg_bfs": (C.c_int, [_P, C.c_int32, _P, C.c_uint32, C.POINTER(Stats)]),
     "gg_bfs_tree": (C.c_int, [_P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(Stats)]),

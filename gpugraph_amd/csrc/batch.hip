@@ -336,16 +336,22 @@ __global__ void k_bs_swap(BatchState *st, const LbsInfo *info, int32_t max_iter)
   if (st->cur_size == 0 || st->sssp_iters >= max_iter) st->sssp_done = 1;
 }
 
-// Vertex-major rows -> instance-major output: in[v][KW] -> out[i][v] for
-// i < k.  A workgroup moves 64 vertices through LDS (row-contiguous reads,
-// 256 B per instance writes; the +1 pad keeps both LDS passes conflict-free).
+// Vertex-major rows -> instance-major output: in[row(v)][KW] -> out[i][v] for
+// i < k, row(v) = perm[v] on a renumbered graph (v in caller ids), else v.  A
+// workgroup moves 64 vertices through LDS (row-contiguous reads, 256 B per
+// instance writes; the +1 pad keeps both LDS passes conflict-free).
 template <typename T, int KW>
 __global__ void __launch_bounds__(256)
-k_rows_to_instances(const T *__restrict__ in, int32_t n, int32_t k, T *__restrict__ out) {
+k_rows_to_instances(const T *__restrict__ in, int32_t n, int32_t k, const int32_t *__restrict__ perm,
+                    T *__restrict__ out) {
   __shared__ T tile[64][KW + 1];
   for (int64_t v0 = (int64_t)blockIdx.x * 64; v0 < n; v0 += (int64_t)gridDim.x * 64) {
     const int nv = n - v0 < 64 ? (int)(n - v0) : 64;
-    for (int idx = threadIdx.x; idx < nv * KW; idx += 256) tile[idx / KW][idx % KW] = in[v0 * KW + idx];
+    for (int idx = threadIdx.x; idx < nv * KW; idx += 256) {
+      const int r = idx / KW, j = idx % KW;
+      const int64_t row = perm ? perm[v0 + r] : v0 + r;
+      tile[r][j] = in[row * KW + j];
+    }
     __syncthreads();
     for (int idx = threadIdx.x; idx < k * 64; idx += 256) {
       const int i = idx / 64, v = idx % 64;
@@ -684,6 +690,7 @@ static int batch_pass(gg_graph *g, const int32_t *h_src, int32_t ns, int32_t *di
   const int32_t ppr_iters = g->h_pinned[8];
   double resid;
   std::memcpy(&resid, g->h_pinned + 10, 8);
+  const int32_t *perm = g->reordered ? g->re_new.as<int32_t>() : nullptr;
   {
     TimedRegion t(g->prof, "batch.output");
     if (ns && dist_out) {
@@ -693,7 +700,7 @@ static int batch_pass(gg_graph *g, const int32_t *h_src, int32_t ns, int32_t *di
         o = g->vtx[2].as<int32_t>();
       }
       hipLaunchKernelGGL((k_rows_to_instances<int32_t, kSsspMax>), dim3(grid_for(n, 64, 16384)), dim3(256), 0, s,
-                         dist, n, ns, o);
+                         dist, n, ns, perm, o);
       GG_HIP(hipGetLastError());
       if (!(flags & GG_OUT_DEVICE))
         GG_HIP(hipMemcpyAsync(dist_out, o, 4 * (size_t)n * ns, hipMemcpyDeviceToHost, s));
@@ -706,7 +713,7 @@ static int batch_pass(gg_graph *g, const int32_t *h_src, int32_t ns, int32_t *di
         o = g->vtx[1].as<float>();
       }
       hipLaunchKernelGGL((k_rows_to_instances<float, kPprMax>), dim3(grid_for(n, 64, 16384)), dim3(256), 0, s, Rf,
-                         n, np, o);
+                         n, np, perm, o);
       GG_HIP(hipGetLastError());
       if (!(flags & GG_OUT_DEVICE))
         GG_HIP(hipMemcpyAsync(rank_out, o, 4 * (size_t)n * np, hipMemcpyDeviceToHost, s));
@@ -755,13 +762,20 @@ extern "C" int gg_batch_run(gg_graph *g, const gg_batch_spec *spec, uint32_t fla
   const int32_t max_iter = spec->max_iter > 0 ? spec->max_iter : INT_MAX;
   int32_t done_s = 0, done_p = 0;
   const size_t n = (size_t)g->n;
+  // Sources and seeds in the graph's internal numbering.
+  std::vector<int32_t> src_int, seed_int;
+  if (spec->num_sssp) src_int.assign(spec->sssp_sources, spec->sssp_sources + spec->num_sssp);
+  if (spec->num_ppr) seed_int.assign(spec->ppr_seeds, spec->ppr_seeds + spec->num_ppr);
+  for (auto &v : src_int) v = map_vertex(g, v);
+  for (auto &v : seed_int) v = map_vertex(g, v);
   while (done_s < spec->num_sssp || done_p < spec->num_ppr) {
     const int32_t ns = std::min(kSsspMax, spec->num_sssp - done_s);
     const int32_t np = std::min(kPprMax, spec->num_ppr - done_p);
     int32_t *dout = spec->sssp_dist ? spec->sssp_dist + done_s * n : nullptr;
     float *rout = spec->ppr_rank ? spec->ppr_rank + done_p * n : nullptr;
-    GG_TRY(batch_pass(g, spec->sssp_sources + done_s, ns, dout, spec->ppr_seeds ? spec->ppr_seeds + done_p : nullptr,
-                      np, rout, spec->damping, spec->tol_l1, max_iter, flags, &acc));
+    GG_TRY(batch_pass(g, src_int.empty() ? nullptr : src_int.data() + done_s, ns, dout,
+                      seed_int.empty() ? nullptr : seed_int.data() + done_p, np,
+                      rout, spec->damping, spec->tol_l1, max_iter, flags, &acc));
     done_s += ns;
     done_p += np;
   }

@@ -157,10 +157,12 @@ namespace {
 
 // BFS tree parents from the depths: one wave per vertex scans its in-edges
 // (CSC, sources ascending) 64 at a time; the first source one level up is the
-// parent (the smallest such id).  pred[src] = src, -1 where unreached.
+// parent (the smallest such id).  pred[src] = src, -1 where unreached.  With
+// old_of (a renumbered graph) the parent is the source with the smallest
+// original id, and pred holds original ids.
 __global__ void __launch_bounds__(256)
 k_bfs_parents(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, const int32_t *__restrict__ depth,
-              int32_t n, int32_t src, int32_t *__restrict__ pred) {
+              int32_t n, int32_t src, const int32_t *__restrict__ old_of, int32_t *__restrict__ pred) {
   const int64_t waves = (int64_t)gridDim.x * (256 / GASengine::kWave);
   const int lane = GASengine::lane_id();
   for (int64_t v = (int64_t)blockIdx.x * (256 / GASengine::kWave) + threadIdx.x / GASengine::kWave; v < n;
@@ -168,42 +170,79 @@ k_bfs_parents(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, co
     const int32_t dv = depth[v];
     int32_t p = -1;
     if (v == src) {
-      p = src;
+      p = old_of ? old_of[src] : src;
     } else if (dv > 0) {
       const int32_t b = co[v], e = co[v + 1];
+      int32_t best = INT_MAX;
       for (int32_t k0 = b; k0 < e; k0 += GASengine::kWave) {
         const int32_t k = k0 + lane;
         const int32_t u = k < e ? ri[k] : -1;
-        const uint64_t hit = GASengine::ballot(k < e && depth[u] == dv - 1);
-        if (hit) {
-          p = __shfl(u, __builtin_ctzll(hit));
-          break;
+        const bool up = k < e && depth[u] == dv - 1;
+        if (!old_of) {
+          const uint64_t hit = GASengine::ballot(up);
+          if (hit) {
+            best = __shfl(u, __builtin_ctzll(hit));
+            break;
+          }
+        } else if (up) {
+          best = min(best, old_of[u]);
         }
       }
+      if (old_of)
+        for (int off = 32; off > 0; off >>= 1) best = min(best, __shfl_xor(best, off));
+      p = best == INT_MAX ? -1 : best;
     }
     if (lane == 0) pred[v] = p;
   }
 }
 
 }  // namespace
-}  // namespace gg
 
-// Declared in bfs_do.hip / cc_hook.hip.
-namespace gg {
+// Cores of the public entry points: internal vertex ids, any output pointer.
 int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth, uint32_t flags, gg_stats *stats);
 int cc_hooking(gg_graph *g, int32_t *label, uint32_t flags, gg_stats *stats);
 int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist, uint32_t flags, gg_stats *stats);
+
+int bfs_core(gg_graph *g, int32_t src, int32_t *depth, uint32_t flags, gg_stats *stats) {
+  if (!(flags & GG_BFS_PUSH_ONLY)) return bfs_direction_optimizing(g, src, depth, flags, stats);
+  int s = src;
+  return run_program<bfs>(g, 1, &s, depth, flags, stats, -1, true);
+}
+
+int sssp_core(gg_graph *g, int32_t src, int32_t *dist, uint32_t flags, gg_stats *stats) {
+  int s = src;
+  if (!(flags & GG_SSSP_GAS)) return sssp_near_far(g, src, dist, flags, stats);
+  return run_program<sssp>(g, 1, &s, dist, flags, stats, sssp::INIT_VALUE, true);
+}
+
+int cc_core(gg_graph *g, int32_t *label, uint32_t flags, gg_stats *stats) {
+  if (flags & GG_CC_HOOK) return cc_hooking(g, label, flags, stats);
+  // Every vertex is labelled: reach_stats counts all of them.
+  return run_program<cc>(g, 0, nullptr, label, flags, stats, INT_MIN, true);
+}
+
+// A renumbered graph runs a core into re_out[0] and maps the result back.
+template <class Core>
+int with_mapped_output(gg_graph *g, void *out, uint32_t flags, Core core) {
+  GG_TRY(activate(g));
+  GG_TRY(g->re_out[0].ensure(4 * (size_t)g->n));
+  GG_TRY(core(g->re_out[0].as<int32_t>(), flags | GG_OUT_DEVICE));
+  return reorder_out_vertices(g, g->re_out[0].p, out, flags);
+}
+
 }  // namespace gg
 
 extern "C" {
 
 int gg_bfs(gg_graph *g, int32_t src, int32_t *depth, uint32_t flags, gg_stats *stats) {
+  using namespace gg;
   gg::clear_error();
   GG_CHECK_ARG(g, "graph is NULL");
   GG_CHECK_ARG(src >= 0 && src < g->n, "src %d out of range [0, %d)", src, g->n);
-  if (!(flags & GG_BFS_PUSH_ONLY)) return gg::bfs_direction_optimizing(g, src, depth, flags, stats);
-  int s = src;
-  return gg::run_program<bfs>(g, 1, &s, depth, flags, stats, -1, true);
+  if (!g->reordered) return bfs_core(g, src, depth, flags, stats);
+  const int32_t s = map_vertex(g, src);
+  return with_mapped_output(g, depth, flags,
+                            [&](int32_t *o, uint32_t f) { return bfs_core(g, s, o, f, stats); });
 }
 
 int gg_bfs_tree(gg_graph *g, int32_t src, int32_t *depth, int32_t *pred, uint32_t flags, gg_stats *stats) {
@@ -212,14 +251,15 @@ int gg_bfs_tree(gg_graph *g, int32_t src, int32_t *depth, int32_t *pred, uint32_
   GG_CHECK_ARG(g && pred, "graph/pred is NULL");
   GG_CHECK_ARG(src >= 0 && src < g->n, "src %d out of range [0, %d)", src, g->n);
   GG_TRY(activate(g));
-  const bool dev = flags & GG_OUT_DEVICE;
+  const bool dev = (flags & GG_OUT_DEVICE) && !g->reordered;
   const size_t bytes = 4 * (size_t)g->n;
   int32_t *d_depth = depth;
   if (!dev || !depth) {
     GG_TRY(g->tree[0].ensure(bytes));
     d_depth = g->tree[0].as<int32_t>();
   }
-  GG_TRY(gg_bfs(g, src, d_depth, flags | GG_OUT_DEVICE, stats));
+  const int32_t s = map_vertex(g, src);
+  GG_TRY(bfs_core(g, s, d_depth, flags | GG_OUT_DEVICE, stats));
   int32_t *d_pred = pred;
   if (!dev) {
     GG_TRY(g->tree[1].ensure(bytes));
@@ -228,37 +268,52 @@ int gg_bfs_tree(gg_graph *g, int32_t src, int32_t *depth, int32_t *pred, uint32_
   {
     TimedRegion t(g->prof, "bfs.parents");
     hipLaunchKernelGGL(k_bfs_parents, dim3(GASengine::grid_for((int64_t)g->n * GASengine::kWave, 256, 16384)),
-                       dim3(256), 0, g->stream, g->dg.column_offsets, g->dg.row_indices, d_depth, g->n, src, d_pred);
+                       dim3(256), 0, g->stream, g->dg.column_offsets, g->dg.row_indices, d_depth, g->n, s,
+                       g->reordered ? g->re_order.as<int32_t>() : (const int32_t *)nullptr, d_pred);
     GG_HIP(hipGetLastError());
   }
-  if (!dev && depth) GG_TRY(copy_out(g, depth, d_depth, bytes, flags));
-  GG_TRY(copy_out(g, pred, d_pred, bytes, flags));
+  if (g->reordered) {
+    GG_TRY(reorder_out_vertices(g, d_depth, depth, flags));
+    GG_TRY(reorder_out_vertices(g, d_pred, pred, flags));
+  } else {
+    if (!dev && depth) GG_TRY(copy_out(g, depth, d_depth, bytes, flags));
+    GG_TRY(copy_out(g, pred, d_pred, bytes, flags));
+  }
   if (flags & GG_INSTRUMENT) finish_profile(g);
   return GG_OK;
 }
 
 int gg_sssp(gg_graph *g, int32_t src, int32_t *dist, uint32_t flags, gg_stats *stats) {
+  using namespace gg;
   gg::clear_error();
   GG_CHECK_ARG(g, "graph is NULL");
   GG_CHECK_ARG(src >= 0 && src < g->n, "src %d out of range [0, %d)", src, g->n);
   GG_CHECK_ARG(g->value_dtype == GG_VALUES_I32, "SSSP needs int32 edge weights");
-  int s = src;
-  if (!(flags & GG_SSSP_GAS)) return gg::sssp_near_far(g, src, dist, flags, stats);
-  return gg::run_program<sssp>(g, 1, &s, dist, flags, stats, sssp::INIT_VALUE, true);
+  if (!g->reordered) return sssp_core(g, src, dist, flags, stats);
+  const int32_t s = map_vertex(g, src);
+  return with_mapped_output(g, dist, flags,
+                            [&](int32_t *o, uint32_t f) { return sssp_core(g, s, o, f, stats); });
 }
 
 int gg_cc(gg_graph *g, int32_t *label, uint32_t flags, gg_stats *stats) {
+  using namespace gg;
   gg::clear_error();
   GG_CHECK_ARG(g, "graph is NULL");
-  if (flags & GG_CC_HOOK) return gg::cc_hooking(g, label, flags, stats);
-  // Every vertex is labelled: reach_stats counts all of them.
-  return gg::run_program<cc>(g, 0, nullptr, label, flags, stats, INT_MIN, true);
+  if (!g->reordered) return cc_core(g, label, flags, stats);
+  GG_TRY(activate(g));
+  GG_TRY(g->re_out[0].ensure(4 * (size_t)g->n));
+  GG_TRY(cc_core(g, g->re_out[0].as<int32_t>(), flags | GG_OUT_DEVICE, stats));
+  return reorder_out_cc(g, g->re_out[0].as<int32_t>(), label, flags);
 }
 
 int gg_pagerank_reference(gg_graph *g, float *rank, uint32_t flags, gg_stats *stats) {
+  using namespace gg;
   gg::clear_error();
   GG_CHECK_ARG(g, "graph is NULL");
-  return gg::run_program<pagerank>(g, 0, nullptr, rank, flags, stats, 0.0f, false);
+  if (!g->reordered) return run_program<pagerank>(g, 0, nullptr, rank, flags, stats, 0.0f, false);
+  return with_mapped_output(g, rank, flags, [&](int32_t *o, uint32_t f) {
+    return run_program<pagerank>(g, 0, nullptr, reinterpret_cast<float *>(o), f, stats, 0.0f, false);
+  });
 }
 
 }  // extern "C"

@@ -365,6 +365,14 @@ int gg_graph_create(int32_t n, int64_t m, const int32_t *row_offsets, const int3
     GG_CHECK_ARG(h[0] == 0, "row_offsets must start at 0, not decrease and end at m (first violation at %d)", h[1]);
     GG_CHECK_ARG(h[2] == 0, "%d column indices out of range (first at slot %d)", h[2], h[3]);
   }
+  if (flags & GG_GRAPH_REORDER) {
+    gg::DevBuf ro0, col0, val0;
+    ro0.swap(g->ro);
+    col0.swap(g->col);
+    val0.swap(g->val);
+    GG_TRY(gg::build_reorder(g, ro0.as<int32_t>(), col0.as<int32_t>(),
+                             value_dtype != GG_VALUES_NONE ? val0.p : nullptr));
+  }
   GASengine::DeviceGraph &d = g->dg;
   d.nodes = n;
   d.edges = m;
@@ -460,7 +468,9 @@ int gg_graph_update_values(gg_graph *g, const void *values, uint32_t flags) {
   GG_CHECK_ARG(g && values, "NULL argument");
   GG_CHECK_ARG(g->value_dtype != GG_VALUES_NONE, "graph was created without values");
   GG_TRY(gg::activate(g));
-  if (g->m)
+  if (g->reordered)
+    GG_TRY(gg::reorder_in_edges(g, values, flags, g->val.p));
+  else if (g->m)
     GG_HIP(hipMemcpyAsync(g->val.p, values, 4 * (size_t)g->m,
                           (flags & GG_OUT_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, g->stream));
   GG_HIP(hipStreamSynchronize(g->stream));
@@ -485,6 +495,14 @@ int gg_graph_device_csr(const gg_graph *g, const int32_t **row_offsets, const in
   if (row_offsets) *row_offsets = g->dg.row_offsets;
   if (col_indices) *col_indices = g->dg.column_indices;
   if (values) *values = g->dg.edge_values;
+  return GG_OK;
+}
+
+int gg_graph_vertex_order(const gg_graph *g, const int32_t **new_of_old, const int32_t **old_of_new) {
+  gg::clear_error();
+  GG_CHECK_ARG(g && new_of_old && old_of_new, "NULL argument");
+  *new_of_old = g->reordered ? g->re_new.as<int32_t>() : nullptr;
+  *old_of_new = g->reordered ? g->re_order.as<int32_t>() : nullptr;
   return GG_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "common.h"
@@ -57,6 +58,10 @@ struct DevBuf {
   template <class T>
   T *as() const {
     return reinterpret_cast<T *>(p);
+  }
+  void swap(DevBuf &o) {
+    std::swap(p, o.p);
+    std::swap(bytes, o.bytes);
   }
 };
 
@@ -147,6 +152,12 @@ struct gg_graph {
   gg::DevBuf pr[8];    // PageRank vectors / CSC-ordered transition weights
   gg::DevBuf batch[8]; // batched-instance state
   gg::DevBuf tree[2];  // gg_bfs_tree: device depth and parent arrays
+  // Degree reordering (GG_GRAPH_REORDER): order[new] = old, new_id[old] =
+  // new, emap[new slot] = old slot; results in internal order before mapping;
+  // staging for host outputs; host copy of new_id for source ids.
+  bool reordered = false;
+  gg::DevBuf re_order, re_new, re_emap, re_out[2], re_in[2], re_stage;
+  std::vector<int32_t> h_new_id;
   // Degree bins of the vertices for row reductions: [0] over CSC in-edges,
   // [1] over CSR out-edges; lists of small / mid / large rows.
   gg::DevBuf bins[2][3];
@@ -202,6 +213,17 @@ int pull_partition(gg_graph *g, const int32_t **parts);
 // Edge values in CSC order (4-byte values): the CSR array itself for
 // undirected graphs, a cached permuted copy for directed ones.
 int csc_values(gg_graph *g, const void **vals);
+// Degree reordering (reorder.hip).  build_reorder renumbers the validated
+// device CSR (ro0, col0, val0) into g->ro / col / val; the map helpers move
+// 4-byte vertex or edge arrays between caller order and internal order
+// (host or device side per GG_OUT_DEVICE in flags).
+int build_reorder(gg_graph *g, const int32_t *ro0, const int32_t *col0, const void *val0);
+int32_t map_vertex(const gg_graph *g, int32_t v);
+int reorder_out_vertices(gg_graph *g, const void *in, void *dst, uint32_t flags);
+int reorder_in_vertices(gg_graph *g, const void *src, uint32_t flags, DevBuf &buf, const void **out);
+int reorder_in_edges(gg_graph *g, const void *src, uint32_t flags, void *dst_dev);
+int reorder_out_edges(gg_graph *g, const void *in, void *dst, uint32_t flags);
+int reorder_out_cc(gg_graph *g, const int32_t *label, int32_t *dst, uint32_t flags);
 // Symmetry / value-symmetry check of an undirected graph (sets mirror_ok,
 // values_asymmetric); GG_ERR_INVALID when the CSR is provably not symmetric.
 int check_undirected(gg_graph *g);

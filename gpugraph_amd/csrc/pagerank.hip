@@ -489,7 +489,10 @@ int gg_pagerank(gg_graph *g, float damping, const float *teleport, double tol_l1
   GG_TRY(g->pr[5].ensure(8 * (size_t)n));  // x[2]
   GG_TRY(g->pr[6].ensure(sizeof(PrState)));
   const float *tele = nullptr;
-  GG_TRY(stage_input(g, teleport, flags, g->pr[7], &tele));
+  if (g->reordered && teleport)
+    GG_TRY(reorder_in_vertices(g, teleport, flags, g->pr[7], reinterpret_cast<const void **>(&tele)));
+  else
+    GG_TRY(stage_input(g, teleport, flags, g->pr[7], &tele));
   float *r0 = g->pr[4].as<float>(), *r1 = r0 + n;
   float *x0 = g->pr[5].as<float>(), *x1 = x0 + n;
   PrState *st = g->pr[6].as<PrState>();
@@ -546,7 +549,10 @@ int gg_pagerank(gg_graph *g, float damping, const float *teleport, double tol_l1
     stats->gpu_ms = ms;
     stats->residual = resid;
   }
-  GG_TRY(copy_out(g, rank, res, 4 * (size_t)n, flags));
+  if (g->reordered)
+    GG_TRY(reorder_out_vertices(g, res, rank, flags));
+  else
+    GG_TRY(copy_out(g, rank, res, 4 * (size_t)n, flags));
   if (resid > tol_l1) return set_error(GG_ERR_NOT_CONVERGED, "PageRank: residual %.3e > tol %.3e after %d sweeps",
                                        resid, tol_l1, iters);
   return GG_OK;
@@ -572,8 +578,16 @@ int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank, const flo
   GG_TRY(g->batch[1].ensure(4 * (size_t)n));  // rowsum
   GG_TRY(g->batch[2].ensure(sizeof(PrState)));
   const float *d_rank = nullptr, *d_c = nullptr;
-  GG_TRY(stage_input(g, rank, flags, g->batch[3], &d_rank));
-  GG_TRY(stage_input(g, c, flags, g->batch[4], &d_c));
+  if (g->reordered) {
+    GG_TRY(reorder_in_vertices(g, rank, flags, g->batch[3], reinterpret_cast<const void **>(&d_rank)));
+    GG_TRY(reorder_in_vertices(g, c, flags, g->batch[4], reinterpret_cast<const void **>(&d_c)));
+  } else {
+    GG_TRY(stage_input(g, rank, flags, g->batch[3], &d_rank));
+    GG_TRY(stage_input(g, c, flags, g->batch[4], &d_c));
+  }
+  // Gradients go straight to the caller's device buffers unless they need
+  // mapping back to the caller's order.
+  const bool dev_out = (flags & GG_OUT_DEVICE) && !g->reordered;
   float *l0 = g->batch[0].as<float>(), *l1 = l0 + n;
   PrState *st = g->batch[2].as<PrState>();
 
@@ -614,7 +628,7 @@ int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank, const flo
   // Gradients.
   float *gw = nullptr, *gt = nullptr;
   if (grad_w) {
-    if (flags & GG_OUT_DEVICE) {
+    if (dev_out) {
       gw = grad_w;
     } else {
       GG_TRY(g->batch[5].ensure(4 * (size_t)std::max<int64_t>(m, 1)));
@@ -639,7 +653,7 @@ int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank, const flo
     g->prof.add_bytes("pagerank.gradient", 16.0 * m + 16.0 * n);
   }
   if (grad_t) {
-    if (flags & GG_OUT_DEVICE) {
+    if (dev_out) {
       gt = grad_t;
     } else {
       GG_TRY(g->batch[6].ensure(4 * (size_t)n));
@@ -665,8 +679,13 @@ int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank, const flo
     stats->gpu_ms = ms;
     stats->residual = resid;
   }
-  if (grad_w && !(flags & GG_OUT_DEVICE)) GG_TRY(copy_out(g, grad_w, gw, 4 * (size_t)m, flags));
-  if (grad_t && !(flags & GG_OUT_DEVICE)) GG_TRY(copy_out(g, grad_t, gt, 4 * (size_t)n, flags));
+  if (g->reordered) {
+    if (grad_w) GG_TRY(reorder_out_edges(g, gw, grad_w, flags));
+    if (grad_t) GG_TRY(reorder_out_vertices(g, gt, grad_t, flags));
+  } else {
+    if (grad_w && !dev_out) GG_TRY(copy_out(g, grad_w, gw, 4 * (size_t)m, flags));
+    if (grad_t && !dev_out) GG_TRY(copy_out(g, grad_t, gt, 4 * (size_t)n, flags));
+  }
   if (iters >= max_iter && resid > tol_l1)
     return set_error(GG_ERR_NOT_CONVERGED, "adjoint: residual %.3e > tol %.3e x ||c||_1 after %d sweeps", resid, tol_l1,
                      iters);

@@ -53,6 +53,9 @@ extern "C" {
                                    otherwise the CSR must be symmetric and is
                                    used as its own CSC */
 #define GG_GRAPH_DEVICE_PTRS 2u /* inputs are device pointers on the current device */
+#define GG_GRAPH_REORDER 4u     /* store the graph renumbered by out-degree, highest first (locality
+                                   of the random vertex gathers); every call still takes and
+                                   returns the caller's vertex ids and edge order */
 
 /* ---- run flags --------------------------------------------------------- */
 #define GG_OUT_DEVICE 1u        /* output pointers are device memory */
@@ -129,6 +132,11 @@ int gg_graph_device_csr(const gg_graph *g, const int32_t **row_offsets,
  * Within a column, source rows ascend. */
 int gg_graph_device_csc(const gg_graph *g, const int32_t **column_offsets,
                         const int32_t **row_indices, const int32_t **csc_edge_ids);
+/* GG_GRAPH_REORDER graphs: device arrays new_of_old[v] (internal id of
+ * caller vertex v) and old_of_new[i]; the device CSR / CSC views above are in
+ * internal numbering.  Both NULL for graphs stored in caller order. */
+int gg_graph_vertex_order(const gg_graph *g, const int32_t **new_of_old,
+                          const int32_t **old_of_new);
 /* Wait for all work issued on the graph's stream. */
 int gg_graph_synchronize(gg_graph *g);
 
