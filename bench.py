@@ -175,6 +175,13 @@ def headline(args, gg, _abi, workloads, torch, np):
               "options": "autotuned (gg_autotune, BFS + SSSP) before timing" if tuned else "defaults",
               "vertex_order": "caller ids, stored renumbered by degree (GG_GRAPH_REORDER)" if not args.no_reorder
               else "as generated"}
+    # CC on the same graph (label propagation and hooking), outside the timed steps.
+    with gg.Graph(csr, reorder=not args.no_reorder) as gc:
+        cc = {}
+        for name, mode in (("labelprop", {}), ("hook", {"hook": True})):
+            ms, st = median_ms(lambda: gc.cc(flags=flags, out=out_bfs, **mode), 5)
+            cc[name] = {"ms": ms, "gteps": st["reached_edges"] / (ms * 1e-3) / 1e9, "iterations": st["iterations"]}
+        per_alg["cc"] = cc
     if not args.no_reorder:
         # The same BFS / SSSP on the graph stored in generated order.
         with gg.Graph(csr) as g0:
@@ -190,9 +197,9 @@ def extra_lattice(gg, workloads, np):
     lat = workloads.config3_lattice(2048)
     out = {}
     with gg.Graph(lat) as gl:
-        for hook in (False, True):
-            ms, st = median_ms(lambda: gl.cc(hook=hook), 3)
-            out["cc_lattice_" + ("hook" if hook else "labelprop")] = {
+        for name, mode in (("labelprop", {}), ("labelprop_gas", {"gas": True}), ("hook", {"hook": True})):
+            ms, st = median_ms(lambda: gl.cc(**mode), 3)
+            out["cc_lattice_" + name] = {
                 "ms": ms, "gteps": st["reached_edges"] / (ms * 1e-3) / 1e9, "iterations": st["iterations"]}
     return out
 

@@ -262,9 +262,11 @@ class Graph:
         _abi.check(self.lib.gg_sssp(self.h, src, ptr, flags, C.byref(st)), "gg_sssp")
         return out, st.as_dict()
 
-    def cc(self, hook: bool = False, flags: int = 0, out=None):
+    def cc(self, hook: bool = False, gas: bool = False, flags: int = 0, out=None):
+        """CC labels (gg_cc): label propagation (default), through the generic
+        GAS path (gas=True) or by hooking (hook=True)."""
         st = _abi.Stats()
-        f = flags | (_abi.GG_CC_HOOK if hook else 0)
+        f = flags | (_abi.GG_CC_HOOK if hook else 0) | (_abi.GG_CC_GAS if gas else 0)
         out, ptr = self._out(out, np.int32, f)
         _abi.check(self.lib.gg_cc(self.h, ptr, f, C.byref(st)), "gg_cc")
         return out, st.as_dict()

@@ -36,6 +36,7 @@ GG_INSTRUMENT = 2
 GG_BFS_PUSH_ONLY = 4
 GG_CC_HOOK = 8
 GG_SSSP_GAS = 16
+GG_CC_GAS = 32
 
 GG_TUNE_BFS = 1
 GG_TUNE_SSSP = 2

@@ -47,36 +47,24 @@ __device__ __forceinline__ void hook(int32_t *parent, int32_t u, int32_t v) {
   }
 }
 
-// Low-degree vertices: one thread each; degree >= 64: one wave.
-__global__ void __launch_bounds__(256)
-k_cc_hook(const int32_t *__restrict__ ro, const int32_t *__restrict__ col, int32_t *parent, int32_t n) {
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t base = tid - lane_id(); base < n; base += stride) {
-    const int32_t u = (int32_t)(base + lane_id());
-    int32_t b = 0, e = 0;
-    if (u < n) {
-      b = ro[u];
-      e = ro[u + 1];
-    }
-    const bool big = e - b >= kWave;
-    if (!big)
-      for (int32_t k = b; k < e; k++) {
-        const int32_t v = col[k];
-        if (v < u) hook(parent, u, v);
-      }
-    uint64_t mask = ballot(big);
-    while (mask) {
-      const int l = __builtin_ctzll(mask);
-      mask &= mask - 1;
-      const int32_t wu = __shfl(u, l), wb = __shfl(b, l), we = __shfl(e, l);
-      for (int32_t k = wb + lane_id(); k < we; k += kWave) {
-        const int32_t v = col[k];
-        if (v < wu) hook(parent, wu, v);
-      }
-    }
+// Hooking as an operator of GASengine::k_pull_reduce: the merge-path tiles of
+// (vertices, adjacency) give every workgroup the same number of edges however
+// skewed the degrees (hub rows spread over many workgroups), and each edge
+// (v, u) with u < v hooks once.  No reduction: the LDS accumulator is unused.
+struct HookOp {
+  const int32_t *col;
+  int32_t *parent;
+
+  __device__ __forceinline__ int32_t identity() { return 0; }
+  __device__ __forceinline__ int32_t edge(int32_t v, int32_t slot) {
+    const int32_t u = col[slot];
+    if (u < v) hook(parent, v, u);
+    return 0;
   }
-}
+  __device__ __forceinline__ void combine(int32_t *, int32_t) {}
+  __device__ __forceinline__ void item_done(int32_t, int32_t, bool) {}
+  __device__ __forceinline__ void finish() {}
+};
 
 __global__ void k_cc_compress(int32_t *parent, int32_t n) {
   for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
@@ -98,6 +86,8 @@ int cc_hooking(gg_graph *g, int32_t *label_out, uint32_t flags, gg_stats *stats)
     GG_TRY(g->vtx[1].ensure(4 * (size_t)n));
     parent = g->vtx[1].as<int32_t>();
   }
+  const int32_t *parts = nullptr;
+  GG_TRY(pull_partition(g, &parts));  // merge-path tiles over (vertices, adjacency), cached per graph
   g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
   hipEvent_t e0, e1;
   GG_HIP(hipEventCreate(&e0));
@@ -106,8 +96,9 @@ int cc_hooking(gg_graph *g, int32_t *label_out, uint32_t flags, gg_stats *stats)
   {
     TimedRegion t(g->prof, "cc.hook");
     GG_HIP(sequence_async<int32_t>(parent, n, s));
-    hipLaunchKernelGGL(k_cc_hook, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, g->dg.row_offsets,
-                       g->dg.column_indices, parent, n);
+    hipLaunchKernelGGL((k_pull_reduce<HookOp, int32_t>), dim3(g->opt.expand_grid), dim3(kExpandBlock), 0, s,
+                       HookOp{g->dg.column_indices, parent}, g->dg.row_offsets, n, g->m, parts,
+                       (const int32_t *)nullptr);
     GG_HIP(hipGetLastError());
   }
   {

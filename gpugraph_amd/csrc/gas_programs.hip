@@ -201,6 +201,7 @@ k_bfs_parents(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, co
 // Cores of the public entry points: internal vertex ids, any output pointer.
 int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth, uint32_t flags, gg_stats *stats);
 int cc_hooking(gg_graph *g, int32_t *label, uint32_t flags, gg_stats *stats);
+int cc_label_propagation(gg_graph *g, int32_t *label, uint32_t flags, gg_stats *stats);
 int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist, uint32_t flags, gg_stats *stats);
 
 int bfs_core(gg_graph *g, int32_t src, int32_t *depth, uint32_t flags, gg_stats *stats) {
@@ -217,6 +218,7 @@ int sssp_core(gg_graph *g, int32_t src, int32_t *dist, uint32_t flags, gg_stats 
 
 int cc_core(gg_graph *g, int32_t *label, uint32_t flags, gg_stats *stats) {
   if (flags & GG_CC_HOOK) return cc_hooking(g, label, flags, stats);
+  if (!(flags & GG_CC_GAS)) return cc_label_propagation(g, label, flags, stats);
   // Every vertex is labelled: reach_stats counts all of them.
   return run_program<cc>(g, 0, nullptr, label, flags, stats, INT_MIN, true);
 }

@@ -393,9 +393,13 @@ k_static_partition(const int32_t *__restrict__ offs, int32_t n, int64_t m, int32
   }
 }
 
+// skip: optional device flag; the kernel returns at once when it is nonzero
+// (lets a captured launch sequence switch the pull off).
 template <class Op, class T>
 __global__ void __launch_bounds__(kExpandBlock)
-k_pull_reduce(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const int32_t *__restrict__ parts) {
+k_pull_reduce(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const int32_t *__restrict__ parts,
+              const int32_t *__restrict__ skip) {
+  if (skip && *skip) return;
   constexpr int TILE = kLbsTile;
   constexpr int VT = kExpandVt;
   __shared__ int32_t s_offs[TILE + 2];  // s_offs[k] = offs[i0 - 1 + k], k = 0..ni+1

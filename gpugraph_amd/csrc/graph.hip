@@ -178,7 +178,7 @@ int check_undirected(gg_graph *g) {
   GG_HIP(GASengine::fill_async<unsigned int>(flags, 0u, 1, g->stream));
   MirrorOp op{g->dg.row_offsets, g->dg.column_indices, g->val.as<uint32_t>(), g->eid.as<int32_t>(), flags};
   hipLaunchKernelGGL((GASengine::k_pull_reduce<MirrorOp, uint32_t>), dim3(2048), dim3(GASengine::kExpandBlock), 0,
-                     g->stream, op, g->dg.row_offsets, g->n, g->m, parts);
+                     g->stream, op, g->dg.row_offsets, g->n, g->m, parts, (const int32_t *)nullptr);
   GG_HIP(hipGetLastError());
   unsigned int h = 0;
   GG_HIP(hipMemcpyAsync(&h, flags, sizeof(h), hipMemcpyDeviceToHost, g->stream));

@@ -289,7 +289,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       {
         TimedRegion t(g->prof, "sssp.pull");
         hipLaunchKernelGGL((k_pull_reduce<PullOp, int32_t>), dim3(2048), dim3(kExpandBlock), 0, s, op,
-                           g->dg.column_offsets, n, m, pparts);
+                           g->dg.column_offsets, n, m, pparts, (const int32_t *)nullptr);
         GG_HIP(hipGetLastError());
       }
       const double pb = 4.0 * n + 4.0 * m + n / 8.0 + 8.0 * degsum;

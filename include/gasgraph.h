@@ -63,6 +63,7 @@ extern "C" {
 #define GG_BFS_PUSH_ONLY 4u     /* BFS through the generic GAS program path (no bottom-up) */
 #define GG_CC_HOOK 8u           /* CC by min-id hooking + pointer jumping instead of label propagation */
 #define GG_SSSP_GAS 16u         /* SSSP through the generic GAS program path (reference semantics) */
+#define GG_CC_GAS 32u           /* CC label propagation through the generic GAS program path */
 
 typedef struct gg_graph gg_graph;
 
@@ -167,7 +168,10 @@ int gg_sssp(gg_graph *g, int32_t src, int32_t *dist, uint32_t flags,
 
 /* Connected-component labels = min vertex id of the component.  Replaces
  * EnactIterativeSearch with Program = cc (Algorithms/CC/cc.h:32-349; driver
- * cc.cu:291-292).  Default: GAS label propagation; GG_CC_HOOK: hooking. */
+ * cc.cu:291-292).  Default: the program's label propagation as a
+ * device-driven loop (hipGraph replays, one host check per 16 iterations);
+ * GG_CC_GAS: the same program through the generic GAS path; GG_CC_HOOK:
+ * hooking + pointer jumping.  Undirected graphs. */
 int gg_cc(gg_graph *g, int32_t *label, uint32_t flags, gg_stats *stats);
 
 /* The reference's PageRank program (Algorithms/PageRank/PageRank.h:33-409):

@@ -36,8 +36,8 @@ def test_traversals_undirected(gpu_available):
                 s, _ = g.sssp(src, flags=flags)
                 np.testing.assert_array_equal(s, exp_s)
         exp_cc = oracle.cc(ro, col)[0]
-        for hook in (False, True):
-            lab, _ = g.cc(hook=hook)
+        for mode in ({}, {"gas": True}, {"hook": True}):
+            lab, _ = g.cc(**mode)
             np.testing.assert_array_equal(lab, exp_cc)
         best, _ = g.autotune(programs=_abi.GG_TUNE_BFS | _abi.GG_TUNE_SSSP, src=srcs[0], reps=1)
         s, _ = g.sssp(srcs[0])

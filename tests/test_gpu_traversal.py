@@ -28,9 +28,9 @@ def test_kats(gpu_available, kat_graphs):
                 d, _ = g.sssp(int(s))
                 assert d.tolist() == exp, (k["name"], s)
         with gg.Graph(csr_of(k["sym_row_offsets"], k["sym_col_indices"])) as g:
-            for hook in (False, True):
-                lab, _ = g.cc(hook=hook)
-                assert lab.tolist() == k["cc_sym"].tolist(), (k["name"], hook)
+            for mode in ({}, {"gas": True}, {"hook": True}):
+                lab, _ = g.cc(**mode)
+                assert lab.tolist() == k["cc_sym"].tolist(), (k["name"], mode)
 
 
 def test_config1_erdos_renyi(gpu_available):
@@ -91,18 +91,34 @@ def test_expand_strategies_agree(gpu_available, threshold):
             np.testing.assert_array_equal(d, oracle.bfs(c.row_offsets, c.col_indices, src))
             dist, _ = g.sssp(src, flags=_abi.GG_SSSP_GAS)
             np.testing.assert_array_equal(dist, oracle.sssp(c.row_offsets, c.col_indices, c.values, src))
-            lab, _ = g.cc()
+            lab, _ = g.cc(gas=True)
             np.testing.assert_array_equal(lab, oracle.cc(c.row_offsets, c.col_indices)[0])
 
 
 def test_lattice_cc(gpu_available):
+    """The three CC engines on the percolated lattice (many iterations of
+    label propagation: several graph replays of the device-driven loop) and on
+    R-MAT with its hubs (wave-cooperative rows); expand grid and wave split
+    varied."""
     c = workloads.config3_lattice(side=256)
     exp, _ = oracle.cc(c.row_offsets, c.col_indices)
     with gg.Graph(c) as g:
-        for hook in (False, True):
-            lab, st = g.cc(hook=hook)
-            np.testing.assert_array_equal(lab, exp)
-        assert st["reached_vertices"] == c.n
+        for mode in ({}, {"gas": True}, {"hook": True}):
+            lab, st = g.cc(**mode)
+            np.testing.assert_array_equal(lab, exp, err_msg=str(mode))
+            assert st["reached_vertices"] == c.n
+        lab, st = g.cc()
+        assert st["iterations"] > 16  # more than one replay of the captured loop
+        g.set_options(expand_grid=64, wave_degree=2)
+        lab, _ = g.cc()
+        np.testing.assert_array_equal(lab, exp)
+    r = gg.gen_rmat(14, 16, undirected=True, seed=25)
+    with gg.Graph(r) as g:
+        exp_r = oracle.cc(r.row_offsets, r.col_indices)[0]
+        for grid, wave in ((2048, 128), (64, 8)):
+            g.set_options(expand_grid=grid, wave_degree=wave)
+            lab, _ = g.cc()
+            np.testing.assert_array_equal(lab, exp_r)
 
 
 def test_direction_switch_parameters(gpu_available):
@@ -152,9 +168,10 @@ def test_error_paths(gpu_available):
         with pytest.raises(gg.GGError):
             g.set_options(wave_degree=0)
     with gg.Graph(c, directed=True) as g:
-        with pytest.raises(gg.GGError) as e:
-            g.cc(hook=True)
-        assert e.value.code == _abi.GG_ERR_UNSUPPORTED
+        for hook in (False, True):
+            with pytest.raises(gg.GGError) as e:
+                g.cc(hook=hook)
+            assert e.value.code == _abi.GG_ERR_UNSUPPORTED
     # An undirected graph must be symmetric: 0->1 without 1->0 is rejected.
     with pytest.raises(gg.GGError) as e:
         gg.Graph(csr_of([0, 1, 1], [1]))
