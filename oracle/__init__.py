@@ -35,6 +35,9 @@ def lib():
         L.oracle_bfs_parents.argtypes = [C.c_int32, _I, _I, _I, C.c_int32, _I]
         L.oracle_bfs_validate.argtypes = [C.c_int32, _I, _I, C.c_int32, _I, _I]
         L.oracle_bfs_validate.restype = C.c_int32
+        L.oracle_threads.restype = C.c_int32
+        L.oracle_bfs_omp.argtypes = [C.c_int32, _I, _I, _I, _I, C.c_int32, _I]
+        L.oracle_sssp_omp.argtypes = [C.c_int32, _I, _I, _I, C.c_int32, C.c_int32, _I]
         L.oracle_sssp_selection.argtypes = [C.c_int32, _I, _I, _I, C.c_int32, _I]
         L.oracle_sssp_dijkstra.argtypes = [C.c_int32, _I, _I, _I, C.c_int32, _I]
         L.oracle_cc.argtypes = [C.c_int32, _I, _I, _I]
@@ -75,6 +78,27 @@ def bfs_parents(ro, col, depth, src):
 def bfs_validate(ro, col, src, depth, pred):
     ro, col, depth, pred = (np.ascontiguousarray(x, np.int32) for x in (ro, col, depth, pred))
     return int(lib().oracle_bfs_validate(len(ro) - 1, _i(ro), _i(col), int(src), _i(depth), _i(pred)))
+
+
+def threads():
+    return int(lib().oracle_threads())
+
+
+def bfs_omp(ro, col, src, in_ro=None, in_col=None):
+    """Parallel BFS (OpenMP); in_ro/in_col = in-edges (default: the CSR, for symmetric graphs)."""
+    ro, col = np.ascontiguousarray(ro, np.int32), np.ascontiguousarray(col, np.int32)
+    iro = ro if in_ro is None else np.ascontiguousarray(in_ro, np.int32)
+    icol = col if in_col is None else np.ascontiguousarray(in_col, np.int32)
+    out = np.empty(len(ro) - 1, np.int32)
+    lib().oracle_bfs_omp(len(ro) - 1, _i(ro), _i(col), _i(iro), _i(icol), int(src), out.ctypes.data_as(_I))
+    return out
+
+
+def sssp_omp(ro, col, w, src, delta=128):
+    ro, col, w = (np.ascontiguousarray(x, np.int32) for x in (ro, col, w))
+    out = np.empty(len(ro) - 1, np.int32)
+    lib().oracle_sssp_omp(len(ro) - 1, _i(ro), _i(col), _i(w), int(src), int(delta), out.ctypes.data_as(_I))
+    return out
 
 
 def sssp(ro, col, w, src, selection=False):

@@ -339,3 +339,184 @@ int32_t oracle_pagerank_adjoint_f64(int32_t n, const int32_t *ro,
   free(ln);
   return it;
 }
+
+/* ---- parallel CPU baselines (OpenMP; same outputs as the serial ones) ---- */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int32_t oracle_threads(void) {
+#ifdef _OPENMP
+  return (int32_t)omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* Direction-optimising level-synchronous BFS (top-down with a CAS on the
+ * depth, bottom-up over unvisited vertices when the frontier's out-edges
+ * exceed the unexplored edges / 14), output equal to oracle_bfs.  `in_ro`,
+ * `in_col`: in-edges (the CSR itself for a symmetric graph). */
+void oracle_bfs_omp(int32_t n, const int32_t *ro, const int32_t *col, const int32_t *in_ro, const int32_t *in_col,
+                    int32_t src, int32_t *depth) {
+  for (int32_t i = 0; i < n; i++) depth[i] = -1;
+  if (src < 0 || src >= n) return;
+  int32_t *front = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  int32_t *next = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  int64_t nf = 1, explored = 0;
+  const int64_t m = ro[n];
+  front[0] = src;
+  depth[src] = 0;
+  for (int32_t level = 0; nf > 0; level++) {
+    int64_t mf = 0;
+#pragma omp parallel for reduction(+ : mf) schedule(static)
+    for (int64_t i = 0; i < nf; i++) mf += ro[front[i] + 1] - ro[front[i]];
+    explored += mf;
+    int64_t nn = 0;
+    if (mf > (m - explored) / 14) {
+      /* bottom-up: every unvisited vertex looks for a parent at this level */
+#pragma omp parallel for reduction(+ : nn) schedule(dynamic, 1024)
+      for (int32_t v = 0; v < n; v++) {
+        if (depth[v] != -1) continue;
+        for (int32_t k = in_ro[v]; k < in_ro[v + 1]; k++)
+          if (depth[in_col[k]] == level) {
+            depth[v] = -2 - level; /* found this level; published after the sweep */
+            nn++;
+            break;
+          }
+      }
+      /* publish the level and compact the new frontier (same static chunks
+         in both loops, so each thread's offset is its chunk's prefix) */
+      const int32_t T = oracle_threads();
+      int64_t *cnt = (int64_t *)calloc((size_t)T + 1, sizeof(int64_t));
+#pragma omp parallel num_threads(T)
+      {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num();
+#else
+        const int t = 0;
+#endif
+        int64_t c = 0;
+#pragma omp for schedule(static)
+        for (int32_t v = 0; v < n; v++)
+          if (depth[v] == -2 - level) c++;
+        cnt[t + 1] = c;
+#pragma omp barrier
+#pragma omp single
+        for (int32_t i = 0; i < T; i++) cnt[i + 1] += cnt[i];
+        int64_t pos = cnt[t];
+#pragma omp for schedule(static)
+        for (int32_t v = 0; v < n; v++)
+          if (depth[v] == -2 - level) {
+            depth[v] = level + 1;
+            next[pos++] = v;
+          }
+      }
+      nn = cnt[T];
+      free(cnt);
+    } else {
+#pragma omp parallel
+      {
+        int32_t *local = (int32_t *)malloc(sizeof(int32_t) * 4096);
+        int32_t cnt = 0;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t i = 0; i < nf; i++) {
+          const int32_t u = front[i];
+          for (int32_t k = ro[u]; k < ro[u + 1]; k++) {
+            const int32_t v = col[k];
+            int32_t expect = -1;
+            if (depth[v] == -1 &&
+                __atomic_compare_exchange_n(&depth[v], &expect, level + 1, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+              if (cnt == 4096) {
+                int64_t at = __atomic_fetch_add(&nn, (int64_t)cnt, __ATOMIC_RELAXED);
+                memcpy(next + at, local, sizeof(int32_t) * (size_t)cnt);
+                cnt = 0;
+              }
+              local[cnt++] = v;
+            }
+          }
+        }
+        int64_t at = __atomic_fetch_add(&nn, (int64_t)cnt, __ATOMIC_RELAXED);
+        memcpy(next + at, local, sizeof(int32_t) * (size_t)cnt);
+        free(local);
+      }
+    }
+    int32_t *t = front;
+    front = next;
+    next = t;
+    nf = nn;
+  }
+  free(front);
+  free(next);
+}
+
+/* Bucketed SSSP (near-far: relax from the vertices below the current bound
+ * until none is left, then move the bound to the bucket of the smallest
+ * pending distance), CAS-min on int32 distances, 64-bit sums and the
+ * 100000000 sentinel as oracle_sssp_selection; output equal to it. */
+void oracle_sssp_omp(int32_t n, const int32_t *ro, const int32_t *col, const int32_t *w, int32_t src, int32_t delta,
+                     int32_t *dist) {
+  for (int32_t i = 0; i < n; i++) dist[i] = SSSP_INF;
+  if (src < 0 || src >= n) return;
+  unsigned char *dirty = (unsigned char *)calloc((size_t)(n > 0 ? n : 1), 1);
+  int32_t *front = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  dist[src] = 0;
+  dirty[src] = 1;
+  int64_t bound = delta;
+  for (;;) {
+    int64_t nf = 0, remain = 0;
+    int32_t dmin = INT32_MAX;
+#pragma omp parallel
+    {
+      int32_t *local = (int32_t *)malloc(sizeof(int32_t) * 4096);
+      int32_t cnt = 0, lmin = INT32_MAX;
+      int64_t lrem = 0;
+#pragma omp for schedule(static)
+      for (int32_t v = 0; v < n; v++) {
+        if (!dirty[v]) continue;
+        if (dist[v] < bound) {
+          dirty[v] = 0;
+          if (cnt == 4096) {
+            int64_t at = __atomic_fetch_add(&nf, (int64_t)cnt, __ATOMIC_RELAXED);
+            memcpy(front + at, local, sizeof(int32_t) * (size_t)cnt);
+            cnt = 0;
+          }
+          local[cnt++] = v;
+        } else {
+          lrem++;
+          if (dist[v] < lmin) lmin = dist[v];
+        }
+      }
+      int64_t at = __atomic_fetch_add(&nf, (int64_t)cnt, __ATOMIC_RELAXED);
+      memcpy(front + at, local, sizeof(int32_t) * (size_t)cnt);
+      __atomic_fetch_add(&remain, lrem, __ATOMIC_RELAXED);
+#pragma omp critical
+      if (lmin < dmin) dmin = lmin;
+      free(local);
+    }
+    if (nf == 0) {
+      if (remain == 0) break;
+      bound = ((int64_t)dmin / delta + 1) * delta;
+      continue;
+    }
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < nf; i++) {
+      const int32_t u = front[i];
+      const int64_t du = __atomic_load_n(&dist[u], __ATOMIC_RELAXED);
+      for (int32_t k = ro[u]; k < ro[u + 1]; k++) {
+        const int32_t v = col[k];
+        const int64_t nd = du + w[k];
+        if (nd >= SSSP_INF) continue;
+        int32_t cur = __atomic_load_n(&dist[v], __ATOMIC_RELAXED);
+        while (nd < cur) {
+          if (__atomic_compare_exchange_n(&dist[v], &cur, (int32_t)nd, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+            dirty[v] = 1;
+            break;
+          }
+        }
+      }
+    }
+  }
+  free(dirty);
+  free(front);
+}

@@ -22,6 +22,8 @@
  *   oracle_pr_reference        pinned: dense solve of its fixed point (tol)
  *   oracle_pagerank_f64        pinned: dense FP64 linear solve + sum(r)=1
  *   oracle_pagerank_adjoint_f64 pinned: finite differences + 3-vertex hand case
+ *   oracle_bfs_omp, oracle_sssp_omp  pinned: equal to oracle_bfs / oracle_sssp
+ *                              (the parallel CPU baselines)
  */
 #ifndef GG_ORACLE_H
 #define GG_ORACLE_H
@@ -89,6 +91,17 @@ int32_t oracle_pagerank_adjoint_f64(int32_t n, const int32_t *ro,
                                     double tol, int32_t max_iter,
                                     double *lambda, double *grad_w,
                                     double *grad_t);
+
+/* Parallel CPU baselines (OpenMP, oracle_threads() threads): outputs equal
+ * to oracle_bfs / oracle_sssp_selection.  in_ro / in_col: in-edges (the
+ * CSR itself for a symmetric graph); delta: bucket width (e.g. 128). */
+int32_t oracle_threads(void);
+void oracle_bfs_omp(int32_t n, const int32_t *ro, const int32_t *col,
+                    const int32_t *in_ro, const int32_t *in_col, int32_t src,
+                    int32_t *depth);
+void oracle_sssp_omp(int32_t n, const int32_t *ro, const int32_t *col,
+                     const int32_t *w, int32_t src, int32_t delta,
+                     int32_t *dist);
 
 #ifdef __cplusplus
 }

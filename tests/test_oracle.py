@@ -284,3 +284,27 @@ def test_bfs_validate_rejects_each_rule():
     cand = [int(u) for u in np.flatnonzero(d == d[v] - 1) if u not in set(rows[col == v].tolist())]
     p6 = p.copy(); p6[v] = cand[0]
     assert oracle.bfs_validate(ro, col, src, d, p6) == 5
+
+
+@pytest.mark.parametrize("seed", [9, 10])
+def test_parallel_baselines_equal_serial(seed):
+    """The OpenMP CPU baselines return exactly the serial oracle's outputs
+    (directed graph with in-edges from oracle.csc; symmetric graph)."""
+    ro, col, w = random_csr(5000, 40000, seed)
+    co, ri = oracle.csc(ro, col)[:2]
+    for src in (0, 4999):
+        np.testing.assert_array_equal(oracle.bfs_omp(ro, col, src, co, ri), oracle.bfs(ro, col, src))
+        for delta in (1, 16, 1000):
+            np.testing.assert_array_equal(oracle.sssp_omp(ro, col, w, src, delta), oracle.sssp(ro, col, w, src))
+    g = gg_rmat_sym(seed)
+    src = int(np.argmax(np.diff(g[0])))
+    np.testing.assert_array_equal(oracle.bfs_omp(g[0], g[1], src), oracle.bfs(g[0], g[1], src))
+    np.testing.assert_array_equal(oracle.sssp_omp(g[0], g[1], g[2], src), oracle.sssp(g[0], g[1], g[2], src))
+    assert oracle.threads() >= 1
+
+
+def gg_rmat_sym(seed):
+    import gpugraph_amd as gg
+
+    c = gg.gen_rmat(13, 16, undirected=True, seed=seed).with_weights("i32", 1, 64, symmetric=True, seed=seed)
+    return c.row_offsets, c.col_indices, c.values
