@@ -389,6 +389,28 @@ __device__ __forceinline__ void lane_add_f64(double *acc_per_instance, double v,
   __syncthreads();
 }
 
+// acc += R[u][l] * P(u -> v) over in-edge slots [b, e) in order, four
+// source rows (256 B each, one per edge for the whole wave) in flight.
+__device__ __forceinline__ void ppr_row_sum(const PprArgs &a, const float *rc, int32_t b, int32_t e, int l, KSum &acc) {
+  int32_t k = b;
+  for (; k + 3 < e; k += 4) {
+    float x[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int32_t u = a.ri[k + q];
+      const float p = a.pcsc ? a.pcsc[k + q] : a.inv_out[u];
+      x[q] = rc[(int64_t)u * kPprMax + l] * p;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc.add(x[q]);
+  }
+  for (; k < e; k++) {
+    const int32_t u = a.ri[k];
+    const float p = a.pcsc ? a.pcsc[k] : a.inv_out[u];
+    acc.add(rc[(int64_t)u * kPprMax + l] * p);
+  }
+}
+
 __device__ __forceinline__ void ppr_finish_vertex(const PprArgs &a, BatchState *st, int32_t v, float sum,
                                                   const float *rc, float *rn, double &resid, double &dang) {
   const int l = lane_id();
@@ -414,11 +436,7 @@ k_ppr_sweep(PprArgs a, BatchState *st) {
     const int32_t b = a.co[v], e = a.co[v + 1];
     if (e - b > kLongRow) continue;
     KSum acc;
-    for (int32_t k = b; k < e; k++) {
-      const int32_t u = a.ri[k];
-      const float p = a.pcsc ? a.pcsc[k] : a.inv_out[u];
-      acc.add(rc[(int64_t)u * kPprMax + l] * p);
-    }
+    ppr_row_sum(a, rc, b, e, l, acc);
     ppr_finish_vertex(a, st, (int32_t)v, acc.value(), rc, rn, resid, dang);
   }
   lane_add_f64<kRowBlock>(st->resid_acc, resid, s_red);
@@ -448,11 +466,7 @@ k_ppr_long_partial(PprArgs a, BatchState *st, const int32_t *__restrict__ longv,
     const int32_t b = a.co[v] + c * kLongRow;
     const int32_t e = min(a.co[v + 1], b + kLongRow);
     KSum acc;
-    for (int32_t k = b; k < e; k++) {
-      const int32_t u = a.ri[k];
-      const float p = a.pcsc ? a.pcsc[k] : a.inv_out[u];
-      acc.add(rc[(int64_t)u * kPprMax + l] * p);
-    }
+    ppr_row_sum(a, rc, b, e, l, acc);
     part[t * kPprMax + l] = make_float2(acc.s, acc.c);
   }
 }

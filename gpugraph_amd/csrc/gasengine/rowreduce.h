@@ -38,6 +38,23 @@ struct KSum {
   __device__ __forceinline__ float value() const { return s + c; }
 };
 
+// acc += edge(v, k) for k = first, first + stride, ... < end, in that order,
+// four edges' loads issued before their adds (the order, and so the result,
+// is that of the plain loop).
+template <class Op>
+__device__ __forceinline__ void strided_sum(Op &op, int32_t v, int32_t first, int32_t end, int32_t stride, KSum &acc) {
+  int32_t k = first;
+  for (; k + 3 * stride < end; k += 4 * stride) {
+    const float x0 = op.edge(v, k), x1 = op.edge(v, k + stride);
+    const float x2 = op.edge(v, k + 2 * stride), x3 = op.edge(v, k + 3 * stride);
+    acc.add(x0);
+    acc.add(x1);
+    acc.add(x2);
+    acc.add(x3);
+  }
+  for (; k < end; k += stride) acc.add(op.edge(v, k));
+}
+
 // Butterfly merge across the wave; lane 0 ends with the deterministic result.
 __device__ __forceinline__ void wave_merge(KSum &a) {
 #pragma unroll
@@ -55,8 +72,7 @@ k_rows_small(Op op, const int32_t *__restrict__ list, int32_t count, const int32
   for (int64_t i = (int64_t)blockIdx.x * kRowBlock + threadIdx.x; i < count; i += (int64_t)gridDim.x * kRowBlock) {
     const int32_t v = list[i];
     KSum acc;
-    const int32_t e = op.row_end(v);
-    for (int32_t k = op.row_begin(v); k < e; k++) acc.add(op.edge(v, k));
+    strided_sum(op, v, op.row_begin(v), op.row_end(v), 1, acc);
     op.item_done(v, acc.value());
   }
   op.finish();
@@ -71,8 +87,7 @@ k_rows_wave(Op op, const int32_t *__restrict__ list, int32_t count, const int32_
   for (int64_t i = (int64_t)blockIdx.x * (kRowBlock / kWave) + threadIdx.x / kWave; i < count; i += waves) {
     const int32_t v = list[i];
     KSum acc;
-    const int32_t e = op.row_end(v);
-    for (int32_t k = op.row_begin(v) + lane_id(); k < e; k += kWave) acc.add(op.edge(v, k));
+    strided_sum(op, v, op.row_begin(v) + lane_id(), op.row_end(v), kWave, acc);
     wave_merge(acc);
     if (lane_id() == 0) op.item_done(v, acc.value());
   }
@@ -88,8 +103,7 @@ k_rows_block(Op op, const int32_t *__restrict__ list, int32_t count, const int32
   for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
     const int32_t v = list[i];
     KSum acc;
-    const int32_t e = op.row_end(v);
-    for (int32_t k = op.row_begin(v) + threadIdx.x; k < e; k += kRowBlock) acc.add(op.edge(v, k));
+    strided_sum(op, v, op.row_begin(v) + (int32_t)threadIdx.x, op.row_end(v), kRowBlock, acc);
     wave_merge(acc);
     if (lane_id() == 0) {
       s_s[threadIdx.x / kWave] = acc.s;

@@ -140,7 +140,7 @@ k_slice_small(Op op, SliceLists L, float2 *__restrict__ part, const int32_t *__r
     const int32_t b = so[v], e = so[v + 1];
     if (e - b > kRowSmallMax) continue;
     KSum acc;
-    for (int32_t k = b; k < e; k++) acc.add(op.edge((int32_t)v, k));
+    strided_sum(op, (int32_t)v, b, e, 1, acc);
     ps[v] = make_float2(acc.s, acc.c);
   }
 }
@@ -160,7 +160,7 @@ k_slice_mid(Op op, SliceLists L, float2 *__restrict__ part, const int32_t *__res
     const int32_t v = sv - s * L.n;
     const int32_t b = L.off[sv], e = L.off[sv + 1];
     KSum acc;
-    for (int32_t k = b + lane_id(); k < e; k += kWave) acc.add(op.edge(v, k));
+    strided_sum(op, v, b + lane_id(), e, kWave, acc);
     wave_merge(acc);
     if (lane_id() == 0) part[sv] = make_float2(acc.s, acc.c);
   }
@@ -181,7 +181,7 @@ k_slice_large(Op op, SliceLists L, float2 *__restrict__ part, const int32_t *__r
     const int32_t v = sv - s * L.n;
     const int32_t b = L.off[sv], e = L.off[sv + 1];
     KSum acc;
-    for (int32_t k = b + (int32_t)threadIdx.x; k < e; k += kRowBlock) acc.add(op.edge(v, k));
+    strided_sum(op, v, b + (int32_t)threadIdx.x, e, kRowBlock, acc);
     wave_merge(acc);
     if (lane_id() == 0) {
       s_s[threadIdx.x / kWave] = acc.s;
