@@ -186,7 +186,11 @@ def headline(args, gg, _abi, workloads, torch, np):
               "options": "autotuned (gg_autotune, BFS + SSSP) before timing" if tuned else "defaults",
               "vertex_order": "caller ids, stored renumbered by degree (GG_GRAPH_REORDER)" if not args.no_reorder
               else "as generated"}
+    if tuned is not None:
+        per_alg["autotune"] = tuned
     # CC on the same graph (label propagation and hooking), outside the timed steps.
+    if args.no_extras:
+        return value, wall, roofline, kernels, per_alg, cpu, config
     with gg.Graph(csr, reorder=not args.no_reorder) as gc:
         cc = {}
         for name, mode in (("labelprop", {}), ("hook", {"hook": True})):
@@ -199,8 +203,6 @@ def headline(args, gg, _abi, workloads, torch, np):
             b0, _ = median_ms(lambda: g0.bfs(src, flags=flags, out=out_bfs), 5)
             s0, _ = median_ms(lambda: g0.sssp(src, flags=flags, out=out_sssp), 5)
         per_alg["generated_order"] = {"bfs_ms": b0, "sssp_ms": s0}
-    if tuned is not None:
-        per_alg["autotune"] = tuned
     return value, wall, roofline, kernels, per_alg, cpu, config
 
 
