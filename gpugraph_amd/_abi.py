@@ -37,6 +37,9 @@ GG_BFS_PUSH_ONLY = 4
 GG_CC_HOOK = 8
 GG_SSSP_GAS = 16
 GG_CC_GAS = 32
+GG_BFS_REVERSE = 64
+GG_CC_WEAK = 128
+GG_PR_ALL_EDGES = 256
 
 GG_TUNE_BFS = 1
 GG_TUNE_SSSP = 2

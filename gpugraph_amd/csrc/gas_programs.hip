@@ -13,6 +13,7 @@
 #include "programs/cc.h"
 #include "programs/pagerank.h"
 #include "programs/sssp.h"
+#include "programs/variants.h"
 
 namespace gg {
 
@@ -59,6 +60,25 @@ struct Slot<pagerank> {
   static constexpr int default_threshold = 1000;  // Algorithms/PageRank/register.h:40
 };
 
+template <>
+struct Slot<bfs_reverse> {
+  static constexpr int index = 4;
+  static constexpr const char *name = "bfs_reverse";
+  static constexpr int default_threshold = 0;
+};
+template <>
+struct Slot<cc_weak> {
+  static constexpr int index = 5;
+  static constexpr const char *name = "cc_weak";
+  static constexpr int default_threshold = 10000;
+};
+template <>
+struct Slot<pagerank_all> {
+  static constexpr int index = 6;
+  static constexpr const char *name = "pagerank_all";
+  static constexpr int default_threshold = 1000;
+};
+
 template <class P>
 using Problem = GASengine::CsrProblem<P>;
 
@@ -91,8 +111,8 @@ template <class P>
 double iteration_bytes(const GASengine::IterationRecord &r) {
   const double F = r.frontier, E = (double)r.edges, N = r.next;
   if (std::is_same<P, sssp>::value) return 16 * F + 12 * E + 8 * N;
-  if (std::is_same<P, cc>::value) return 16 * F + 8 * E + 8 * N;
-  if (std::is_same<P, bfs>::value) return 12 * F + 8 * E + 8 * N;
+  if (std::is_same<P, cc>::value || std::is_same<P, cc_weak>::value) return 16 * F + 8 * E + 8 * N;
+  if (std::is_same<P, bfs>::value || std::is_same<P, bfs_reverse>::value) return 12 * F + 8 * E + 8 * N;
   return 12 * F + 12 * E + 8 * N;  // pagerank: gather reads ri + rank + outdeg per in-edge
 }
 
@@ -205,8 +225,9 @@ int cc_label_propagation(gg_graph *g, int32_t *label, uint32_t flags, gg_stats *
 int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist, uint32_t flags, gg_stats *stats);
 
 int bfs_core(gg_graph *g, int32_t src, int32_t *depth, uint32_t flags, gg_stats *stats) {
-  if (!(flags & GG_BFS_PUSH_ONLY)) return bfs_direction_optimizing(g, src, depth, flags, stats);
   int s = src;
+  if (flags & GG_BFS_REVERSE) return run_program<bfs_reverse>(g, 1, &s, depth, flags, stats, -1, true);
+  if (!(flags & GG_BFS_PUSH_ONLY)) return bfs_direction_optimizing(g, src, depth, flags, stats);
   return run_program<bfs>(g, 1, &s, depth, flags, stats, -1, true);
 }
 
@@ -217,6 +238,10 @@ int sssp_core(gg_graph *g, int32_t src, int32_t *dist, uint32_t flags, gg_stats 
 }
 
 int cc_core(gg_graph *g, int32_t *label, uint32_t flags, gg_stats *stats) {
+  if (flags & GG_CC_WEAK) {
+    if (!g->directed) return set_error(GG_ERR_INVALID, "GG_CC_WEAK is for directed graphs (use gg_cc without it)");
+    return run_program<cc_weak>(g, 0, nullptr, label, flags, stats, INT_MIN, true);
+  }
   if (flags & GG_CC_HOOK) return cc_hooking(g, label, flags, stats);
   if (!(flags & GG_CC_GAS)) return cc_label_propagation(g, label, flags, stats);
   // Every vertex is labelled: reach_stats counts all of them.
@@ -252,6 +277,7 @@ int gg_bfs_tree(gg_graph *g, int32_t src, int32_t *depth, int32_t *pred, uint32_
   gg::clear_error();
   GG_CHECK_ARG(g && pred, "graph/pred is NULL");
   GG_CHECK_ARG(src >= 0 && src < g->n, "src %d out of range [0, %d)", src, g->n);
+  GG_CHECK_ARG(!(flags & GG_BFS_REVERSE), "gg_bfs_tree builds trees of forward BFS only");
   GG_TRY(activate(g));
   const bool dev = (flags & GG_OUT_DEVICE) && !g->reordered;
   const size_t bytes = 4 * (size_t)g->n;
@@ -312,10 +338,15 @@ int gg_pagerank_reference(gg_graph *g, float *rank, uint32_t flags, gg_stats *st
   using namespace gg;
   gg::clear_error();
   GG_CHECK_ARG(g, "graph is NULL");
-  if (!g->reordered) return run_program<pagerank>(g, 0, nullptr, rank, flags, stats, 0.0f, false);
-  return with_mapped_output(g, rank, flags, [&](int32_t *o, uint32_t f) {
-    return run_program<pagerank>(g, 0, nullptr, reinterpret_cast<float *>(o), f, stats, 0.0f, false);
-  });
+  const bool all = flags & GG_PR_ALL_EDGES;
+  GG_CHECK_ARG(!all || g->directed, "GG_PR_ALL_EDGES is for directed graphs");
+  auto run = [&](float *o, uint32_t f) {
+    return all ? run_program<pagerank_all>(g, 0, nullptr, o, f, stats, 0.0f, false)
+               : run_program<pagerank>(g, 0, nullptr, o, f, stats, 0.0f, false);
+  };
+  if (!g->reordered) return run(rank, flags);
+  return with_mapped_output(g, rank, flags,
+                            [&](int32_t *o, uint32_t f) { return run(reinterpret_cast<float *>(o), f); });
 }
 
 }  // extern "C"

@@ -186,7 +186,7 @@ struct gg_graph {
   bool mirror_ok = false, values_asymmetric = false;
 
   // Generic-program state kept across runs (one per program type).
-  std::shared_ptr<void> prog_state[4];
+  std::shared_ptr<void> prog_state[8];
 
   gg::EventProfiler prof;
 

@@ -64,6 +64,11 @@ extern "C" {
 #define GG_CC_HOOK 8u           /* CC by min-id hooking + pointer jumping instead of label propagation */
 #define GG_SSSP_GAS 16u         /* SSSP through the generic GAS program path (reference semantics) */
 #define GG_CC_GAS 32u           /* CC label propagation through the generic GAS program path */
+#define GG_BFS_REVERSE 64u      /* BFS over in-edges (hops to src), GAS path, EXPAND_IN_EDGES */
+#define GG_CC_WEAK 128u         /* directed graphs: weakly connected components, GAS path,
+                                   EXPAND_ALL_EDGES */
+#define GG_PR_ALL_EDGES 256u    /* gg_pagerank_reference on a directed graph read as undirected:
+                                   GATHER_ALL_EDGES, degree = out + in */
 
 typedef struct gg_graph gg_graph;
 

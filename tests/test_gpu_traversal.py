@@ -276,3 +276,48 @@ def test_bfs_tree(gpu_available, kat_graphs):
                 g.synchronize()
                 np.testing.assert_array_equal(dd.cpu().numpy(), d)
                 np.testing.assert_array_equal(pd.cpu().numpy(), p)
+
+
+def symmetrized(c):
+    """The directed graph read as undirected: every edge in both directions
+    (a multigraph where both u->v and v->u exist)."""
+    rows = np.repeat(np.arange(c.n), c.degrees())
+    su = np.concatenate([rows, c.col_indices])
+    sv = np.concatenate([c.col_indices, rows])
+    order = np.lexsort((sv, su))
+    su, sv = su[order], sv[order]
+    ro = np.concatenate([[0], np.cumsum(np.bincount(su, minlength=c.n))]).astype(np.int32)
+    return ro, sv.astype(np.int32)
+
+
+def test_in_and_all_edge_programs(gpu_available):
+    """Program variants through the generic engine's in-edge and all-edge
+    paths on a directed graph: reverse BFS (EXPAND_IN_EDGES) equals BFS of the
+    transpose; weakly connected components (EXPAND_ALL_EDGES) equal CC of the
+    symmetrized graph; the reference PageRank over all edges
+    (GATHER_ALL_EDGES, degree out + in) matches CPUPR of the symmetrized
+    graph within its own tolerance.  Also on a renumbered graph."""
+    c = gg.gen_rmat(13, 8, undirected=False, seed=91)
+    co, ri = oracle.csc(c.row_offsets, c.col_indices)[:2]
+    sro, scol = symmetrized(c)
+    indeg = np.diff(co)
+    srcs = (0, int(np.argmax(indeg)))
+    exp_cc = oracle.cc(sro, scol)[0]
+    for reorder in (False, True):
+        with gg.Graph(c, directed=True, reorder=reorder) as g:
+            for src in srcs:
+                d, _ = g.bfs(src, flags=_abi.GG_BFS_REVERSE)
+                np.testing.assert_array_equal(d, oracle.bfs(co, ri, src), err_msg=f"reorder={reorder} src={src}")
+            lab, _ = g.cc(flags=_abi.GG_CC_WEAK)
+            np.testing.assert_array_equal(lab, exp_cc)
+            r, st = g.pagerank_reference(flags=_abi.GG_PR_ALL_EDGES)
+            exp, _ = oracle.pr_reference(sro, scol)
+            assert np.linalg.norm(r.astype(np.float64) - exp) / np.linalg.norm(exp) < 0.02
+            with pytest.raises(gg.GGError):
+                g.bfs_tree(0, flags=_abi.GG_BFS_REVERSE)
+    u = gg.gen_rmat(10, 8, undirected=True, seed=92)
+    with gg.Graph(u) as g:
+        with pytest.raises(gg.GGError):
+            g.cc(flags=_abi.GG_CC_WEAK)
+        with pytest.raises(gg.GGError):
+            g.pagerank_reference(flags=_abi.GG_PR_ALL_EDGES)
