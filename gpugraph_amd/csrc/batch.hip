@@ -272,18 +272,20 @@ k_bs_relax(BatchState *st, const int32_t *__restrict__ ro, const int32_t *__rest
         m1[q] = act[(int64_t)u * 4 + 2 + (lane >> 5)];
       }
       // Row loads only for active instances: a sparse mask touches a few
-      // cache lines instead of both 256 B halves of two rows.
+      // cache lines instead of both 256 B halves of two rows.  The target
+      // row is read with plain (cacheable) loads: a stale, larger distance
+      // only lets an unneeded atomicMin through, which returns the truth.
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int64_t urow = (int64_t)uu[q] * kSsspMax, vrow = (int64_t)vv[q] * kSsspMax;
         du0[q] = dv0[q] = du1[q] = dv1[q] = kInf;
         if ((m0[q] >> (lane & 31)) & 1u) {
           du0[q] = dist[urow + lane];
-          dv0[q] = __atomic_load_n(&dist[vrow + lane], __ATOMIC_RELAXED);
+          dv0[q] = dist[vrow + lane];
         }
         if ((m1[q] >> (lane & 31)) & 1u) {
           du1[q] = dist[urow + 64 + lane];
-          dv1[q] = __atomic_load_n(&dist[vrow + 64 + lane], __ATOMIC_RELAXED);
+          dv1[q] = dist[vrow + 64 + lane];
         }
       }
 #pragma unroll

@@ -162,6 +162,7 @@ k_lp_expand(CcState *st, const int32_t *__restrict__ ro, const int32_t *__restri
 
 // Pull operator for GASengine::k_pull_reduce over the (symmetric) adjacency.
 struct CcPullOp {
+  static constexpr bool kPureEdge = true;
   const int32_t *nbr;
   int32_t *label;
   CcState *st;                    // output bitmap st->b_nxt (cleared before the pull), count st->pull_changed

@@ -35,11 +35,34 @@
 //   bool    edge(int32_t v, int32_t val, int32_t slot, int32_t &out)
 //   void    finish()                     once per thread at kernel end (all
 //                                        threads; per-thread accumulators)
+// Optional staged form (k_lbs_expand evaluates a thread's VT edges of a tile
+// in two passes, so the loads of all VT edges are in flight together; tail
+// slots are clamped to the tile's last edge and their loads discarded):
+//   using Loads = ...                    what one edge's loads produce
+//   Loads load(int32_t v, int32_t val, int32_t slot)   loads only, no effects
+//   bool  apply(int32_t v, Loads l, int32_t &out)      as edge(), given l
 #pragma once
 
 #include "util.h"
 
 namespace GASengine {
+
+template <class Op, class = void>
+struct StagedEdge {
+  static constexpr bool value = false;
+};
+template <class Op>
+struct StagedEdge<Op, decltype((void)sizeof(typename Op::Loads))> {
+  static constexpr bool value = true;
+};
+template <class Op, bool = StagedEdge<Op>::value>
+struct StagedEdgeLoads {
+  using type = int32_t;
+};
+template <class Op>
+struct StagedEdgeLoads<Op, true> {
+  using type = typename Op::Loads;
+};
 
 constexpr int kCountBlock = 256;
 constexpr int kCountIpt = 4;
@@ -219,15 +242,37 @@ k_lbs_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__restr
       }
     }
     __syncthreads();
-    for (int32_t base = 0; base < ne; base += kExpandBlock) {
-      const int32_t k = base + threadIdx.x;
-      bool keep = false;
-      int32_t out = 0;
-      if (k < ne) {
-        const int32_t s = s_slot[k];
-        keep = op.edge(s_vert[s], Op::kItemValues ? s_val[s] : 0, s_begin[s] + (j0 + k - s_offs[s]), out);
+    if constexpr (StagedEdge<Op>::value) {
+      if (ne > 0) {  // ne <= TILE = VT * kExpandBlock: one pass
+        typename StagedEdgeLoads<Op>::type pre[VT];
+        int32_t sl[VT];
+#pragma unroll
+        for (int r = 0; r < VT; r++) {
+          const int32_t k = min((int32_t)threadIdx.x + r * kExpandBlock, ne - 1);
+          const int32_t s = s_slot[k];
+          sl[r] = s;
+          pre[r] = op.load(s_vert[s], Op::kItemValues ? s_val[s] : 0, s_begin[s] + (j0 + k - s_offs[s]));
+        }
+#pragma unroll
+        for (int r = 0; r < VT; r++) {
+          const int32_t k = (int32_t)threadIdx.x + r * kExpandBlock;
+          bool keep = false;
+          int32_t out = 0;
+          if (k < ne) keep = op.apply(s_vert[sl[r]], pre[r], out);
+          wave_append(keep, out, s_out, &s_nout);
+        }
       }
-      wave_append(keep, out, s_out, &s_nout);  // LDS atomics, one per wave
+    } else {
+      for (int32_t base = 0; base < ne; base += kExpandBlock) {
+        const int32_t k = base + threadIdx.x;
+        bool keep = false;
+        int32_t out = 0;
+        if (k < ne) {
+          const int32_t s = s_slot[k];
+          keep = op.edge(s_vert[s], Op::kItemValues ? s_val[s] : 0, s_begin[s] + (j0 + k - s_offs[s]), out);
+        }
+        wave_append(keep, out, s_out, &s_nout);  // LDS atomics, one per wave
+      }
     }
     __syncthreads();
     const int32_t nout = s_nout;
@@ -373,6 +418,19 @@ k_frontier_prologue(Op op, const int32_t *__restrict__ frontier, const int32_t *
 //   void combine(T *lds_acc, T val)               LDS reduction (e.g. atomicMin)
 //   void item_done(int32_t v, T acc, bool whole)  result of v for this tile
 //   void finish()
+//   static constexpr bool kPureEdge (optional)   edge() only loads: the kernel
+//        then evaluates a thread's VT edges back to back (tail slots clamped
+//        to the tile's last edge, their values dropped) so that each load
+//        stage of all VT edges is in flight at once, and combines afterwards.
+
+template <class Op, class = void>
+struct PureEdge {
+  static constexpr bool value = false;
+};
+template <class Op>
+struct PureEdge<Op, decltype((void)Op::kPureEdge)> {
+  static constexpr bool value = Op::kPureEdge;
+};
 
 template <int DUMMY = 0>
 __global__ void __launch_bounds__(256)
@@ -443,10 +501,30 @@ k_pull_reduce(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
       }
     }
     __syncthreads();
-    for (int32_t k = threadIdx.x; k < ne; k += kExpandBlock) {
-      const int32_t s = s_slot[k];
-      const T val = op.edge(i0 - 1 + s, j0 + k);
-      op.combine(&s_acc[s], val);
+    if constexpr (PureEdge<Op>::value) {
+      if (ne > 0) {  // ne <= TILE = VT * kExpandBlock: one pass
+        int32_t sl[VT];
+        T val[VT];
+#pragma unroll
+        for (int r = 0; r < VT; r++) {
+          const int32_t k = min((int32_t)threadIdx.x + r * kExpandBlock, ne - 1);
+          sl[r] = s_slot[k];
+        }
+#pragma unroll
+        for (int r = 0; r < VT; r++) {
+          const int32_t k = min((int32_t)threadIdx.x + r * kExpandBlock, ne - 1);
+          val[r] = op.edge(i0 - 1 + sl[r], j0 + k);
+        }
+#pragma unroll
+        for (int r = 0; r < VT; r++)
+          if ((int32_t)threadIdx.x + r * kExpandBlock < ne) op.combine(&s_acc[sl[r]], val[r]);
+      }
+    } else {
+      for (int32_t k = threadIdx.x; k < ne; k += kExpandBlock) {
+        const int32_t s = s_slot[k];
+        const T val = op.edge(i0 - 1 + s, j0 + k);
+        op.combine(&s_acc[s], val);
+      }
     }
     __syncthreads();
     for (int k = threadIdx.x; k <= ni; k += kExpandBlock) {

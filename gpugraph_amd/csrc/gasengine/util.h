@@ -129,10 +129,12 @@ __device__ __forceinline__ bool bit_test(const uint32_t *bits, int32_t v) {
 }
 
 // Atomic test-and-set; true iff this call flipped the bit from 0 to 1.  A
-// plain read first skips the atomic for bits already set.
+// plain (cacheable) read first skips the atomic for bits already set; bits
+// are only set while kernels run, so a stale read can only miss a set bit,
+// and the atomic's returned word then decides.
 __device__ __forceinline__ bool bit_test_and_set(uint32_t *bits, int32_t v) {
   const uint32_t m = 1u << (v & 31);
-  if (__atomic_load_n(&bits[v >> 5], __ATOMIC_RELAXED) & m) return false;
+  if (bits[v >> 5] & m) return false;
   return (atomicOr(&bits[v >> 5], m) & m) == 0;
 }
 

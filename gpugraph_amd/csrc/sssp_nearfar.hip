@@ -9,21 +9,22 @@
 // (GG_SSSP_GAS) keeps the reference's own schedule.
 //
 // State: dist[n], dirty[n] (byte: distance dropped since the vertex was last
-// expanded), frontier queue + frontier bitmap.  Per iteration:
-//   collect  every dirty vertex with dist < bound joins the frontier (queue
-//            and bitmap, dirty cleared); counts the frontier's out-edges and
+// expanded), frontier queue.  Per iteration:
+//   collect  every dirty vertex with dist < bound joins the frontier (queue,
+//            dirty cleared); counts the frontier's out-edges and
 //            the smallest distance still dirty.  Empty frontier with dirty
 //            vertices left: bound jumps to the bucket of that distance.
 //   push     frontier edges <= m / pull_alpha: LBS expand over the queue,
 //            nd = dist[v] + w, plain pre-check then atomicMin(dist[nb]); an
 //            improvement stores dirty[nb] = 1 (no atomic, no enqueue).
-//   pull     otherwise: merge-path pass over every vertex's in-edges,
-//            min over frontier in-neighbours of dist[u] + w reduced in LDS,
-//            written with a plain store when the tile holds all of the
-//            vertex's in-edges (atomicMin at tile seams).
+//   pull     otherwise: merge-path pass over every vertex's in-edges, min
+//            of dist[u] + w over all in-neighbours u reduced in LDS (each
+//            thread's edges loaded back to back), written with a plain store
+//            when the tile holds all of the vertex's in-edges (atomicMin at
+//            tile seams).
 //
-// Byte model: collect n + n/8 + 4(dirty) + 4F; push 16F + 12E + 8N;
-// pull 4n + 4m + n/8 + 8E_F + 8N (E_F = out-edges of the frontier).
+// Byte model: collect n + 4(dirty) + 4F; push 16F + 12E + 8N;
+// pull 4n + 12m (offsets; source, weight and distance per in-edge).
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -71,28 +72,40 @@ struct PushOp {
   __device__ __forceinline__ int32_t degree(int32_t v) { return ro[v + 1] - ro[v]; }
   __device__ __forceinline__ int32_t row_begin(int32_t v) { return ro[v]; }
   __device__ __forceinline__ int32_t item_value(int32_t v) { return dist[v]; }
-  __device__ __forceinline__ bool edge(int32_t, int32_t dv, int32_t slot, int32_t &) {
-    const int32_t nb = col[slot];
-    const int64_t nd64 = (int64_t)dv + w[slot];
-    const int32_t cur = __atomic_load_n(&dist[nb], __ATOMIC_RELAXED);
-    if (nd64 >= cur) return false;
-    const int32_t nd = (int32_t)nd64;
-    const int32_t old = atomicMin(&dist[nb], nd);
+  // Staged form: column and weight of all of a thread's edges are loaded
+  // before the first distance probe.  The probe is a plain (cacheable) load
+  // made just before the edge's atomic, so it sees the thread's own earlier
+  // updates; a stale, larger value only lets an unneeded atomicMin through,
+  // whose returned old value filters it.
+  struct Loads {
+    int32_t nb;
+    int64_t nd;
+  };
+  __device__ __forceinline__ Loads load(int32_t, int32_t dv, int32_t slot) { return {col[slot], (int64_t)dv + w[slot]}; }
+  __device__ __forceinline__ bool apply(int32_t, const Loads &l, int32_t &) {
+    if (l.nd >= dist[l.nb]) return false;
+    const int32_t nd = (int32_t)l.nd;
+    const int32_t old = atomicMin(&dist[l.nb], nd);
     if (nd >= old) return false;
-    tally.add(old, ro, nb);
-    dirty[nb] = 1;
+    tally.add(old, ro, l.nb);
+    dirty[l.nb] = 1;
     return false;
+  }
+  __device__ __forceinline__ bool edge(int32_t v, int32_t dv, int32_t slot, int32_t &out) {
+    return apply(v, load(v, dv, slot), out);
   }
   __device__ __forceinline__ void finish() { tally.flush(counters); }
 };
 
-__device__ __forceinline__ bool fbit(const uint32_t *__restrict__ b, int32_t u) { return (b[u >> 5] >> (u & 31)) & 1u; }
-
+// Relaxes every in-edge, not only those of frontier vertices: the pass reads
+// all in-edges anyway, dist[u] + w is a path length whatever u's state, and
+// distances lowered earlier in the same pass propagate at once.  dist[u] is a
+// plain load (it may be stale, i.e. larger, and then only relaxes less).
 struct PullOp {
+  static constexpr bool kPureEdge = true;
   const int32_t *ro;
   const int32_t *ri;
   const int32_t *wc;
-  const uint32_t *front;
   int32_t *dist;
   uint8_t *dirty;
   unsigned long long *counters;
@@ -100,9 +113,7 @@ struct PullOp {
 
   __device__ __forceinline__ int32_t identity() { return INT_MAX; }
   __device__ __forceinline__ int32_t edge(int32_t, int32_t slot) {
-    const int32_t u = ri[slot];
-    if (!fbit(front, u)) return INT_MAX;
-    const int64_t c = (int64_t)dist[u] + wc[slot];
+    const int64_t c = (int64_t)dist[ri[slot]] + wc[slot];
     return c < kInf ? (int32_t)c : INT_MAX;
   }
   __device__ __forceinline__ void combine(int32_t *acc, int32_t val) {
@@ -110,7 +121,9 @@ struct PullOp {
   }
   __device__ __forceinline__ void item_done(int32_t v, int32_t acc, bool whole) {
     if (acc == INT_MAX) return;
-    const int32_t cur = __atomic_load_n(&dist[v], __ATOMIC_RELAXED);
+    // Exact when `whole` (no other workgroup writes v during the pass);
+    // otherwise possibly stale, and the atomicMin below decides.
+    const int32_t cur = dist[v];
     if (acc >= cur) return;
     int32_t old = cur;
     if (whole) {
@@ -131,7 +144,7 @@ __global__ void k_sssp_init(int32_t *dist, uint8_t *dirty, int32_t src) {
 }
 
 // A workgroup takes 1024 consecutive vertices per step, four coalesced passes
-// of 256; each wave owns the two bitmap words of its 64 vertices per pass.
+// of 256.
 // One global atomic per step for the queue, one per workgroup per counter.
 constexpr int kCollectBlock = 256;
 constexpr int kCollectStep = 4 * kCollectBlock;
@@ -139,11 +152,10 @@ constexpr int kCollectStep = 4 * kCollectBlock;
 __global__ void __launch_bounds__(kCollectBlock)
 k_sssp_collect(uint8_t *__restrict__ dirty, const int32_t *__restrict__ dist, const int32_t *__restrict__ ro,
                int32_t n, int32_t bound, int32_t *__restrict__ queue, int32_t *__restrict__ qsize,
-               uint32_t *__restrict__ front, unsigned long long *__restrict__ counters) {
+               unsigned long long *__restrict__ counters) {
   __shared__ int32_t s_scan[kCollectBlock / kWave + 1];
   __shared__ int32_t s_base;
   __shared__ int32_t s_min[kCollectBlock / kWave];
-  const int32_t words = (n + 31) >> 5;
   const int lane = lane_id(), wid = threadIdx.x / kWave;
   unsigned long long remain = 0, degsum = 0;
   int32_t dmin = INT_MAX;
@@ -163,11 +175,6 @@ k_sssp_collect(uint8_t *__restrict__ dirty, const int32_t *__restrict__ dist, co
           remain++;
           dmin = min(dmin, d);
         }
-      }
-      const uint64_t mask = ballot(sel);
-      if (lane < 2) {
-        const int64_t ww = (base + k * kCollectBlock + wid * kWave) / 32 + lane;
-        if (ww < words) front[ww] = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
       }
       selmask |= sel ? (1u << k) : 0u;
     }
@@ -202,11 +209,9 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   GG_TRY(activate(g));
   const int32_t n = g->n;
   const int64_t m = g->m;
-  const int32_t words = (n + 31) / 32;
   hipStream_t s = g->stream;
   LbsBuffers lb;
   GG_TRY(lbs_buffers(g, &lb));
-  GG_TRY(g->bits[3].ensure(4 * (size_t)words));
   GG_TRY(g->vtx[3].ensure((size_t)n));
   const int32_t *pparts = nullptr;
   const void *wcv = nullptr;
@@ -226,7 +231,6 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     GG_TRY(g->vtx[2].ensure(4 * (size_t)n));
     dist = g->vtx[2].as<int32_t>();
   }
-  uint32_t *front = g->bits[3].as<uint32_t>();
   uint8_t *dirty = g->vtx[3].as<uint8_t>();
   int32_t *queue = g->frontier[0].as<int32_t>();
   unsigned long long *ctr = g->counters.as<unsigned long long>();
@@ -265,7 +269,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     {
       TimedRegion t(g->prof, "sssp.collect");
       hipLaunchKernelGGL(k_sssp_collect, dim3(collect_grid), dim3(256), 0, s, dirty, dist, g->dg.row_offsets, n,
-                         (int32_t)std::min<int64_t>(bound, kInf), queue, qsize, front, ctr);
+                         (int32_t)std::min<int64_t>(bound, kInf), queue, qsize, ctr);
       GG_HIP(hipGetLastError());
     }
     GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 20 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
@@ -273,7 +277,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     const int64_t nf = reinterpret_cast<int32_t *>(g->h_pinned + 16)[0];
     const int64_t remain = g->h64()[16], dmin = g->h64()[17], degsum = g->h64()[18];
     const int64_t improved_so_far = g->h64()[4];
-    const double cb = n + n / 8.0 + 4.0 * (nf + remain) + 4.0 * nf;
+    const double cb = n + 4.0 * (nf + remain) + 4.0 * nf;
     g->prof.add_launch_bytes("sssp.collect", cb);
     bytes += cb;
     if (nf == 0) {
@@ -285,14 +289,14 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     (void)N;
     prev_improved = improved_so_far;
     if (pull_alpha > 0 && degsum > m / pull_alpha) {
-      PullOp op{g->dg.row_offsets, g->dg.row_indices, static_cast<const int32_t *>(wcv), front, dist, dirty, ctr, {}};
+      PullOp op{g->dg.row_offsets, g->dg.row_indices, static_cast<const int32_t *>(wcv), dist, dirty, ctr, {}};
       {
         TimedRegion t(g->prof, "sssp.pull");
         hipLaunchKernelGGL((k_pull_reduce<PullOp, int32_t>), dim3(2048), dim3(kExpandBlock), 0, s, op,
                            g->dg.column_offsets, n, m, pparts, (const int32_t *)nullptr);
         GG_HIP(hipGetLastError());
       }
-      const double pb = 4.0 * n + 4.0 * m + n / 8.0 + 8.0 * degsum;
+      const double pb = 4.0 * n + 12.0 * m;
       g->prof.add_launch_bytes("sssp.pull", pb);
       bytes += pb;
       edges += m;
