@@ -43,7 +43,6 @@
 
 namespace gg {
 
-int ensure_bins(gg_graph *g, int dir);
 int ensure_transition(gg_graph *g);
 
 namespace {
@@ -575,7 +574,6 @@ static int batch_pass(gg_graph *g, const int32_t *h_src, int32_t ns, int32_t *di
   }
   if (np) {
     GG_TRY(g->batch[4].ensure(2 * 4 * (size_t)n * kPprMax));
-    GG_TRY(ensure_bins(g, 0));
     GG_TRY(ensure_transition(g));
   }
   GG_TRY(g->batch[5].ensure(sizeof(BatchState)));

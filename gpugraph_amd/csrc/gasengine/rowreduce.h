@@ -1,17 +1,17 @@
-// Row reductions over degree-binned vertex lists (PageRank forward over CSC
-// in-edges, its adjoint over CSR out-edges).  Each vertex's row is reduced by
-// one thread (small rows), one wave (mid) or one workgroup (large), always in
-// the same order, with Neumaier-compensated FP32 accumulation, so results
-// are reproducible run to run and accurate to about one ulp of the sum.
-// The reference materialises (vertex, value) pairs and reduces them with
-// ReduceByKey (enactor_vertex_centric.cuh:2104-2424); this never writes the
-// per-edge values.
+// Compensated FP32 row-sum building blocks (KSum, strided_sum, wave_merge)
+// shared by the PageRank sweeps (segreduce.h, xcd_slices.h) and the batched
+// PPR kernels, and the degree bins (small / mid / large rows) used by the
+// per-row PageRank set-up and gradient kernels.  Sums run in a fixed order
+// with Neumaier-compensated FP32 accumulation, so results are reproducible
+// run to run and accurate to about one ulp of the sum.  The reference
+// materialises (vertex, value) pairs and reduces them with ReduceByKey
+// (enactor_vertex_centric.cuh:2104-2424); these never write per-edge values.
 //
 // RowOp concept (device, by value):
 //   void    begin()                       per-thread setup at kernel start
 //   int32_t row_begin(int32_t v), row_end(int32_t v)
 //   float   edge(int32_t v, int32_t k)    contribution of slot k of v's row
-//   void    item_done(int32_t v, float)   one thread per vertex
+//   void    item_done(int32_t v, float)   once per vertex
 //   void    finish()                      all threads, once (block flushes)
 #pragma once
 
@@ -64,62 +64,6 @@ __device__ __forceinline__ void wave_merge(KSum &a) {
   }
 }
 
-template <class Op>
-__global__ void __launch_bounds__(kRowBlock)
-k_rows_small(Op op, const int32_t *__restrict__ list, int32_t count, const int32_t *__restrict__ done) {
-  if (done && *done) return;
-  op.begin();
-  for (int64_t i = (int64_t)blockIdx.x * kRowBlock + threadIdx.x; i < count; i += (int64_t)gridDim.x * kRowBlock) {
-    const int32_t v = list[i];
-    KSum acc;
-    strided_sum(op, v, op.row_begin(v), op.row_end(v), 1, acc);
-    op.item_done(v, acc.value());
-  }
-  op.finish();
-}
-
-template <class Op>
-__global__ void __launch_bounds__(kRowBlock)
-k_rows_wave(Op op, const int32_t *__restrict__ list, int32_t count, const int32_t *__restrict__ done) {
-  if (done && *done) return;
-  op.begin();
-  const int64_t waves = (int64_t)gridDim.x * (kRowBlock / kWave);
-  for (int64_t i = (int64_t)blockIdx.x * (kRowBlock / kWave) + threadIdx.x / kWave; i < count; i += waves) {
-    const int32_t v = list[i];
-    KSum acc;
-    strided_sum(op, v, op.row_begin(v) + lane_id(), op.row_end(v), kWave, acc);
-    wave_merge(acc);
-    if (lane_id() == 0) op.item_done(v, acc.value());
-  }
-  op.finish();
-}
-
-template <class Op>
-__global__ void __launch_bounds__(kRowBlock)
-k_rows_block(Op op, const int32_t *__restrict__ list, int32_t count, const int32_t *__restrict__ done) {
-  __shared__ float s_s[kRowBlock / kWave], s_c[kRowBlock / kWave];
-  if (done && *done) return;
-  op.begin();
-  for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
-    const int32_t v = list[i];
-    KSum acc;
-    strided_sum(op, v, op.row_begin(v) + (int32_t)threadIdx.x, op.row_end(v), kRowBlock, acc);
-    wave_merge(acc);
-    if (lane_id() == 0) {
-      s_s[threadIdx.x / kWave] = acc.s;
-      s_c[threadIdx.x / kWave] = acc.c;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      KSum t;
-      for (int w = 0; w < kRowBlock / kWave; w++) t.merge(s_s[w], s_c[w]);
-      op.item_done(v, t.value());
-    }
-    __syncthreads();
-  }
-  op.finish();
-}
-
 // Classify vertices by row length into three lists (small / mid / large).
 template <int D = 0>
 __global__ void __launch_bounds__(kRowBlock)
@@ -133,19 +77,6 @@ k_build_bins(const int32_t *__restrict__ off, int32_t n, int32_t *__restrict__ s
     block_append<kRowBlock>(d > kRowSmallMax && d <= kRowWaveMax, (int32_t)v, mid, &counts[1]);
     block_append<kRowBlock>(d > kRowWaveMax, (int32_t)v, large, &counts[2]);
   }
-}
-
-// Launch the three binned kernels for one row reduction.
-template <class Op>
-inline void rows_launch(const Op &op, int32_t *const lists[3], const int32_t counts[3], const int32_t *done,
-                        hipStream_t s) {
-  if (counts[0]) hipLaunchKernelGGL(k_rows_small<Op>, dim3(grid_for(counts[0], kRowBlock, 4096)), dim3(kRowBlock), 0, s,
-                                    op, lists[0], counts[0], done);
-  if (counts[1])
-    hipLaunchKernelGGL(k_rows_wave<Op>, dim3(grid_for((int64_t)counts[1] * kWave, kRowBlock, 4096)), dim3(kRowBlock), 0,
-                       s, op, lists[1], counts[1], done);
-  if (counts[2]) hipLaunchKernelGGL(k_rows_block<Op>, dim3(counts[2] < 4096 ? counts[2] : 4096), dim3(kRowBlock), 0, s,
-                                    op, lists[2], counts[2], done);
 }
 
 }  // namespace GASengine

@@ -95,6 +95,19 @@ int pull_partition(gg_graph *g, const int32_t **parts) {
   return GG_OK;
 }
 
+int csr_partition(gg_graph *g, const int32_t **parts) {
+  if (!g->csr_parts_ready) {
+    GG_TRY(g->csr_parts.ensure(4 * GASengine::lbs_partition_entries(g->n, g->m)));
+    const int64_t tiles = ((int64_t)g->n + g->m) / GASengine::kLbsTile + 2;
+    hipLaunchKernelGGL(GASengine::k_static_partition<0>, dim3(GASengine::grid_for(tiles, 256, 4096)), dim3(256), 0,
+                       g->stream, g->dg.row_offsets, g->n, g->m, g->csr_parts.as<int32_t>());
+    GG_HIP(hipGetLastError());
+    g->csr_parts_ready = true;
+  }
+  *parts = g->csr_parts.as<int32_t>();
+  return GG_OK;
+}
+
 // Mirror check of an undirected CSR, run over every (v, slot) with the pull
 // merge path: for slot k of v's row (neighbour u, its j-th occurrence there)
 // the mirror is the j-th occurrence of v in u's row.  flags: 1 = a row is

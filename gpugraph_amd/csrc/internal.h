@@ -179,6 +179,10 @@ struct gg_graph {
   // (directed graphs; undirected graphs read the CSR values directly).
   gg::DevBuf pull_parts, w_csc;
   bool pull_parts_ready = false;
+  // The same partition over (vertices, CSR out-edges), and the per-tile
+  // carries of the segmented row sums.
+  gg::DevBuf csr_parts, seg_carry;
+  bool csr_parts_ready = false;
   int64_t values_version = 0, w_csc_version = -1;
   // Undirected graphs: rows sorted and symmetry confirmed (mirror_ok); the
   // mirrored entries' values differ (values_asymmetric; mirror permutation
@@ -210,6 +214,8 @@ void finish_profile(gg_graph *g);
 int lbs_buffers(gg_graph *g, GASengine::LbsBuffers *b);
 // Static merge-path partition of (vertices, CSC in-edges) for k_pull_reduce.
 int pull_partition(gg_graph *g, const int32_t **parts);
+// The same over (vertices, CSR out-edges).
+int csr_partition(gg_graph *g, const int32_t **parts);
 // Edge values in CSC order (4-byte values): the CSR array itself for
 // undirected graphs, a cached permuted copy for directed ones.
 int csc_values(gg_graph *g, const void **vals);

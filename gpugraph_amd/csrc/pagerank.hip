@@ -3,9 +3,9 @@
 //
 // Forward:  r = d (P^T r + D/n 1) + (1-d) t,  D = sum of r over dangling
 //           vertices, P_uv = w_uv / S_u (F32 values) or 1/outdeg(u).
-//           One sweep = row reductions over the CSC in-edge lists
-//           (GASengine::rows_launch, degree-binned, Neumaier-compensated
-//           FP32), writing r_next and, unweighted, the contribution
+//           One sweep = row sums over the CSC in-edge lists
+//           (GASengine::seg_rows_launch: merge-path tiles, Neumaier-
+//           compensated FP32, fixed fold order), writing r_next and, unweighted, the contribution
 //           x = r_next / outdeg used by the next sweep; the same pass adds
 //           |r_next - r| and the dangling mass of r_next in FP64 (one atomic
 //           per workgroup).  A one-thread finalize kernel closes the
@@ -24,11 +24,12 @@
 // dangling term or teleport vector; that form runs through the generic GAS
 // path (gg_pagerank_reference).  The adjoint has no reference counterpart.
 //
-// Byte model per sweep: unweighted 24n + 8m (+4n with a teleport vector):
-// per vertex offsets, list entry, old/new rank, new contribution, 1/outdeg;
-// per edge the source id and its contribution.  Weighted 20n + 12m (+4n):
-// per edge source id, P_uv and r_u.  Adjoint sweep the same; gradient pass
-// 16m + 16n.
+// Byte model per sweep: unweighted 20n + 8m (+4n with a teleport vector):
+// per vertex offsets, old/new rank, new contribution, 1/outdeg; per edge
+// the source id and its contribution.  Weighted 16n + 12m (+4n): per edge
+// source id, P_uv and r_u.  Adjoint sweep 24n + 8m (weighted + 4m): per
+// vertex offsets, old/new lambda, c, row sum, 1/outdeg; per edge target id,
+// its lambda (and P_uv).  Gradient pass 16m + 16n.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -37,6 +38,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "gasengine/rowreduce.h"
+#include "gasengine/segreduce.h"
 #include "gasengine/xcd_slices.h"
 #include "internal.h"
 
@@ -80,12 +82,20 @@ struct PrForwardOp {
   float d, uniform_tele;
   double inv_n;
   PrState *st;
-  int par;
+  // This sweep's vectors, picked in begin() by the iteration's parity with
+  // constant indices (a run-time index into r / x would keep the whole op in
+  // scratch memory).
+  const float *rc, *xc;
+  float *rn, *xn;
   float dn;
   double resid, dang;
 
   __device__ __forceinline__ void begin() {
-    par = st->iters & 1;
+    const bool odd = st->iters & 1;
+    rc = odd ? r[1] : r[0];
+    rn = odd ? r[0] : r[1];
+    xc = odd ? x[1] : x[0];
+    xn = odd ? x[0] : x[1];
     dn = (float)(st->aux_cur * inv_n);
     resid = 0.0;
     dang = 0.0;
@@ -94,16 +104,16 @@ struct PrForwardOp {
   __device__ __forceinline__ int32_t row_end(int32_t v) { return off[v + 1]; }
   __device__ __forceinline__ float edge(int32_t, int32_t k) {
     const int32_t u = src[k];
-    return pcsc ? r[par][u] * pcsc[k] : x[par][u];
+    return pcsc ? rc[u] * pcsc[k] : xc[u];
   }
   __device__ __forceinline__ void item_done(int32_t v, float sum) {
     const float tv = tele ? (1.0f - d) * tele[v] : uniform_tele;
-    const float rn = d * (sum + dn) + tv;
-    r[par ^ 1][v] = rn;
+    const float rv = d * (sum + dn) + tv;
+    rn[v] = rv;
     const float io = inv_out[v];
-    if (!pcsc) x[par ^ 1][v] = rn * io;
-    resid += fabs((double)rn - (double)r[par][v]);
-    if (io == 0.0f) dang += rn;
+    if (!pcsc) xn[v] = rv * io;
+    resid += fabs((double)rv - (double)rc[v]);
+    if (io == 0.0f) dang += rv;
   }
   __device__ __forceinline__ void finish() {
     block_add_f64<kRowBlock>(&st->resid_acc, resid);
@@ -122,12 +132,15 @@ struct PrAdjointOp {
   float d;
   double inv_n;
   PrState *st;
-  int par;
+  const float *lc;  // this sweep's lambda and the next one (picked in begin())
+  float *lnx;
   float ln;
   double resid, lsum;
 
   __device__ __forceinline__ void begin() {
-    par = st->iters & 1;
+    const bool odd = st->iters & 1;
+    lc = odd ? lam[1] : lam[0];
+    lnx = odd ? lam[0] : lam[1];
     ln = (float)(st->aux_cur * inv_n);
     resid = 0.0;
     lsum = 0.0;
@@ -135,16 +148,16 @@ struct PrAdjointOp {
   __device__ __forceinline__ int32_t row_begin(int32_t u) { return off[u]; }
   __device__ __forceinline__ int32_t row_end(int32_t u) { return off[u + 1]; }
   __device__ __forceinline__ float edge(int32_t, int32_t e) {
-    const float l = lam[par][dst[e]];
+    const float l = lc[dst[e]];
     return pcsr ? pcsr[e] * l : l;
   }
   __device__ __forceinline__ void item_done(int32_t u, float sum) {
     const float io = inv_out[u];
     const float su = pcsr ? sum : sum * io;
     const float ln_u = c[u] + d * (su + (io == 0.0f ? ln : 0.0f));
-    lam[par ^ 1][u] = ln_u;
+    lnx[u] = ln_u;
     rowsum[u] = su;
-    resid += fabs((double)ln_u - (double)lam[par][u]);
+    resid += fabs((double)ln_u - (double)lc[u]);
     lsum += ln_u;
   }
   __device__ __forceinline__ void finish() {
@@ -202,15 +215,47 @@ __global__ void k_inv_outdeg(const int32_t *__restrict__ ro, float *__restrict__
   }
 }
 
-// S_u = sum of u's out-edge weights (FP64), P_uv = w_uv / S_u rounded to FP32
-// for every CSR slot.
-__global__ void k_row_normalise(const int32_t *__restrict__ ro, const float *__restrict__ w, int32_t n,
-                                double *__restrict__ S, float *__restrict__ pcsr) {
-  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int32_t e = ro[u]; e < ro[u + 1]; e++) s += (double)w[e];
-    S[u] = s;
-    for (int32_t e = ro[u]; e < ro[u + 1]; e++) pcsr[e] = (float)((double)w[e] / s);
+// S_u = sum of u's out-edge weights (FP64, in a fixed order), then
+// P_uv = w_uv / S_u rounded to FP32 for every CSR slot of the row.  Rows come
+// from the CSR degree bins: MODE 0 one thread per row (small bin), 1 one
+// wave (mid), 2 one workgroup (large), so a hub row is not one thread's job.
+template <int MODE>
+__global__ void __launch_bounds__(kRowBlock)
+k_row_normalise(const int32_t *__restrict__ ro, const float *__restrict__ w, const int32_t *__restrict__ list,
+                int32_t count, double *__restrict__ S, float *__restrict__ pcsr) {
+  __shared__ double red[kRowBlock / kWave];
+  constexpr int step = MODE == 0 ? 1 : MODE == 1 ? kWave : kRowBlock;
+  const int me = MODE == 0 ? 0 : MODE == 1 ? lane_id() : (int)threadIdx.x;
+  int64_t i, stride;
+  if (MODE == 0) {
+    i = (int64_t)blockIdx.x * kRowBlock + threadIdx.x;
+    stride = (int64_t)gridDim.x * kRowBlock;
+  } else if (MODE == 1) {
+    i = (int64_t)blockIdx.x * (kRowBlock / kWave) + threadIdx.x / kWave;
+    stride = (int64_t)gridDim.x * (kRowBlock / kWave);
+  } else {
+    i = blockIdx.x;
+    stride = gridDim.x;
+  }
+  for (; i < count; i += stride) {
+    const int32_t u = list[i];
+    const int32_t b = ro[u], e = ro[u + 1];
+    double acc = 0.0;
+    for (int32_t k = b + me; k < e; k += step) acc += (double)w[k];
+    if (MODE >= 1) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+      acc = __shfl(acc, 0);  // every lane uses lane 0's rounding
+    }
+    if (MODE == 2) {
+      if (lane_id() == 0) red[threadIdx.x / kWave] = acc;
+      __syncthreads();
+      acc = 0.0;
+      for (int q = 0; q < kRowBlock / kWave; q++) acc += red[q];
+      __syncthreads();
+    }
+    if (me == 0) S[u] = acc;
+    for (int32_t k = b + me; k < e; k += step) pcsr[k] = (float)((double)w[k] / acc);
   }
 }
 
@@ -308,8 +353,20 @@ int ensure_transition(gg_graph *g) {
     GG_TRY(g->pr[1].ensure(8 * (size_t)n));
     GG_TRY(g->pr[2].ensure(4 * (size_t)std::max<int64_t>(m, 1)));
     GG_TRY(g->pr[3].ensure(4 * (size_t)std::max<int64_t>(m, 1)));
-    hipLaunchKernelGGL(k_row_normalise, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, g->dg.row_offsets,
-                       g->val.as<float>(), n, g->pr[1].as<double>(), g->pr[2].as<float>());
+    GG_TRY(ensure_bins(g, 1));
+    const float *w = g->val.as<float>();
+    double *S = g->pr[1].as<double>();
+    float *pcsr = g->pr[2].as<float>();
+    const int32_t *ro = g->dg.row_offsets;
+    if (const int32_t c0 = g->bin_count[1][0])
+      hipLaunchKernelGGL(k_row_normalise<0>, dim3(grid_for(c0, kRowBlock, 8192)), dim3(kRowBlock), 0, s, ro, w,
+                         g->bins[1][0].as<int32_t>(), c0, S, pcsr);
+    if (const int32_t c1 = g->bin_count[1][1])
+      hipLaunchKernelGGL(k_row_normalise<1>, dim3(grid_for((int64_t)c1 * kWave, kRowBlock, 8192)), dim3(kRowBlock), 0,
+                         s, ro, w, g->bins[1][1].as<int32_t>(), c1, S, pcsr);
+    if (const int32_t c2 = g->bin_count[1][2])
+      hipLaunchKernelGGL(k_row_normalise<2>, dim3(c2 < 4096 ? c2 : 4096), dim3(kRowBlock), 0, s, ro, w,
+                         g->bins[1][2].as<int32_t>(), c2, S, pcsr);
     GG_HIP(hipGetLastError());
     const void *wc = nullptr;
     GG_TRY(csc_values(g, &wc));
@@ -325,7 +382,8 @@ int ensure_transition(gg_graph *g) {
 // Whether PageRank sweeps on this graph use the XCD-sliced rows.  Auto (-1)
 // means off: on R-MAT scale 22 (average in-degree 16) the eight sub-rows per
 // vertex cost more in offsets and partials than the L2-resident gathers save
-// (36.6 ms vs 30.0 ms per solve); gg_autotune tries both.
+// (19.3 ms vs 12.9 ms per forward solve against the merge-path sweep);
+// gg_autotune tries both.
 bool use_slices(const gg_graph *g) { return g->opt.xcd_slices == 1; }
 
 // Slice structure of direction dir (0: CSC in-edges, 1: CSR out-edges),
@@ -427,15 +485,22 @@ int stage_input(gg_graph *g, const float *p, uint32_t flags, gg::DevBuf &buf, co
 }
 
 // Run sweeps (forward or adjoint op) to convergence; returns iterations.  The
-// sweep is either the degree-binned row reduction or, with slices, the
-// XCD-sliced partials + combine.
+// sweep is the merge-path segmented row sum over CSC (dir 0) or CSR (dir 1)
+// rows or, with slices, the XCD-sliced partials + combine.
 template <class Op>
 int iterate(gg_graph *g, Op op, int dir, const SliceLists *slices, PrState *st, double tol, int32_t max_iter,
             const char *kname, double bytes_per_sweep, int32_t *iters_out, double *resid_out) {
   hipStream_t s = g->stream;
-  int32_t *lists[3] = {g->bins[dir][0].as<int32_t>(), g->bins[dir][1].as<int32_t>(), g->bins[dir][2].as<int32_t>()};
   const int32_t *done = &st->done;
   float2 *part = slices ? g->sl_part.as<float2>() : nullptr;
+  const int32_t *offs = dir == 0 ? g->dg.column_offsets : g->dg.row_offsets;
+  const int32_t *parts = nullptr;
+  SegCarry *carry = nullptr;
+  if (!slices) {
+    GG_TRY(dir == 0 ? pull_partition(g, &parts) : csr_partition(g, &parts));
+    GG_TRY(g->seg_carry.ensure(sizeof(SegCarry) * (size_t)seg_tiles(g->n, g->m)));
+    carry = g->seg_carry.as<SegCarry>();
+  }
   struct H {
     double resid_last;
     int32_t iters, done;
@@ -447,7 +512,7 @@ int iterate(gg_graph *g, Op op, int dir, const SliceLists *slices, PrState *st, 
         if (slices)
           slice_rows_launch(op, *slices, part, done, s);
         else
-          rows_launch(op, lists, g->bin_count[dir], done, s);
+          seg_rows_launch(op, offs, g->n, g->m, parts, carry, done, g->opt.expand_grid, s);
       }
       hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(1), 0, s, st, tol, max_iter);
     }
@@ -483,7 +548,6 @@ int gg_pagerank(gg_graph *g, float damping, const float *teleport, double tol_l1
   hipStream_t s = g->stream;
   const bool weighted = g->value_dtype == GG_VALUES_F32;
   g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
-  GG_TRY(ensure_bins(g, 0));
   GG_TRY(ensure_transition(g));
   GG_TRY(g->pr[4].ensure(8 * (size_t)n));  // r[2]
   GG_TRY(g->pr[5].ensure(8 * (size_t)n));  // x[2]
@@ -520,7 +584,7 @@ int gg_pagerank(gg_graph *g, float damping, const float *teleport, double tol_l1
   op.uniform_tele = (float)((1.0 - damping) / (double)n);
   op.inv_n = 1.0 / (double)n;
   op.st = st;
-  const double per_sweep = (weighted ? 20.0 * n + 12.0 * m : 24.0 * n + 8.0 * m) + (tele ? 4.0 * n : 0.0);
+  const double per_sweep = (weighted ? 16.0 * n + 12.0 * m : 20.0 * n + 8.0 * m) + (tele ? 4.0 * n : 0.0);
   SliceLists L{};
   const bool sliced = use_slices(g);
   if (sliced) {
@@ -612,7 +676,7 @@ int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank, const flo
   op.d = damping;
   op.inv_n = 1.0 / (double)n;
   op.st = st;
-  const double per_sweep = weighted ? 20.0 * n + 12.0 * m : 24.0 * n + 8.0 * m;
+  const double per_sweep = weighted ? 24.0 * n + 12.0 * m : 24.0 * n + 8.0 * m;
   SliceLists L{};
   const bool sliced = use_slices(g);
   if (sliced) {
