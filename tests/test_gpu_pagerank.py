@@ -224,3 +224,70 @@ def test_xcd_sliced_adjoint_and_update(gpu_available):
     exp2, _, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=normalised_p32(gg.CSR(c.row_offsets, c.col_indices, w2)),
                                      tol=1e-14)
     assert l1(r2, exp2) <= L1_TOL
+
+
+def csr_from_edges(n, src, dst):
+    src, dst = np.asarray(src, np.int64), np.asarray(dst, np.int64)
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    ro = np.zeros(n + 1, np.int64)
+    np.add.at(ro, src + 1, 1)
+    return gg.CSR(np.cumsum(ro).astype(np.int32), dst.astype(np.int32))
+
+
+def tile_shape_graphs():
+    """Directed graphs whose in-rows (forward sweep) and out-rows (adjoint)
+    cross the 1024-position merge-path tiles in every way: rows of 1023 / 1024
+    / 1025 / 5000 edges spanning one to six tiles, mostly empty rows, and
+    rows of one edge."""
+    rng = np.random.default_rng(48)
+    out = []
+    for big in (1023, 1024, 1025, 5000):
+        n = 3000
+        src, dst = [], []
+        for hub in (0, 7, 1500):  # three hubs: long in-rows and long out-rows
+            s = rng.choice(n, big if big < n else n - 1, replace=big > n - 1)
+            src += list(s)
+            dst += [hub] * len(s)
+            src += [hub] * len(s)
+            dst += list(rng.integers(0, n, len(s)))
+        ring = np.arange(0, n, 3)  # a third of the vertices on a ring, the rest with empty rows
+        src += list(ring)
+        dst += list(np.roll(ring, -1))
+        out.append(csr_from_edges(n, src, dst))
+    n = 20000  # 90 % empty in-rows, single edges elsewhere
+    s = rng.integers(0, n, 2000)
+    out.append(csr_from_edges(n, s, rng.choice(n, 2000, replace=False)))
+    return out
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_segmented_sweep_tile_shapes(gpu_available, weighted):
+    """Merge-path segmented sweeps: FP64 oracle agreement (L1 <= 1e-6) and the
+    same bits for any grid size and run (the fold order is fixed by the
+    tiles), forward and adjoint."""
+    rng = np.random.default_rng(49)
+    for c in tile_shape_graphs():
+        if weighted:
+            c = c.with_weights("f32", symmetric=False, seed=50)
+        p = normalised_p32(c) if weighted else None
+        exp, _, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=p, tol=1e-14)
+        cv = rng.standard_normal(c.n)
+        w64 = c.values.astype(np.float64) if weighted else None
+        _, gw64, gt64, _ = oracle.pagerank_adjoint_f64(c.row_offsets, c.col_indices, w64, exp, cv, tol=1e-15)
+        runs = []
+        with gg.Graph(c, directed=True) as g:
+            for grid in (64, 2048, 65536):
+                g.set_options(expand_grid=grid)
+                r, st = g.pagerank(0.85, tol_l1=2e-7)
+                gw, gt, _ = g.pagerank_adjoint(r, cv.astype(np.float32), tol_l1=2e-7)
+                runs.append((r, gw, gt))
+        r, gw, gt = runs[0]
+        assert l1(r, exp) <= L1_TOL
+        assert st["residual"] <= 2e-7
+        assert np.linalg.norm(gt - gt64) / np.linalg.norm(gt64) < 1e-4
+        assert np.linalg.norm(gw - gw64) / np.linalg.norm(gw64) < 1e-4
+        for r2, gw2, gt2 in runs[1:]:
+            np.testing.assert_array_equal(r2, r)
+            np.testing.assert_array_equal(gw2, gw)
+            np.testing.assert_array_equal(gt2, gt)
