@@ -160,26 +160,34 @@ k_lp_expand(CcState *st, const int32_t *__restrict__ ro, const int32_t *__restri
   if (threadIdx.x == 0 && t) atomicAdd(&st->edges, t);
 }
 
-// Pull operator for GASengine::k_pull_reduce over the (symmetric) adjacency.
+// Pull operator for GASengine::k_pull_staged over the (symmetric) adjacency.
+// Runs on GASengine::k_pull_staged (neighbour ids load with the tile, their
+// labels after; label[v] is prefetched).
 struct CcPullOp {
-  static constexpr bool kPureEdge = true;
   const int32_t *nbr;
   int32_t *label;
   CcState *st;                    // output bitmap st->b_nxt (cleared before the pull), count st->pull_changed
   unsigned long long count;
 
+  struct Edge {
+    int32_t u;
+  };
+  __device__ __forceinline__ Edge load_edge(int32_t slot) { return {nbr[slot]}; }
+  __device__ __forceinline__ int32_t gather(const Edge &e) { return label[e.u]; }
   __device__ __forceinline__ int32_t identity() { return INT_MAX; }
-  __device__ __forceinline__ int32_t edge(int32_t, int32_t slot) { return label[nbr[slot]]; }
   __device__ __forceinline__ void combine(int32_t *acc, int32_t val) {
     if (val < *acc) atomicMin(acc, val);
   }
-  // k_pull_reduce calls this with consecutive vertices on consecutive lanes
+  __device__ __forceinline__ int32_t vertex_state(int32_t v) { return label[v]; }
+  // k_pull_staged calls this with consecutive vertices on consecutive lanes
   // (v = first + lane for the active lanes), so the changed marks of a wave
   // are OR-ed into each bitmap word with one atomic; only bits the atomic
   // newly set are counted (a row spanning two tiles can be marked twice).
-  __device__ __forceinline__ void item_done(int32_t v, int32_t acc, bool whole) {
+  // cur = label[v] at the start of the tile: exact when `whole`, otherwise
+  // possibly stale (only ever larger), and the atomicMin decides.
+  __device__ __forceinline__ void item_done(int32_t v, int32_t acc, bool whole, int32_t cur) {
     bool changed = false;
-    if (acc < label[v]) {  // a stale read is only larger
+    if (acc < cur) {
       if (whole)
         label[v] = acc, changed = true;  // all of v's edges are in this tile: the only writer
       else
@@ -314,7 +322,7 @@ int cc_label_propagation(gg_graph *g, int32_t *label_out, uint32_t flags, gg_sta
                        (int32_t)words);
     hipLaunchKernelGGL(k_lp_expand, dim3(grid), dim3(kBlock), 0, s, st, g->dg.row_offsets, g->dg.column_indices,
                        label, g->opt.wave_degree);
-    hipLaunchKernelGGL((k_pull_reduce<CcPullOp, int32_t>), dim3(grid), dim3(kExpandBlock), 0, s, pull_op,
+    hipLaunchKernelGGL((k_pull_staged<CcPullOp, int32_t>), dim3(grid), dim3(kExpandBlock), 0, s, pull_op,
                        g->dg.row_offsets, n, g->m, parts, &st->pull_skip);
     hipLaunchKernelGGL(k_lp_swap, dim3(1), dim3(1), 0, s, st, n, g->m, max_iter);
   }

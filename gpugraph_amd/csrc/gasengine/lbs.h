@@ -418,20 +418,6 @@ k_frontier_prologue(Op op, const int32_t *__restrict__ frontier, const int32_t *
 //   void combine(T *lds_acc, T val)               LDS reduction (e.g. atomicMin)
 //   void item_done(int32_t v, T acc, bool whole)  result of v for this tile
 //   void finish()
-//   static constexpr bool kPureEdge (optional)   edge() only loads: the kernel
-//        then evaluates a thread's VT edges back to back (tail slots clamped
-//        to the tile's last edge, their values dropped) so that each load
-//        stage of all VT edges is in flight at once, and combines afterwards.
-
-template <class Op, class = void>
-struct PureEdge {
-  static constexpr bool value = false;
-};
-template <class Op>
-struct PureEdge<Op, decltype((void)Op::kPureEdge)> {
-  static constexpr bool value = Op::kPureEdge;
-};
-
 template <int DUMMY = 0>
 __global__ void __launch_bounds__(256)
 k_static_partition(const int32_t *__restrict__ offs, int32_t n, int64_t m, int32_t *__restrict__ parts) {
@@ -501,30 +487,10 @@ k_pull_reduce(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
       }
     }
     __syncthreads();
-    if constexpr (PureEdge<Op>::value) {
-      if (ne > 0) {  // ne <= TILE = VT * kExpandBlock: one pass
-        int32_t sl[VT];
-        T val[VT];
-#pragma unroll
-        for (int r = 0; r < VT; r++) {
-          const int32_t k = min((int32_t)threadIdx.x + r * kExpandBlock, ne - 1);
-          sl[r] = s_slot[k];
-        }
-#pragma unroll
-        for (int r = 0; r < VT; r++) {
-          const int32_t k = min((int32_t)threadIdx.x + r * kExpandBlock, ne - 1);
-          val[r] = op.edge(i0 - 1 + sl[r], j0 + k);
-        }
-#pragma unroll
-        for (int r = 0; r < VT; r++)
-          if ((int32_t)threadIdx.x + r * kExpandBlock < ne) op.combine(&s_acc[sl[r]], val[r]);
-      }
-    } else {
-      for (int32_t k = threadIdx.x; k < ne; k += kExpandBlock) {
-        const int32_t s = s_slot[k];
-        const T val = op.edge(i0 - 1 + s, j0 + k);
-        op.combine(&s_acc[s], val);
-      }
+    for (int32_t k = threadIdx.x; k < ne; k += kExpandBlock) {
+      const int32_t s = s_slot[k];
+      const T val = op.edge(i0 - 1 + s, j0 + k);
+      op.combine(&s_acc[s], val);
     }
     __syncthreads();
     for (int k = threadIdx.x; k <= ni; k += kExpandBlock) {
@@ -534,6 +500,134 @@ k_pull_reduce(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
       op.item_done(v, s_acc[k], whole);
     }
     __syncthreads();
+  }
+  op.finish();
+}
+
+// ---- staged pull: two dependent memory round trips per tile -------------
+//
+// k_pull_reduce waits on the tile's offsets, then its edges' loads, then
+// their gathers, then each vertex's current state.  Here every load that
+// does not depend on a gather is issued at once at the start of a tile (the
+// tile's edges are the contiguous slots [j0, j1), known from the partition
+// alone): the row offsets, each thread's VT edges (Edge loads) and the
+// current state of the tile's vertices; the next tile's partition bounds are
+// fetched a tile ahead.  The merge-path slot map is then built in LDS, the
+// VT gathers go out together, and the results are combined and handed to
+// item_done with the prefetched state.
+//
+// StagedPullOp concept:
+//   using Edge = ...                        per-edge loads (e.g. source and weight)
+//   Edge    load_edge(int32_t slot)         loads only
+//   T       gather(const Edge &e)           dependent loads (e.g. dist[src] + w)
+//   T       identity()
+//   void    combine(T *lds_acc, T val)
+//   int32_t vertex_state(int32_t v)         prefetched per-vertex value
+//   void    item_done(int32_t v, T acc, bool whole, int32_t state)
+//           whole: every in-edge of v lies in this tile, so no other
+//           workgroup writes v during the launch and `state` is exact
+//   void    finish()
+template <class Op, class T>
+__global__ void __launch_bounds__(kExpandBlock)
+k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const int32_t *__restrict__ parts,
+              const int32_t *__restrict__ skip) {
+  if (skip && *skip) return;
+  constexpr int TILE = kLbsTile;
+  constexpr int VT = kExpandVt;
+  constexpr int Q = (TILE + 2 + kExpandBlock - 1) / kExpandBlock;  // per-thread share of the offsets
+  __shared__ int32_t s_offs[TILE + 2];  // s_offs[k] = offs[i0 - 1 + k], k = 0..ni+1
+  __shared__ T s_acc[TILE + 1];
+  __shared__ int32_t s_state[TILE + 1];
+  __shared__ int32_t s_slot[TILE];
+  const int tid = threadIdx.x;
+  const int64_t path = (int64_t)n + m;
+  int64_t t = blockIdx.x;
+  int32_t i0 = 0, i1 = 0;
+  if (t * TILE < path) {
+    i0 = parts[t];
+    i1 = parts[t + 1];
+  }
+  for (; t * TILE < path; t += gridDim.x) {
+    const int64_t d0 = t * TILE;
+    const int64_t d1 = d0 + TILE < path ? d0 + TILE : path;
+    const int32_t j0 = (int32_t)(d0 - i0), j1 = (int32_t)(d1 - i1);
+    const int32_t ni = i1 - i0, ne = j1 - j0;
+    const int64_t tn = t + gridDim.x;
+    int32_t n0 = 0, n1 = 0;
+    if (tn * TILE < path) {  // next tile's bounds, a tile ahead
+      n0 = parts[tn];
+      n1 = parts[tn + 1];
+    }
+    // Stage 1: edges, offsets and vertex state, all independent.
+    typename Op::Edge ed[VT];
+    if (ne > 0) {
+#pragma unroll
+      for (int r = 0; r < VT; r++) ed[r] = op.load_edge(j0 + min(tid + r * kExpandBlock, ne - 1));
+    }
+    int32_t ov[Q], sv[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int32_t k = min(tid + q * kExpandBlock, ni + 1);
+      const int32_t i = i0 - 1 + k;  // <= i1 <= n: offs has n + 1 entries
+      ov[q] = i < 0 ? 0 : offs[i];
+      sv[q] = op.vertex_state(i < 0 ? 0 : i < n ? i : n - 1);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int32_t k = tid + q * kExpandBlock;
+      if (k <= ni + 1) s_offs[k] = ov[q];
+      if (k <= ni) {
+        s_state[k] = sv[q];
+        s_acc[k] = op.identity();
+      }
+    }
+    __syncthreads();
+    {
+      const int32_t dl = tid * VT;
+      if (dl < ni + ne) {
+        int32_t lo = dl - ne > 0 ? dl - ne : 0, hi = dl < ni ? dl : ni;
+        while (lo < hi) {
+          const int32_t mid = (lo + hi) >> 1;
+          if (s_offs[mid + 1] <= j0 + (dl - mid) - 1)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        int32_t li = lo, lj = dl - lo;
+#pragma unroll
+        for (int k = 0; k < VT; k++) {
+          if (li + lj >= ni + ne) break;
+          if (li < ni && s_offs[li + 1] <= j0 + lj) {
+            li++;
+          } else {
+            s_slot[lj] = li;
+            lj++;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // Stage 2: the gathers, VT in flight per thread.
+    if (ne > 0) {
+      T val[VT];
+#pragma unroll
+      for (int r = 0; r < VT; r++) val[r] = op.gather(ed[r]);
+#pragma unroll
+      for (int r = 0; r < VT; r++) {
+        const int32_t k = tid + r * kExpandBlock;
+        if (k < ne) op.combine(&s_acc[s_slot[k]], val[r]);
+      }
+    }
+    __syncthreads();
+    for (int k = tid; k <= ni; k += kExpandBlock) {
+      const int32_t v = i0 - 1 + k;
+      if (v < 0) continue;
+      const bool whole = k >= 1 && s_offs[k] >= j0 && s_offs[k + 1] <= j1;
+      op.item_done(v, s_acc[k], whole, s_state[k]);
+    }
+    __syncthreads();
+    i0 = n0;
+    i1 = n1;
   }
   op.finish();
 }

@@ -50,6 +50,7 @@ struct Tally {
       mf += (unsigned long long)(ro[v + 1] - ro[v]);
     }
   }
+
   // All threads of the workgroup must call it.
   __device__ __forceinline__ void flush(unsigned long long *counters) {
     block_counter_add<kExpandBlock>(&counters[4], improved);
@@ -101,8 +102,9 @@ struct PushOp {
 // all in-edges anyway, dist[u] + w is a path length whatever u's state, and
 // distances lowered earlier in the same pass propagate at once.  dist[u] is a
 // plain load (it may be stale, i.e. larger, and then only relaxes less).
+// Runs on GASengine::k_pull_staged: an edge's source and weight load with
+// the tile, the gather of dist[u] after, dist[v] is prefetched.
 struct PullOp {
-  static constexpr bool kPureEdge = true;
   const int32_t *ro;
   const int32_t *ri;
   const int32_t *wc;
@@ -111,20 +113,23 @@ struct PullOp {
   unsigned long long *counters;
   Tally tally;
 
-  __device__ __forceinline__ int32_t identity() { return INT_MAX; }
-  __device__ __forceinline__ int32_t edge(int32_t, int32_t slot) {
-    const int64_t c = (int64_t)dist[ri[slot]] + wc[slot];
+  struct Edge {
+    int32_t u, w;
+  };
+  __device__ __forceinline__ Edge load_edge(int32_t slot) { return {ri[slot], wc[slot]}; }
+  __device__ __forceinline__ int32_t gather(const Edge &e) {
+    const int64_t c = (int64_t)dist[e.u] + e.w;
     return c < kInf ? (int32_t)c : INT_MAX;
   }
+  __device__ __forceinline__ int32_t identity() { return INT_MAX; }
   __device__ __forceinline__ void combine(int32_t *acc, int32_t val) {
     if (val < *acc) atomicMin(acc, val);
   }
-  __device__ __forceinline__ void item_done(int32_t v, int32_t acc, bool whole) {
-    if (acc == INT_MAX) return;
-    // Exact when `whole` (no other workgroup writes v during the pass);
-    // otherwise possibly stale, and the atomicMin below decides.
-    const int32_t cur = dist[v];
-    if (acc >= cur) return;
+  __device__ __forceinline__ int32_t vertex_state(int32_t v) { return dist[v]; }
+  // cur: dist[v] at the start of the tile; exact when `whole`, otherwise
+  // possibly stale (larger), and the atomicMin decides.
+  __device__ __forceinline__ void item_done(int32_t v, int32_t acc, bool whole, int32_t cur) {
+    if (acc == INT_MAX || acc >= cur) return;
     int32_t old = cur;
     if (whole) {
       dist[v] = acc;
@@ -292,7 +297,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       PullOp op{g->dg.row_offsets, g->dg.row_indices, static_cast<const int32_t *>(wcv), dist, dirty, ctr, {}};
       {
         TimedRegion t(g->prof, "sssp.pull");
-        hipLaunchKernelGGL((k_pull_reduce<PullOp, int32_t>), dim3(2048), dim3(kExpandBlock), 0, s, op,
+        hipLaunchKernelGGL((k_pull_staged<PullOp, int32_t>), dim3(2048), dim3(kExpandBlock), 0, s, op,
                            g->dg.column_offsets, n, m, pparts, (const int32_t *)nullptr);
         GG_HIP(hipGetLastError());
       }
