@@ -38,24 +38,16 @@ namespace {
 using namespace GASengine;
 constexpr int32_t kInf = 100000000;
 
-// counters: [2] LBS edge counter, [4] improvements, [5] newly reached,
-// [6] out-degree sum of newly reached, [16] dirty left, [17] min dist left,
-// [18] out-degree sum of the frontier.
+// counters: [2] LBS edge counter, [4] improvements, [16] dirty left,
+// [17] min dist left, [18] out-degree sum of the frontier.  Reached vertices
+// and their degree sum (the run's statistics) come from one pass over the
+// final distances (reach_stats), not from the relaxations.
 struct Tally {
-  unsigned long long improved = 0, reached = 0, mf = 0;
-  __device__ __forceinline__ void add(int32_t old, const int32_t *ro, int32_t v) {
-    improved++;
-    if (old == kInf) {
-      reached++;
-      mf += (unsigned long long)(ro[v + 1] - ro[v]);
-    }
-  }
-
+  unsigned long long improved = 0;
+  __device__ __forceinline__ void add() { improved++; }
   // All threads of the workgroup must call it.
   __device__ __forceinline__ void flush(unsigned long long *counters) {
     block_counter_add<kExpandBlock>(&counters[4], improved);
-    block_counter_add<kExpandBlock>(&counters[5], reached);
-    block_counter_add<kExpandBlock>(&counters[6], mf);
   }
 };
 
@@ -88,7 +80,7 @@ struct PushOp {
     const int32_t nd = (int32_t)l.nd;
     const int32_t old = atomicMin(&dist[l.nb], nd);
     if (nd >= old) return false;
-    tally.add(old, ro, l.nb);
+    tally.add();
     dirty[l.nb] = 1;
     return false;
   }
@@ -105,7 +97,6 @@ struct PushOp {
 // Runs on GASengine::k_pull_staged: an edge's source and weight load with
 // the tile, the gather of dist[u] after, dist[v] is prefetched.
 struct PullOp {
-  const int32_t *ro;
   const int32_t *ri;
   const int32_t *wc;
   int32_t *dist;
@@ -137,7 +128,7 @@ struct PullOp {
       old = atomicMin(&dist[v], acc);
       if (acc >= old) return;
     }
-    tally.add(old, ro, v);
+    tally.add();
     dirty[v] = 1;
   }
   __device__ __forceinline__ void finish() { tally.flush(counters); }
@@ -261,13 +252,6 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   int iterations = 0, pulls = 0;
   int64_t edges = 0, prev_edges = 0, prev_improved = 0;
   double bytes = 5.0 * n;
-  int64_t reached_edges0;
-  {
-    int32_t h[2];
-    GG_HIP(hipMemcpyAsync(h, g->dg.row_offsets + src, 8, hipMemcpyDeviceToHost, s));
-    GG_HIP(hipStreamSynchronize(s));
-    reached_edges0 = h[1] - h[0];
-  }
   const int collect_grid = grid_for(n, kCollectStep, 2048);
   while (iterations < g->opt.iter_num) {
     GG_HIP(slot_resets_async(ctr, {{8, 0ull}, {9, 0ull}, {16, 0ull}, {17, (unsigned long long)INT_MAX}, {18, 0ull}}, s));
@@ -294,7 +278,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     (void)N;
     prev_improved = improved_so_far;
     if (pull_alpha > 0 && degsum > m / pull_alpha) {
-      PullOp op{g->dg.row_offsets, g->dg.row_indices, static_cast<const int32_t *>(wcv), dist, dirty, ctr, {}};
+      PullOp op{g->dg.row_indices, static_cast<const int32_t *>(wcv), dist, dirty, ctr, {}};
       {
         TimedRegion t(g->prof, "sssp.pull");
         hipLaunchKernelGGL((k_pull_staged<PullOp, int32_t>), dim3(2048), dim3(kExpandBlock), 0, s, op,
@@ -324,7 +308,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   GG_HIP(hipEventRecord(e1, s));
   GG_HIP(hipEventSynchronize(e1));
-  const int64_t improved = g->h64()[4], reached = 1 + g->h64()[5], reached_edges = reached_edges0 + g->h64()[6];
+  const int64_t improved = g->h64()[4];
   bytes += 8.0 * improved;  // dist + dirty writes of every improvement
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
@@ -334,10 +318,9 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   finish_profile(g);
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
+    GG_TRY(reach_stats(g, dist, kInf, stats));
     stats->iterations = iterations;
     stats->pull_iterations = pulls;
-    stats->reached_vertices = reached;
-    stats->reached_edges = reached_edges;
     stats->edges_inspected = edges;
     stats->alg_bytes = bytes;
     stats->gpu_ms = ms;
