@@ -67,8 +67,11 @@ struct BatchState {
   int32_t ppr_iters, ppr_done;
   double resid_last_max;
   double dang_cur[kPprMax];
-  double dang_acc[kPprMax];
-  double resid_acc[kPprMax];
+  // Per-instance sums of a sweep, striped over kStripes copies (stride
+  // kPprMax doubles; see GASengine::block_counter_add), folded by
+  // k_ppr_finalize.
+  double dang_acc[kStripes * kPprMax];
+  double resid_acc[kStripes * kPprMax];
 };
 
 // ---------------------------------------------------------------- SSSP --
@@ -385,7 +388,7 @@ __device__ __forceinline__ void lane_add_f64(double *acc_per_instance, double v,
   if (threadIdx.x < kWave) {
     double t = 0.0;
     for (int j = 0; j < BLOCK / kWave; j++) t += s_red[j * kWave + threadIdx.x];
-    if (t != 0.0) atomicAdd(&acc_per_instance[threadIdx.x], t);
+    if (t != 0.0) atomicAdd(stripe_of<kPprMax>(acc_per_instance) + threadIdx.x, t);
   }
   __syncthreads();
 }
@@ -499,10 +502,15 @@ __global__ void k_ppr_finalize(BatchState *st, int32_t nseeds, double tol, int32
   __shared__ double s_max[kPprMax];
   if (st->ppr_done) return;
   const int l = threadIdx.x;
-  const double r = l < nseeds ? st->resid_acc[l] : 0.0;
-  st->dang_cur[l] = st->dang_acc[l];
-  st->dang_acc[l] = 0.0;
-  st->resid_acc[l] = 0.0;
+  double r = 0.0, dg = 0.0;
+  for (int q = 0; q < kStripes; q++) {
+    r += st->resid_acc[q * kPprMax + l];
+    dg += st->dang_acc[q * kPprMax + l];
+    st->resid_acc[q * kPprMax + l] = 0.0;
+    st->dang_acc[q * kPprMax + l] = 0.0;
+  }
+  if (l >= nseeds) r = 0.0;
+  st->dang_cur[l] = dg;
   s_max[l] = r;
   __syncthreads();
   if (l == 0) {
@@ -547,11 +555,8 @@ __global__ void k_bs_state_init(BatchState *st, int32_t *q0, int32_t *q1, uint32
   st->ppr_iters = 0;
   st->ppr_done = ppr_on ? 0 : 1;
   st->resid_last_max = INFINITY;
-  for (int i = 0; i < kPprMax; i++) {
-    st->dang_cur[i] = 0.0;
-    st->dang_acc[i] = 0.0;
-    st->resid_acc[i] = 0.0;
-  }
+  for (int i = 0; i < kPprMax; i++) st->dang_cur[i] = 0.0;
+  for (int i = 0; i < kStripes * kPprMax; i++) st->dang_acc[i] = st->resid_acc[i] = 0.0;
 }
 
 }  // namespace

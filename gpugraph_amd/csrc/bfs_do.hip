@@ -71,6 +71,10 @@ __device__ __forceinline__ uint32_t fbit(const uint32_t *__restrict__ b, int32_t
   return (b[u >> 5] >> (u & 31)) & 1u;
 }
 
+// ro: CSR offsets for the out-degree sum of the new frontier; nullptr on an
+// undirected (symmetric) graph, where the in-row length is the out-degree.
+// Each wave owns the two bitmap words of its 64 vertices per step, so the
+// next step's visited word and in-row offsets are loaded a step ahead.
 __global__ void __launch_bounds__(256)
 k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, const int32_t *__restrict__ ro,
                 int32_t *__restrict__ depth, uint32_t *__restrict__ visited, const uint32_t *__restrict__ front,
@@ -80,19 +84,30 @@ k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, 
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   const int lane = lane_id();
   unsigned long long found_cnt = 0, mf = 0, examined = 0, scanned = 0;
-  for (int64_t p = (((int64_t)blockIdx.x * blockDim.x) + threadIdx.x) >> 6; p * 64 < n; p += nwaves) {
-    const int32_t w = (int32_t)(2 * p + (lane >> 5));
-    const int32_t v = (int32_t)(p * 64 + lane);
-    bool todo = false;
-    int32_t b = 0, e = 0;
+  int64_t p = (((int64_t)blockIdx.x * blockDim.x) + threadIdx.x) >> 6;
+  uint32_t vw = 0;  // visited word of this lane's vertex, and its in-row
+  int32_t b = 0, e = 0;
+  {
+    const int64_t v = p * 64 + lane;
     if (v < n) {
-      todo = !((visited[w] >> (lane & 31)) & 1u);
-      if (todo) {
-        b = co[v];
-        e = co[v + 1];
-        todo = e > b;
+      vw = visited[2 * p + (lane >> 5)];
+      b = co[v];
+      e = co[v + 1];
+    }
+  }
+  for (; p * 64 < n; p += nwaves) {
+    const int32_t v = (int32_t)(p * 64 + lane);
+    uint32_t vw_next = 0;
+    int32_t b_next = 0, e_next = 0;
+    {
+      const int64_t pn = p + nwaves, vn = pn * 64 + lane;
+      if (vn < n) {
+        vw_next = visited[2 * pn + (lane >> 5)];
+        b_next = co[vn];
+        e_next = co[vn + 1];
       }
     }
+    bool todo = v < n && !((vw >> (lane & 31)) & 1u) && e > b;
     if (todo) scanned++;
     const bool coop = todo && (e - b) > serial_max;
     bool found = false;
@@ -128,7 +143,7 @@ k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, 
     const uint64_t mask = ballot(found);
     if (found) {
       depth[v] = next_depth;
-      mf += (unsigned long long)(ro[v + 1] - ro[v]);
+      mf += (unsigned long long)(ro ? ro[v + 1] - ro[v] : e - b);
     }
     if (lane < 2) {
       const int32_t ww = (int32_t)(2 * p + lane);
@@ -139,6 +154,9 @@ k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, 
       }
     }
     if (lane == 0) found_cnt += (unsigned long long)__popcll(mask);
+    vw = vw_next;
+    b = b_next;
+    e = e_next;
   }
   block_counter_add<256>(&counters[0], found_cnt);
   block_counter_add<256>(&counters[1], mf);
@@ -198,7 +216,8 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
   uint32_t *visited = g->bits[0].as<uint32_t>();
   uint32_t *fbits[2] = {g->bits[1].as<uint32_t>(), g->bits[2].as<uint32_t>()};
   int32_t *queue[2] = {g->frontier[0].as<int32_t>(), g->frontier[1].as<int32_t>()};
-  // counters: [0..3] per level (see BfsPushOp); slots 8, 9 hold the int32 queue sizes
+  // counters (striped, GASengine::block_counter_add): [0..3] per level (see
+  // BfsPushOp); slots 8, 9 hold the int32 queue sizes (stripe 0 only)
   unsigned long long *ctr = g->counters.as<unsigned long long>();
   int32_t *qsz[2] = {reinterpret_cast<int32_t *>(ctr + 8), reinterpret_cast<int32_t *>(ctr + 9)};
   g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
@@ -212,14 +231,13 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
     TimedRegion t(g->prof, "bfs.init");
     GG_HIP(fill_async<int32_t>(depth, -1, n, s));
     GG_HIP(fill_async<uint32_t>(visited, 0u, words, s));
-    GG_HIP(fill_async<unsigned long long>(ctr, 0ull, 16, s));
+    GG_HIP(fill_async<unsigned long long>(ctr, 0ull, kCounterWords, s));
     hipLaunchKernelGGL(k_bfs_init, dim3(1), dim3(1), 0, s, depth, visited, queue[0], qsz[0], g->dg.row_offsets, src,
                        ctr);
     GG_HIP(hipGetLastError());
   }
   g->prof.add_bytes("bfs.init", 4.0 * n + words * 4.0);
-  GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-  GG_HIP(hipStreamSynchronize(s));
+  GG_TRY(read_counters(g, 16));
   int64_t mf = g->h64()[1];
   int64_t mu = g->m - mf;
   int64_t nf = 1, prev_nf = 0;
@@ -254,12 +272,12 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
       {
         TimedRegion t(g->prof, "bfs.bottom_up");
         hipLaunchKernelGGL(k_bfs_bottom_up, dim3(grid_for((int64_t)words * 32, 256, 2048)), dim3(256), 0, s,
-                           g->dg.column_offsets, g->dg.row_indices, g->dg.row_offsets, depth, visited, fbits[cur],
+                           g->dg.column_offsets, g->dg.row_indices, g->directed ? g->dg.row_offsets : nullptr,
+                           depth, visited, fbits[cur],
                            fbits[nxt], n, levels + 1, g->opt.wave_degree, ctr);
         GG_HIP(hipGetLastError());
       }
-      GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-      GG_HIP(hipStreamSynchronize(s));
+      GG_TRY(read_counters(g, 4));
       const int64_t N = g->h64()[0], E = g->h64()[2], U = g->h64()[3];
       lvl_bytes = 8.0 * U + 4.0 * E + n / 8.0 + 8.0 * N;
       g->prof.add_launch_bytes("bfs.bottom_up", lvl_bytes);
@@ -272,8 +290,7 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
       lbs_launch(op, false, queue[cur], qsz[cur], nf, (int64_t)n + g->m, lb, queue[nxt], qsz[nxt], s,
                  lbs_timer);
       GG_HIP(hipGetLastError());
-      GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 10 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-      GG_HIP(hipStreamSynchronize(s));
+      GG_TRY(read_counters(g, 10));
       const int64_t E = g->h64()[2];
       const int64_t N = g->h_pinned[(8 + nxt) * 2];
       lvl_bytes = 12.0 * nf + 8.0 * E + 8.0 * N;

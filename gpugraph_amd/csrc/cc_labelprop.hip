@@ -47,6 +47,7 @@ using namespace GASengine;
 constexpr int kIters = 16;  // iterations per graph replay
 constexpr int kBlock = 256;
 constexpr int kPullDiv = 8;  // pull while the frontier exceeds n / kPullDiv
+constexpr int kCcStride = 16;  // 128 bytes between the stripes of a counter
 
 struct CcState {
   int32_t *q_cur, *q_nxt;
@@ -60,6 +61,10 @@ struct CcState {
   int32_t prev_pull, pull_skip;       // last iteration pulled; pull kernel off (done || !pull)
   int32_t fill;                       // queue fill position of the bitmap compaction
   unsigned long long pull_changed;    // vertices changed by this iteration's pull
+  // Striped per-workgroup sums (GASengine::block_counter_add, stride
+  // kCcStride), folded into edges / pull_changed by k_lp_swap.
+  unsigned long long edge_stripes[kStripes * kCcStride];
+  unsigned long long changed_stripes[kStripes * kCcStride];
 };
 
 __global__ void k_lp_state_init(CcState *st, int32_t *q0, int32_t *q1, uint32_t *b0, uint32_t *b1, int32_t n) {
@@ -80,6 +85,7 @@ __global__ void k_lp_state_init(CcState *st, int32_t *q0, int32_t *q1, uint32_t 
   st->pull_skip = !st->pull || st->done;
   st->fill = 0;
   st->pull_changed = 0;
+  for (int q = 0; q < kStripes * kCcStride; q++) st->edge_stripes[q] = st->changed_stripes[q] = 0;
 }
 
 __device__ __forceinline__ bool offer(int32_t *label, int32_t v, int32_t lu, uint32_t *bits) {
@@ -96,7 +102,6 @@ k_lp_expand(CcState *st, const int32_t *__restrict__ ro, const int32_t *__restri
             int32_t wave_degree) {
   __shared__ int32_t s_q[kBlock * 4];
   __shared__ int32_t s_n, s_base;
-  __shared__ unsigned long long s_red[kBlock / kWave];
   if (st->done || st->pull) return;
   const int32_t nf = st->cur_size;
   const int32_t *q = st->q_cur;
@@ -156,8 +161,7 @@ k_lp_expand(CcState *st, const int32_t *__restrict__ ro, const int32_t *__restri
     }
     __syncthreads();
   }
-  const unsigned long long t = block_sum_u64<kBlock>(edges, s_red);
-  if (threadIdx.x == 0 && t) atomicAdd(&st->edges, t);
+  block_counter_add<kBlock, kCcStride>(st->edge_stripes, edges);
 }
 
 // Pull operator for GASengine::k_pull_staged over the (symmetric) adjacency.
@@ -205,7 +209,7 @@ struct CcPullOp {
     const uint32_t old = atomicOr(&st->b_nxt[w], word);
     count += (unsigned long long)__popc(word & ~old);
   }
-  __device__ __forceinline__ void finish() { block_counter_add<kExpandBlock>(&st->pull_changed, count); }
+  __device__ __forceinline__ void finish() { block_counter_add<kExpandBlock, kCcStride>(st->changed_stripes, count); }
 };
 
 // Start of an iteration: clear the output bitmap before a pull, and before
@@ -244,6 +248,15 @@ k_lp_prep(CcState *st, int32_t words) {
 
 __global__ void k_lp_swap(CcState *st, int32_t n, int64_t m, int32_t max_iter) {
   if (st->done) return;
+  unsigned long long pushed = 0, changed = 0;
+  for (int q = 0; q < kStripes; q++) {
+    pushed += st->edge_stripes[q * kCcStride];
+    changed += st->changed_stripes[q * kCcStride];
+    st->edge_stripes[q * kCcStride] = 0;
+    st->changed_stripes[q * kCcStride] = 0;
+  }
+  st->edges += pushed;
+  st->pull_changed += changed;
   if (st->pull) {
     st->edges += (unsigned long long)m;
     st->pulls++;

@@ -48,33 +48,47 @@ int copy_out(gg_graph *g, void *dst, const void *src, size_t bytes, uint32_t fla
 
 __global__ void k_reach(const int32_t *__restrict__ res, int32_t sentinel, const int32_t *__restrict__ ro, int32_t n,
                         unsigned long long *__restrict__ out) {
-  __shared__ unsigned long long red[256 / 64];
   unsigned long long cnt = 0, deg = 0;
   for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
     if (res[v] != sentinel) {
       cnt++;
       deg += (unsigned long long)(ro[v + 1] - ro[v]);
     }
-  cnt = GASengine::block_sum_u64<256>(cnt, red);
-  deg = GASengine::block_sum_u64<256>(deg, red);
-  if (threadIdx.x == 0) {
-    if (cnt) atomicAdd(&out[0], cnt);
-    if (deg) atomicAdd(&out[1], deg);
-  }
+  GASengine::block_counter_add<256>(&out[0], cnt);
+  GASengine::block_counter_add<256>(&out[1], deg);
 }
 
 int reach_stats(gg_graph *g, const int32_t *result, int32_t sentinel, gg_stats *st) {
   if (!st) return GG_OK;
-  GG_TRY(g->counters.ensure(64 * sizeof(unsigned long long)));
-  auto *c = g->counters.as<unsigned long long>() + 32;
-  GG_HIP(GASengine::fill_async<unsigned long long>(c, 0ull, 2, g->stream));
+  GG_TRY(g->counters.ensure(GASengine::kCounterWords * sizeof(unsigned long long)));
+  auto *c = g->counters.as<unsigned long long>();
+  GG_HIP(GASengine::slot_resets_async(c, {{32, 0ull}, {33, 0ull}}, g->stream));
   hipLaunchKernelGGL(k_reach, dim3(GASengine::grid_for(g->n, 256, 2048)), dim3(256), 0, g->stream, result, sentinel,
-                     g->dg.row_offsets, g->n, c);
+                     g->dg.row_offsets, g->n, c + 32);
   GG_HIP(hipGetLastError());
-  GG_HIP(hipMemcpyAsync(g->h_pinned, c, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g->stream));
+  GG_TRY(read_counters(g, 34));
+  st->reached_vertices = g->h64()[32];
+  st->reached_edges = g->h64()[33];
+  return GG_OK;
+}
+
+int read_counters(gg_graph *g, int count, uint64_t min_mask) {
+  using GASengine::kCounterSlots;
+  using GASengine::kStripes;
+  GG_CHECK_ARG(count > 0 && count <= kCounterSlots, "counter range");
+  uint64_t *raw = reinterpret_cast<uint64_t *>(g->h64() + kCounterSlots);
+  GG_HIP(hipMemcpy2DAsync(raw, count * sizeof(uint64_t), g->counters.p, kCounterSlots * sizeof(uint64_t),
+                          count * sizeof(uint64_t), kStripes, hipMemcpyDeviceToHost, g->stream));
   GG_HIP(hipStreamSynchronize(g->stream));
-  st->reached_vertices = g->h64()[0];
-  st->reached_edges = g->h64()[1];
+  for (int k = 0; k < count; k++) {
+    const bool is_min = (min_mask >> k) & 1u;
+    uint64_t t = raw[k];
+    for (int q = 1; q < kStripes; q++) {
+      const uint64_t v = raw[(size_t)q * count + k];
+      t = is_min ? (v < t ? v : t) : t + v;
+    }
+    g->h64()[k] = (int64_t)t;
+  }
   return GG_OK;
 }
 
@@ -186,7 +200,7 @@ int check_undirected(gg_graph *g) {
   const int32_t *parts = nullptr;
   GG_TRY(pull_partition(g, &parts));
   GG_TRY(g->eid.ensure(4 * (size_t)std::max<int64_t>(g->m, 1)));
-  GG_TRY(g->counters.ensure(64 * sizeof(unsigned long long)));
+  GG_TRY(g->counters.ensure(GASengine::kCounterWords * sizeof(unsigned long long)));
   unsigned int *flags = reinterpret_cast<unsigned int *>(g->counters.as<unsigned long long>() + 40);
   GG_HIP(GASengine::fill_async<unsigned int>(flags, 0u, 1, g->stream));
   MirrorOp op{g->dg.row_offsets, g->dg.column_indices, g->val.as<uint32_t>(), g->eid.as<int32_t>(), flags};
@@ -218,7 +232,7 @@ int lbs_buffers(gg_graph *g, GASengine::LbsBuffers *b) {
   GG_TRY(g->partials.ensure(4 * (n / GASengine::kCountTile + 2)));
   GG_TRY(g->parts.ensure(4 * GASengine::lbs_partition_entries(g->n, g->m)));
   GG_TRY(g->info.ensure(sizeof(GASengine::LbsInfo)));
-  GG_TRY(g->counters.ensure(64 * sizeof(unsigned long long)));
+  GG_TRY(g->counters.ensure(GASengine::kCounterWords * sizeof(unsigned long long)));
   b->scan_local = g->scan_local.as<int32_t>();
   b->partials = g->partials.as<int32_t>();
   b->info = g->info.as<GASengine::LbsInfo>();
@@ -352,7 +366,7 @@ int gg_graph_create(int32_t n, int64_t m, const int32_t *row_offsets, const int3
   g->directed = (flags & GG_GRAPH_DIRECTED) ? 1 : 0;
   g->value_dtype = value_dtype;
   gg_default_options(&g->opt);
-  GG_HIP(hipHostMalloc((void **)&g->h_pinned, 64 * sizeof(int64_t)));
+  GG_HIP(hipHostMalloc((void **)&g->h_pinned, (GASengine::kCounterSlots + GASengine::kCounterWords) * sizeof(int64_t)));
 
   const hipMemcpyKind kind = dev_in ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   GG_TRY(g->ro.ensure(4 * ((size_t)n + 1)));
@@ -365,7 +379,7 @@ int gg_graph_create(int32_t n, int64_t m, const int32_t *row_offsets, const int3
   }
   // Validate the CSR on the device before any kernel indexes with it.
   {
-    GG_TRY(g->counters.ensure(64 * sizeof(unsigned long long)));
+    GG_TRY(g->counters.ensure(GASengine::kCounterWords * sizeof(unsigned long long)));
     int32_t *bad = reinterpret_cast<int32_t *>(g->counters.as<unsigned long long>());
     const int32_t init[4] = {0, INT32_MAX, 0, INT32_MAX};
     GG_HIP(hipMemcpyAsync(bad, init, sizeof(init), hipMemcpyHostToDevice, g->stream));

@@ -171,7 +171,7 @@ struct gg_graph {
   int32_t sl_mid_base[2][9] = {}, sl_large_base[2][9] = {};
   bool sl_ready[2] = {false, false};
   int64_t sl_val_version[2] = {-1, -1};
-  int32_t *h_pinned = nullptr;  // pinned readback area (64 x int64)
+  int32_t *h_pinned = nullptr;  // pinned readback area (64 x int64, then the raw stripes of read_counters)
   bool pr_cache_valid = false;
 
   // Graph-derived caches: the static merge-path partition of the CSC (pull
@@ -207,6 +207,11 @@ int copy_out(gg_graph *g, void *dst, const void *src, size_t bytes, uint32_t fla
 // reached vertices / reached-edge (degree) sums for a result array: counts
 // entries != sentinel.  Synchronises the stream.
 int reach_stats(gg_graph *g, const int32_t *result, int32_t sentinel, gg_stats *st);
+// Copy counters [0, count) of the graph's striped counter array
+// (GASengine::kStripes stripes of kCounterSlots words) to h64()[0, count),
+// each folded over its stripes: a sum, or a minimum for the bits set in
+// min_mask.  Synchronises the stream.
+int read_counters(gg_graph *g, int count, uint64_t min_mask = 0);
 // Finish a run: resolve profiler events.
 void finish_profile(gg_graph *g);
 // Frontier queues, scan/partition buffers and counters of the specialised

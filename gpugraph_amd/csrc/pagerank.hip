@@ -48,13 +48,28 @@ namespace {
 using namespace GASengine;
 constexpr int kItersPerCheck = 8;
 
+constexpr int kPrStride = 16;  // doubles between the stripes of a sum (128 bytes)
+
 struct PrState {
-  double resid_acc, aux_acc;  // accumulated by the sweep
   double aux_cur;             // dangling mass (forward) / sum of lambda (adjoint) of the current vector
   double scale;               // residual scale: 1 (forward) or ||c||_1 (adjoint)
   double resid_last;
   int32_t iters, done;
+  // Per-workgroup FP64 sums, striped (see GASengine::block_counter_add):
+  // accumulated by a sweep (or k_pr_init), folded by k_pr_finalize /
+  // k_pr_start.
+  double resid_acc[kStripes * kPrStride], aux_acc[kStripes * kPrStride], scale_acc[kStripes * kPrStride];
 };
+
+// Fold a striped FP64 sum and clear its stripes (one thread).
+__device__ __forceinline__ double take_stripes(double *acc) {
+  double t = 0.0;
+  for (int q = 0; q < kStripes; q++) {
+    t += acc[q * kPrStride];
+    acc[q * kPrStride] = 0.0;
+  }
+  return t;
+}
 
 template <int BLOCK>
 __device__ __forceinline__ void block_add_f64(double *p, double v) {
@@ -66,7 +81,7 @@ __device__ __forceinline__ void block_add_f64(double *p, double v) {
   if (threadIdx.x == 0) {
     double t = 0;
     for (int w = 0; w < BLOCK / kWave; w++) t += red[w];
-    if (t != 0.0) atomicAdd(p, t);
+    if (t != 0.0) atomicAdd(stripe_of<kPrStride>(p), t);
   }
   __syncthreads();
 }
@@ -116,8 +131,8 @@ struct PrForwardOp {
     if (io == 0.0f) dang += rv;
   }
   __device__ __forceinline__ void finish() {
-    block_add_f64<kRowBlock>(&st->resid_acc, resid);
-    block_add_f64<kRowBlock>(&st->aux_acc, dang);
+    block_add_f64<kRowBlock>(st->resid_acc, resid);
+    block_add_f64<kRowBlock>(st->aux_acc, dang);
   }
 };
 
@@ -161,17 +176,15 @@ struct PrAdjointOp {
     lsum += ln_u;
   }
   __device__ __forceinline__ void finish() {
-    block_add_f64<kRowBlock>(&st->resid_acc, resid);
-    block_add_f64<kRowBlock>(&st->aux_acc, lsum);
+    block_add_f64<kRowBlock>(st->resid_acc, resid);
+    block_add_f64<kRowBlock>(st->aux_acc, lsum);
   }
 };
 
 __global__ void k_pr_finalize(PrState *st, double tol, int32_t max_iter) {
   if (st->done) return;
-  st->resid_last = st->resid_acc;
-  st->aux_cur = st->aux_acc;
-  st->resid_acc = 0.0;
-  st->aux_acc = 0.0;
+  st->resid_last = take_stripes(st->resid_acc);
+  st->aux_cur = take_stripes(st->aux_acc);
   st->iters++;
   if (st->resid_last <= tol * st->scale || st->iters >= max_iter) st->done = 1;
 }
@@ -194,15 +207,15 @@ k_pr_init(float *r0, float *x0, const float *tele, const float *inv_out, int32_t
       if (inv_out[v] == 0.0f) aux += rv;
     }
   }
-  block_add_f64<kRowBlock>(&st->aux_acc, aux);
-  block_add_f64<kRowBlock>(&st->scale, mag);
+  block_add_f64<kRowBlock>(st->aux_acc, aux);
+  block_add_f64<kRowBlock>(st->scale_acc, mag);
 }
 
 __global__ void k_pr_start(PrState *st, int adjoint) {
+  st->scale = take_stripes(st->scale_acc);
   if (!adjoint || st->scale == 0.0) st->scale = 1.0;
-  st->aux_cur = st->aux_acc;
-  st->aux_acc = 0.0;
-  st->resid_acc = 0.0;
+  st->aux_cur = take_stripes(st->aux_acc);
+  (void)take_stripes(st->resid_acc);
   st->resid_last = INFINITY;
   st->iters = 0;
   st->done = 0;
@@ -325,7 +338,7 @@ int ensure_bins(gg_graph *g, int dir) {
   if (g->bins_ready[dir]) return GG_OK;
   const int32_t *off = dir == 0 ? g->dg.column_offsets : g->dg.row_offsets;
   for (int b = 0; b < 3; b++) GG_TRY(g->bins[dir][b].ensure(4 * (size_t)g->n));
-  GG_TRY(g->counters.ensure(64 * sizeof(unsigned long long)));
+  GG_TRY(g->counters.ensure(kCounterWords * sizeof(unsigned long long)));
   int32_t *cnt = reinterpret_cast<int32_t *>(g->counters.as<unsigned long long>() + 48);
   GG_HIP(fill_async<int32_t>(cnt, 0, 3, g->stream));
   hipLaunchKernelGGL(k_build_bins<0>, dim3(grid_for(g->n, kRowBlock, 4096)), dim3(kRowBlock), 0, g->stream, off, g->n,

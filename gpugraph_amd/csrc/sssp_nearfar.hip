@@ -38,8 +38,10 @@ namespace {
 using namespace GASengine;
 constexpr int32_t kInf = 100000000;
 
-// counters: [2] LBS edge counter, [4] improvements, [16] dirty left,
-// [17] min dist left, [18] out-degree sum of the frontier.  Reached vertices
+// counters (striped, GASengine::block_counter_add): [2] LBS edge counter,
+// [4] improvements, [16] dirty left, [17] min dist left (minimum over the
+// stripes), [18] out-degree sum of the frontier; slots 8, 9 hold the int32
+// queue sizes (stripe 0 only).  Reached vertices
 // and their degree sum (the run's statistics) come from one pass over the
 // final distances (reach_stats), not from the relaxations.
 struct Tally {
@@ -195,7 +197,7 @@ k_sssp_collect(uint8_t *__restrict__ dirty, const int32_t *__restrict__ dist, co
   if (threadIdx.x == 0) {
     int32_t mn = INT_MAX;
     for (int w = 0; w < kCollectBlock / kWave; w++) mn = min(mn, s_min[w]);
-    if (mn != INT_MAX) atomicMin(&counters[17], (unsigned long long)mn);
+    if (mn != INT_MAX) atomicMin(stripe_of(&counters[17]), (unsigned long long)mn);
   }
 }
 
@@ -243,7 +245,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     TimedRegion t(g->prof, "sssp.init");
     GG_HIP(fill_async<int32_t>(dist, kInf, n, s));
     GG_HIP(fill_async<uint8_t>(dirty, (uint8_t)0, n, s));
-    GG_HIP(fill_async<unsigned long long>(ctr, 0ull, 24, s));
+    GG_HIP(fill_async<unsigned long long>(ctr, 0ull, kCounterWords, s));
     hipLaunchKernelGGL(k_sssp_init, dim3(1), dim3(1), 0, s, dist, dirty, src);
     GG_HIP(hipGetLastError());
   }
@@ -261,8 +263,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
                          (int32_t)std::min<int64_t>(bound, kInf), queue, qsize, ctr);
       GG_HIP(hipGetLastError());
     }
-    GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 20 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    GG_HIP(hipStreamSynchronize(s));
+    GG_TRY(read_counters(g, 20, 1ull << 17));
     const int64_t nf = reinterpret_cast<int32_t *>(g->h_pinned + 16)[0];
     const int64_t remain = g->h64()[16], dmin = g->h64()[17], degsum = g->h64()[18];
     const int64_t improved_so_far = g->h64()[4];
@@ -305,9 +306,8 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     }
     iterations++;
   }
-  GG_HIP(hipMemcpyAsync(g->h_pinned, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   GG_HIP(hipEventRecord(e1, s));
-  GG_HIP(hipEventSynchronize(e1));
+  GG_TRY(read_counters(g, 8));
   const int64_t improved = g->h64()[4];
   bytes += 8.0 * improved;  // dist + dirty writes of every improvement
   float ms = 0.f;
