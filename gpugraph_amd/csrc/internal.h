@@ -106,6 +106,23 @@ class EventProfiler : public GASengine::KernelTimer {
   hipEvent_t take();
 };
 
+// An instantiated capture of a device-driven iteration loop, cached on a
+// graph handle with the kernel arguments it was captured with (`key`); a
+// call whose arguments differ captures again.
+struct CapturedLoop {
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint64_t> key;
+  CapturedLoop() = default;
+  CapturedLoop(const CapturedLoop &) = delete;
+  CapturedLoop &operator=(const CapturedLoop &) = delete;
+  ~CapturedLoop() { reset(); }
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key.clear();
+  }
+};
+
 // Scoped helper for kernels launched outside the generic enactor.
 struct TimedRegion {
   EventProfiler &p;
@@ -191,6 +208,11 @@ struct gg_graph {
 
   // Generic-program state kept across runs (one per program type).
   std::shared_ptr<void> prog_state[8];
+
+  // Device-driven BFS / SSSP loops: captured launch sequences and their
+  // device-side loop state.
+  gg::CapturedLoop bfs_loop, sssp_loop;
+  gg::DevBuf loop_ctl;
 
   gg::EventProfiler prof;
 

@@ -29,6 +29,7 @@
 
 #include <climits>
 #include <cstring>
+#include <vector>
 
 #include "internal.h"
 
@@ -37,6 +38,7 @@ namespace {
 
 using namespace GASengine;
 constexpr int32_t kInf = 100000000;
+constexpr int kLoopIters = 12;  // iterations per replay of the captured loop
 
 // counters (striped, GASengine::block_counter_add): [2] LBS edge counter,
 // [4] improvements, [16] dirty left, [17] min dist left (minimum over the
@@ -141,6 +143,63 @@ __global__ void k_sssp_init(int32_t *dist, uint8_t *dirty, int32_t src) {
   dirty[src] = 1;
 }
 
+// State of the device-driven loop: k_sssp_ctrl takes, after each collect,
+// the decisions the host loop takes (bucket advance, push or pull, end).
+struct SsspCtl {
+  int32_t bound;      // frontier bound of the current bucket
+  int32_t push_size;  // frontier items the LBS push sees: the queue size when pushing, else 0
+  int32_t pull_skip;  // 0 when this iteration pulls
+  int32_t done;
+  int32_t iters, pulls;
+  long long edges;    // edges inspected
+  double bytes;       // algorithmic bytes (model in the header comment)
+};
+
+__global__ void k_sssp_ctl_init(SsspCtl *c, int32_t bound, double bytes0) {
+  c->bound = bound;
+  c->push_size = 0;
+  c->pull_skip = 1;
+  c->done = 0;
+  c->iters = 0;
+  c->pulls = 0;
+  c->edges = 0;
+  c->bytes = bytes0;
+}
+
+__global__ void k_sssp_ctrl(SsspCtl *c, const unsigned long long *__restrict__ ctr, int32_t n, int64_t m,
+                            int64_t delta, int64_t pull_alpha, int32_t max_iter) {
+  c->push_size = 0;
+  c->pull_skip = 1;
+  if (c->done) return;
+  if (c->iters >= max_iter) {
+    c->done = 1;
+    return;
+  }
+  const int64_t nf = reinterpret_cast<const int32_t *>(ctr + 8)[0];
+  const unsigned long long remain = stripe_sum(ctr + 16), degsum = stripe_sum(ctr + 18);
+  unsigned long long dmin = ctr[17];
+  for (int q = 1; q < kStripes; q++) dmin = min(dmin, ctr[17 + q * kCounterSlots]);
+  c->bytes += n + 4.0 * (double)(nf + (int64_t)remain) + 4.0 * nf;
+  if (nf == 0) {
+    if (remain == 0)
+      c->done = 1;
+    else
+      c->bound = (int32_t)min<int64_t>(((int64_t)dmin / delta + 1) * delta, kInf);
+    return;
+  }
+  if (pull_alpha > 0 && (int64_t)degsum > m / pull_alpha) {
+    c->pull_skip = 0;
+    c->bytes += 4.0 * n + 12.0 * m;
+    c->edges += m;
+    c->pulls++;
+  } else {
+    c->push_size = (int32_t)nf;
+    c->bytes += 16.0 * nf + 12.0 * (double)degsum;
+    c->edges += (long long)degsum;
+  }
+  c->iters++;
+}
+
 // A workgroup takes 1024 consecutive vertices per step, four coalesced passes
 // of 256.
 // One global atomic per step for the queue, one per workgroup per counter.
@@ -150,7 +209,11 @@ constexpr int kCollectStep = 4 * kCollectBlock;
 __global__ void __launch_bounds__(kCollectBlock)
 k_sssp_collect(uint8_t *__restrict__ dirty, const int32_t *__restrict__ dist, const int32_t *__restrict__ ro,
                int32_t n, int32_t bound, int32_t *__restrict__ queue, int32_t *__restrict__ qsize,
-               unsigned long long *__restrict__ counters) {
+               unsigned long long *__restrict__ counters, const SsspCtl *__restrict__ ctl) {
+  if (ctl) {  // device-driven loop: the bound is on the device, and a finished run skips
+    if (ctl->done) return;
+    bound = ctl->bound;
+  }
   __shared__ int32_t s_scan[kCollectBlock / kWave + 1];
   __shared__ int32_t s_base;
   __shared__ int32_t s_min[kCollectBlock / kWave];
@@ -252,15 +315,75 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   g->prof.add_bytes("sssp.init", 5.0 * n);
   int64_t bound = delta;
   int iterations = 0, pulls = 0;
-  int64_t edges = 0, prev_edges = 0, prev_improved = 0;
+  int64_t edges = 0, prev_improved = 0;
   double bytes = 5.0 * n;
   const int collect_grid = grid_for(n, kCollectStep, 2048);
-  while (iterations < g->opt.iter_num) {
+  PullOp pull_op{g->dg.row_indices, static_cast<const int32_t *>(wcv), dist, dirty, ctr, {}};
+  PushOp push_op{g->dg.row_offsets, g->dg.column_indices, static_cast<const int32_t *>(g->dg.edge_values), dist,
+                 dirty, ctr, {}};
+  if (!g->prof.enabled) {
+    // Device-driven: kLoopIters iterations per replay of a captured launch
+    // sequence (reset, collect, k_sssp_ctrl, LBS push over ctl->push_size,
+    // pull unless ctl->pull_skip); the host reads the loop state once per
+    // replay.  Per-kernel events (GG_INSTRUMENT) need the host loop below.
+    GG_TRY(g->loop_ctl.ensure(sizeof(SsspCtl)));
+    SsspCtl *ctl = g->loop_ctl.as<SsspCtl>();
+    hipLaunchKernelGGL(k_sssp_ctl_init, dim3(1), dim3(1), 0, s, ctl, (int32_t)std::min<int64_t>(delta, kInf), bytes);
+    GG_HIP(hipGetLastError());
+    const std::vector<uint64_t> key = {
+        (uint64_t)dist, (uint64_t)dirty, (uint64_t)queue, (uint64_t)ctr, (uint64_t)ctl, (uint64_t)wcv,
+        (uint64_t)pparts, (uint64_t)lb.scan_local, (uint64_t)lb.partials, (uint64_t)lb.info, (uint64_t)lb.parts,
+        (uint64_t)lb.edge_counter, (uint64_t)lb.expand_grid, (uint64_t)lb.vt, (uint64_t)delta,
+        (uint64_t)pull_alpha, (uint64_t)g->opt.iter_num, (uint64_t)g->frontier[1].p};
+    if (!g->sssp_loop.exec || g->sssp_loop.key != key) {
+      g->sssp_loop.reset();
+      hipGraph_t graph = nullptr;
+      GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      for (int it = 0; it < kLoopIters; it++) {
+        (void)slot_resets_async(ctr, {{8, 0ull}, {9, 0ull}, {16, 0ull}, {17, (unsigned long long)INT_MAX}, {18, 0ull}},
+                                s);
+        hipLaunchKernelGGL(k_sssp_collect, dim3(collect_grid), dim3(256), 0, s, dirty, dist, g->dg.row_offsets, n, 0,
+                           queue, qsize, ctr, (const SsspCtl *)ctl);
+        hipLaunchKernelGGL(k_sssp_ctrl, dim3(1), dim3(1), 0, s, ctl, ctr, n, m, delta, pull_alpha, g->opt.iter_num);
+        lbs_launch(push_op, false, queue, &ctl->push_size, n, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(),
+                   qsize + 1, s, [](int) {});
+        if (pull_alpha > 0)
+          hipLaunchKernelGGL((k_pull_staged<PullOp, int32_t>), dim3(2048), dim3(kExpandBlock), 0, s, pull_op,
+                             g->dg.column_offsets, n, m, pparts, (const int32_t *)&ctl->pull_skip);
+      }
+      const hipError_t le = hipGetLastError();
+      const hipError_t ce = hipStreamEndCapture(s, &graph);
+      hipError_t ie = hipErrorUnknown;
+      if (le == hipSuccess && ce == hipSuccess) ie = hipGraphInstantiate(&g->sssp_loop.exec, graph, nullptr, nullptr, 0);
+      if (graph) (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        g->sssp_loop.exec = nullptr;
+        return set_error(GG_ERR_HIP, "SSSP loop capture failed: %s / %s / %s", hipGetErrorString(le),
+                         hipGetErrorString(ce), hipGetErrorString(ie));
+      }
+      g->sssp_loop.key = key;
+    }
+    for (;;) {
+      GG_HIP(hipGraphLaunch(g->sssp_loop.exec, s));
+      GG_HIP(hipMemcpyAsync(g->h_pinned, ctl, sizeof(SsspCtl), hipMemcpyDeviceToHost, s));
+      GG_HIP(hipStreamSynchronize(s));
+      SsspCtl h;
+      std::memcpy(&h, g->h_pinned, sizeof(h));
+      if (h.done) {
+        iterations = h.iters;
+        pulls = h.pulls;
+        edges = h.edges;
+        bytes = h.bytes;
+        break;
+      }
+    }
+  }
+  while (g->prof.enabled && iterations < g->opt.iter_num) {
     GG_HIP(slot_resets_async(ctr, {{8, 0ull}, {9, 0ull}, {16, 0ull}, {17, (unsigned long long)INT_MAX}, {18, 0ull}}, s));
     {
       TimedRegion t(g->prof, "sssp.collect");
       hipLaunchKernelGGL(k_sssp_collect, dim3(collect_grid), dim3(256), 0, s, dirty, dist, g->dg.row_offsets, n,
-                         (int32_t)std::min<int64_t>(bound, kInf), queue, qsize, ctr);
+                         (int32_t)std::min<int64_t>(bound, kInf), queue, qsize, ctr, (const SsspCtl *)nullptr);
       GG_HIP(hipGetLastError());
     }
     GG_TRY(read_counters(g, 20, 1ull << 17));
@@ -279,10 +402,9 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     (void)N;
     prev_improved = improved_so_far;
     if (pull_alpha > 0 && degsum > m / pull_alpha) {
-      PullOp op{g->dg.row_indices, static_cast<const int32_t *>(wcv), dist, dirty, ctr, {}};
       {
         TimedRegion t(g->prof, "sssp.pull");
-        hipLaunchKernelGGL((k_pull_staged<PullOp, int32_t>), dim3(2048), dim3(kExpandBlock), 0, s, op,
+        hipLaunchKernelGGL((k_pull_staged<PullOp, int32_t>), dim3(2048), dim3(kExpandBlock), 0, s, pull_op,
                            g->dg.column_offsets, n, m, pparts, (const int32_t *)nullptr);
         GG_HIP(hipGetLastError());
       }
@@ -292,10 +414,8 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       edges += m;
       pulls++;
     } else {
-      PushOp op{g->dg.row_offsets, g->dg.column_indices, static_cast<const int32_t *>(g->dg.edge_values), dist, dirty,
-                ctr, {}};
       // PushOp never enqueues: the output queue stays empty.
-      lbs_launch(op, false, queue, qsize, nf, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(), qsize + 1, s,
+      lbs_launch(push_op, false, queue, qsize, nf, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(), qsize + 1, s,
                  lbs_timer);
       GG_HIP(hipGetLastError());
       const double pb = 16.0 * nf + 12.0 * degsum;
@@ -314,7 +434,6 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)prev_edges;
   finish_profile(g);
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
