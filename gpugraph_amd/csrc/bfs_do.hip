@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "internal.h"
 
@@ -30,6 +31,7 @@ namespace gg {
 namespace {
 
 using namespace GASengine;
+constexpr int kLoopLevels = 6;  // levels per replay of the captured loop (even)
 
 // counters: [0] next frontier size, [1] out-degree sum of the new frontier
 // (m_f), [2] edges examined, [3] unvisited vertices scanned.
@@ -41,6 +43,7 @@ struct BfsPushOp {
   uint32_t *visited;
   unsigned long long *counters;
   int32_t next_depth;
+  const int32_t *level;  // device-driven loop: depth = *level + 1 (else next_depth)
   unsigned long long mf;
 
   __device__ __forceinline__ void prologue(int32_t) {}
@@ -50,7 +53,7 @@ struct BfsPushOp {
   __device__ __forceinline__ bool edge(int32_t, int32_t, int32_t slot, int32_t &out) {
     const int32_t nb = col[slot];
     if (!bit_test_and_set(visited, nb)) return false;
-    depth[nb] = next_depth;
+    depth[nb] = level ? *level + 1 : next_depth;
     mf += (unsigned long long)(ro[nb + 1] - ro[nb]);
     out = nb;
     return true;
@@ -79,7 +82,10 @@ __global__ void __launch_bounds__(256)
 k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, const int32_t *__restrict__ ro,
                 int32_t *__restrict__ depth, uint32_t *__restrict__ visited, const uint32_t *__restrict__ front,
                 uint32_t *__restrict__ next, int32_t n, int32_t next_depth, int32_t serial_max,
-                unsigned long long *__restrict__ counters) {
+                unsigned long long *__restrict__ counters, const int32_t *__restrict__ skip,
+                const int32_t *__restrict__ level) {
+  if (skip && *skip) return;
+  if (level) next_depth = *level + 1;
   const int32_t words = (n + 31) >> 5;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   const int lane = lane_id();
@@ -165,7 +171,8 @@ k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, 
 }
 
 __global__ void k_queue_to_bits(const int32_t *__restrict__ queue, const int32_t *__restrict__ qsize,
-                                uint32_t *__restrict__ bits) {
+                                uint32_t *__restrict__ bits, const int32_t *__restrict__ enable) {
+  if (enable && !*enable) return;
   const int32_t nq = *qsize;
   for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
     const int32_t v = queue[i];
@@ -175,7 +182,8 @@ __global__ void k_queue_to_bits(const int32_t *__restrict__ queue, const int32_t
 
 __global__ void __launch_bounds__(256)
 k_bits_to_queue(const uint32_t *__restrict__ bits, int32_t words, int32_t *__restrict__ queue,
-                int32_t *__restrict__ qsize) {
+                int32_t *__restrict__ qsize, const int32_t *__restrict__ enable) {
+  if (enable && !*enable) return;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < words; base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t w = base + threadIdx.x;
     uint32_t word = w < words ? bits[w] : 0u;
@@ -191,6 +199,89 @@ k_bits_to_queue(const uint32_t *__restrict__ bits, int32_t words, int32_t *__res
       queue[start++] = (int32_t)(w * 32 + bb);
     }
   }
+}
+
+__global__ void k_bits_clear_if(uint32_t *__restrict__ bits, int32_t words, const int32_t *__restrict__ enable) {
+  if (!*enable) return;
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (int64_t)gridDim.x * blockDim.x)
+    bits[w] = 0u;
+}
+
+// State of the device-driven loop: k_bfs_ctrl_pre / k_bfs_ctrl_post take
+// the host loop's per-level decisions (direction switches, end) and keep its
+// statistics.
+struct BfsCtl {
+  int32_t level;      // levels done; the next level writes depth level + 1
+  int32_t bottom_up;  // direction of the current level
+  int32_t q2b, b2q;   // conversions before this level
+  int32_t bu_skip;    // 1 unless this level runs bottom-up
+  int32_t push_size;  // frontier items of a top-down level, else 0
+  int32_t done, pull_levels;
+  long long nf, prev_nf, mf, mu, reached, reached_edges, examined;
+  double bytes;
+};
+
+// One thread, after k_bfs_init (counters[1] = out-degree of the source).
+__global__ void k_bfs_ctl_init(BfsCtl *c, const unsigned long long *__restrict__ ctr, int64_t m, double bytes0) {
+  const long long mf = (long long)ctr[1];
+  c->level = 0;
+  c->bottom_up = 0;
+  c->q2b = c->b2q = 0;
+  c->bu_skip = 1;
+  c->push_size = 0;
+  c->done = 0;
+  c->pull_levels = 0;
+  c->nf = 1;
+  c->prev_nf = 0;
+  c->mf = mf;
+  c->mu = m - mf;
+  c->reached = 1;
+  c->reached_edges = mf;
+  c->examined = 0;
+  c->bytes = bytes0;
+}
+
+__global__ void k_bfs_ctrl_pre(BfsCtl *c, int32_t n, int32_t alpha, int32_t beta, int32_t *__restrict__ qsize_cur) {
+  c->q2b = c->b2q = 0;
+  c->bu_skip = 1;
+  c->push_size = 0;
+  if (c->done) return;
+  if (!c->bottom_up && c->mf > c->mu / alpha && c->nf > c->prev_nf) {
+    c->q2b = 1;
+    c->bottom_up = 1;
+  } else if (c->bottom_up && c->nf < n / beta && c->nf < c->prev_nf) {
+    c->b2q = 1;
+    c->bottom_up = 0;
+    *qsize_cur = 0;  // k_bits_to_queue appends
+  }
+  if (c->bottom_up)
+    c->bu_skip = 0;
+  else
+    c->push_size = (int32_t)c->nf;
+}
+
+__global__ void k_bfs_ctrl_post(BfsCtl *c, const unsigned long long *__restrict__ ctr,
+                                const int32_t *__restrict__ qsize_nxt, int32_t n, int32_t max_levels) {
+  if (c->done) return;
+  const long long E = (long long)stripe_sum(ctr + 2);
+  long long N;
+  if (c->bottom_up) {
+    N = (long long)stripe_sum(ctr + 0);
+    c->bytes += 8.0 * (double)stripe_sum(ctr + 3) + 4.0 * E + n / 8.0 + 8.0 * N;
+    c->pull_levels++;
+  } else {
+    N = *qsize_nxt;
+    c->bytes += 12.0 * c->nf + 8.0 * E + 8.0 * N;
+  }
+  c->examined += E;
+  c->prev_nf = c->nf;
+  c->nf = N;
+  c->mf = (long long)stripe_sum(ctr + 1);
+  c->mu -= c->mf;
+  c->reached += N;
+  c->reached_edges += c->mf;
+  c->level++;
+  if (N == 0 || c->level >= max_levels) c->done = 1;
 }
 
 }  // namespace
@@ -237,31 +328,101 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
     GG_HIP(hipGetLastError());
   }
   g->prof.add_bytes("bfs.init", 4.0 * n + words * 4.0);
-  GG_TRY(read_counters(g, 16));
-  int64_t mf = g->h64()[1];
-  int64_t mu = g->m - mf;
-  int64_t nf = 1, prev_nf = 0;
-  int64_t reached = 1, reached_edges = mf, examined_total = 0;
+  int64_t mf = 0, mu = 0, nf = 1, prev_nf = 0;
+  int64_t reached = 1, reached_edges = 0, examined_total = 0;
   double bytes = 4.0 * n + words * 4.0;
   const int alpha = g->opt.bfs_alpha, beta = g->opt.bfs_beta;
   bool bottom_up = false;
   int cur = 0;  // index of the current queue / frontier bitmap
   int levels = 0, pull_levels = 0;
+  if (!g->prof.enabled) {
+    // Device-driven: kLoopLevels levels per replay of a captured launch
+    // sequence (an even count, so level parity and the ping-pong buffers
+    // agree across replays); per level k_bfs_ctrl_pre picks the direction,
+    // the conversions, bottom-up kernel and LBS top-down (over
+    // ctl->push_size) run or skip on its flags, and k_bfs_ctrl_post folds
+    // the counters.  Per-kernel events (GG_INSTRUMENT) need the host loop.
+    GG_TRY(g->loop_ctl.ensure(sizeof(BfsCtl)));
+    BfsCtl *ctl = g->loop_ctl.as<BfsCtl>();
+    hipLaunchKernelGGL(k_bfs_ctl_init, dim3(1), dim3(1), 0, s, ctl, ctr, g->m, bytes);
+    GG_HIP(hipGetLastError());
+    const std::vector<uint64_t> key = {
+        (uint64_t)depth, (uint64_t)visited, (uint64_t)fbits[0], (uint64_t)fbits[1], (uint64_t)queue[0],
+        (uint64_t)queue[1], (uint64_t)ctr, (uint64_t)ctl, (uint64_t)lb.scan_local, (uint64_t)lb.partials,
+        (uint64_t)lb.info, (uint64_t)lb.parts, (uint64_t)lb.edge_counter, (uint64_t)lb.expand_grid, (uint64_t)lb.vt,
+        (uint64_t)alpha, (uint64_t)beta, (uint64_t)g->opt.wave_degree, (uint64_t)g->opt.iter_num};
+    if (!g->bfs_loop.exec || g->bfs_loop.key != key) {
+      g->bfs_loop.reset();
+      hipGraph_t graph = nullptr;
+      GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      for (int lv = 0; lv < kLoopLevels; lv++) {
+        const int c = lv & 1, x = c ^ 1;
+        hipLaunchKernelGGL(k_bfs_ctrl_pre, dim3(1), dim3(1), 0, s, ctl, n, alpha, beta, qsz[c]);
+        hipLaunchKernelGGL(k_bits_clear_if, dim3(grid_for(words, 256, 4096)), dim3(256), 0, s, fbits[c], words,
+                           (const int32_t *)&ctl->q2b);
+        hipLaunchKernelGGL(k_queue_to_bits, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, queue[c], qsz[c], fbits[c],
+                           (const int32_t *)&ctl->q2b);
+        hipLaunchKernelGGL(k_bits_to_queue, dim3(grid_for(words, 256, 4096)), dim3(256), 0, s, fbits[c], words,
+                           queue[c], qsz[c], (const int32_t *)&ctl->b2q);
+        (void)slot_resets_async(ctr, {{0, 0ull}, {1, 0ull}, {2, 0ull}, {3, 0ull}, {8 + x, 0ull}}, s);
+        hipLaunchKernelGGL(k_bfs_bottom_up, dim3(grid_for((int64_t)words * 32, 256, 2048)), dim3(256), 0, s,
+                           g->dg.column_offsets, g->dg.row_indices, g->directed ? g->dg.row_offsets : nullptr, depth,
+                           visited, fbits[c], fbits[x], n, 0, g->opt.wave_degree, ctr, (const int32_t *)&ctl->bu_skip,
+                           (const int32_t *)&ctl->level);
+        BfsPushOp op{g->dg.row_offsets, g->dg.column_indices, depth, visited, ctr, 0, &ctl->level, 0ull};
+        lbs_launch(op, false, queue[c], &ctl->push_size, n, (int64_t)n + g->m, lb, queue[x], qsz[x], s, [](int) {});
+        hipLaunchKernelGGL(k_bfs_ctrl_post, dim3(1), dim3(1), 0, s, ctl, ctr, qsz[x], n, g->opt.iter_num);
+      }
+      const hipError_t le = hipGetLastError();
+      const hipError_t ce = hipStreamEndCapture(s, &graph);
+      hipError_t ie = hipErrorUnknown;
+      if (le == hipSuccess && ce == hipSuccess) ie = hipGraphInstantiate(&g->bfs_loop.exec, graph, nullptr, nullptr, 0);
+      if (graph) (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        g->bfs_loop.exec = nullptr;
+        return set_error(GG_ERR_HIP, "BFS loop capture failed: %s / %s / %s", hipGetErrorString(le),
+                         hipGetErrorString(ce), hipGetErrorString(ie));
+      }
+      g->bfs_loop.key = key;
+    }
+    for (;;) {
+      GG_HIP(hipGraphLaunch(g->bfs_loop.exec, s));
+      GG_HIP(hipMemcpyAsync(g->h_pinned, ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, s));
+      GG_HIP(hipStreamSynchronize(s));
+      BfsCtl h;
+      std::memcpy(&h, g->h_pinned, sizeof(h));
+      if (h.done) {
+        levels = h.level;
+        pull_levels = h.pull_levels;
+        reached = h.reached;
+        reached_edges = h.reached_edges;
+        examined_total = h.examined;
+        bytes = h.bytes;
+        nf = 0;
+        break;
+      }
+    }
+  } else {
+    GG_TRY(read_counters(g, 16));
+    mf = g->h64()[1];
+    mu = g->m - mf;
+    reached_edges = mf;
+  }
 
-  while (nf > 0 && levels < g->opt.iter_num) {
+  while (g->prof.enabled && nf > 0 && levels < g->opt.iter_num) {
     // Direction choice.
     if (!bottom_up && mf > mu / alpha && nf > prev_nf) {
       TimedRegion t(g->prof, "bfs.queue_to_bits");
       GG_HIP(fill_async<uint32_t>(fbits[cur], 0u, words, s));
       hipLaunchKernelGGL(k_queue_to_bits, dim3(grid_for(nf, 256, 4096)), dim3(256), 0, s, queue[cur], qsz[cur],
-                         fbits[cur]);
+                         fbits[cur], (const int32_t *)nullptr);
       GG_HIP(hipGetLastError());
       bottom_up = true;
     } else if (bottom_up && nf < n / beta && nf < prev_nf) {
       TimedRegion t(g->prof, "bfs.bits_to_queue");
       GG_HIP(slot_resets_async(ctr, {{8 + cur, 0ull}}, s));
       hipLaunchKernelGGL(k_bits_to_queue, dim3(grid_for(words, 256, 4096)), dim3(256), 0, s, fbits[cur], words,
-                         queue[cur], qsz[cur]);
+                         queue[cur], qsz[cur], (const int32_t *)nullptr);
       GG_HIP(hipGetLastError());
       bottom_up = false;
     }
@@ -274,7 +435,8 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
         hipLaunchKernelGGL(k_bfs_bottom_up, dim3(grid_for((int64_t)words * 32, 256, 2048)), dim3(256), 0, s,
                            g->dg.column_offsets, g->dg.row_indices, g->directed ? g->dg.row_offsets : nullptr,
                            depth, visited, fbits[cur],
-                           fbits[nxt], n, levels + 1, g->opt.wave_degree, ctr);
+                           fbits[nxt], n, levels + 1, g->opt.wave_degree, ctr, (const int32_t *)nullptr,
+                           (const int32_t *)nullptr);
         GG_HIP(hipGetLastError());
       }
       GG_TRY(read_counters(g, 4));
@@ -286,7 +448,7 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
       nf = N;
       pull_levels++;
     } else {
-      BfsPushOp op{g->dg.row_offsets, g->dg.column_indices, depth, visited, ctr, levels + 1, 0ull};
+      BfsPushOp op{g->dg.row_offsets, g->dg.column_indices, depth, visited, ctr, levels + 1, nullptr, 0ull};
       lbs_launch(op, false, queue[cur], qsz[cur], nf, (int64_t)n + g->m, lb, queue[nxt], qsz[nxt], s,
                  lbs_timer);
       GG_HIP(hipGetLastError());
