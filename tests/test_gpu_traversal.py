@@ -156,6 +156,53 @@ def test_device_outputs_and_instrumentation(gpu_available):
         assert all(v["launches"] > 0 and v["total_ms"] >= 0 for v in prof.values())
 
 
+@pytest.mark.parametrize("undirected", [True, False])
+def test_device_driven_loops_match_host_loops(gpu_available, undirected):
+    """BFS and SSSP run their iteration loops on the device (captured
+    graphs) unless GG_INSTRUMENT asks for per-kernel events, which uses the
+    host-driven loops.  Both must give the same results and statistics,
+    through option changes and output buffers that force a new capture, and
+    under an iteration cap."""
+    import torch
+
+    c = gg.gen_rmat(15, 16, undirected=undirected, seed=26).with_weights("i32", 1, 64, symmetric=undirected, seed=27)
+    src = workloads.max_degree_vertex(c)
+    exp_b = oracle.bfs(c.row_offsets, c.col_indices, src)
+    exp_s = oracle.sssp(c.row_offsets, c.col_indices, c.values, src)
+    # BFS statistics are fixed by the graph; an SSSP pull reads distances
+    # other workgroups lower during the same pass, so its iteration count
+    # (not its result) depends on timing once pulls are on.
+    keys = {"bfs": ("iterations", "pull_iterations", "reached_vertices", "reached_edges", "edges_inspected"),
+            "sssp": ("reached_vertices", "reached_edges")}
+    outs = [torch.empty(c.n, dtype=torch.int32, device="cuda") for _ in range(2)]
+    with gg.Graph(c, directed=not undirected, reorder=undirected) as g:
+        for opts in ({}, {"pull_alpha": 2, "sssp_delta": 64}, {"bfs_alpha": 1, "bfs_beta": 1000000, "lbs_tile": 512},
+                     {"pull_alpha": 1, "expand_grid": 64}):
+            g.set_options(**gg.default_options().as_dict())
+            g.set_options(**opts)
+            for out in outs:
+                for prog, exp in (("bfs", exp_b), ("sssp", exp_s)):
+                    stats = {}
+                    for fl in (_abi.GG_OUT_DEVICE, _abi.GG_OUT_DEVICE | _abi.GG_INSTRUMENT):
+                        _, st = getattr(g, prog)(src, flags=fl, out=out.data_ptr())
+                        g.synchronize()
+                        np.testing.assert_array_equal(out.cpu().numpy(), exp, err_msg=str((prog, opts, fl)))
+                        stats[fl] = {k: st[k] for k in keys[prog]}
+                        if prog == "sssp" and g.options().pull_alpha == 0:
+                            stats[fl].update(iterations=st["iterations"], edges_inspected=st["edges_inspected"])
+                    assert stats[_abi.GG_OUT_DEVICE] == stats[_abi.GG_OUT_DEVICE | _abi.GG_INSTRUMENT], (prog, opts)
+        # iteration cap: both loops stop after the same number of levels / iterations
+        g.set_options(**gg.default_options().as_dict())
+        g.set_options(iter_num=2)
+        for prog in ("bfs", "sssp"):
+            res = []
+            for fl in (0, _abi.GG_INSTRUMENT):
+                d, st = getattr(g, prog)(src, flags=fl)
+                res.append((d, st["iterations"]))
+            np.testing.assert_array_equal(res[0][0], res[1][0])
+            assert res[0][1] == res[1][1] == 2, prog
+
+
 def test_error_paths(gpu_available):
     c = workloads.config1_er()
     with gg.Graph(c) as g:
