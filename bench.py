@@ -42,6 +42,31 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+# Kernel classes of the bench's HIP-event profile -> rows of the committed
+# rocprofv3 --pmc summary (FETCH_SIZE x2 + WRITE_SIZE x1, calibrated by a
+# 1 GiB copy), for `roofline.traffic`.
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "headline_rmat22_hbm_traffic_pmc.md")
+TRAFFIC_ROWS = {"sssp.pull": "k_pull_staged<PullOp, int>", "sssp.push": "k_lbs_expand<PushOp, 4>",
+                "bfs.bottom_up": "k_bfs_bottom_up", "bfs.top_down": "k_lbs_expand<BfsPushOp, 4>"}
+
+
+def measured_traffic(kernel_class):
+    """HBM bytes per launch of a kernel class from the committed counter
+    summary (fetch + write per working dispatch), or None."""
+    row = TRAFFIC_ROWS.get(kernel_class)
+    if row is None or not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    for line in open(TRAFFIC_SUMMARY):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0].startswith(row):
+            try:
+                return {"bytes": (float(cells[2]) + float(cells[3])) * 1e6,
+                        "source": "profiles/" + os.path.basename(TRAFFIC_SUMMARY)}
+            except ValueError:
+                return None
+    return None
+
+
 def median_ms(fn, reps):
     import numpy as np
 
@@ -69,7 +94,12 @@ def headline(args, gg, _abi, workloads, torch, np):
         f"{'' if args.no_reorder else ' + renumbering by degree'})")
     flags = _abi.GG_OUT_DEVICE
     tuned = None
-    if not args.no_tune:
+    if args.options:
+        fixed = json.loads(args.options)
+        g.set_options(**fixed)
+        tuned = {"fixed": fixed}
+        log(f"options: {fixed}")
+    elif not args.no_tune:
         t0 = time.perf_counter()
         best, trials = g.autotune(programs=_abi.GG_TUNE_BFS | _abi.GG_TUNE_SSSP, src=src, reps=3)
         defaults = gg.default_options().as_dict()
@@ -126,9 +156,14 @@ def headline(args, gg, _abi, workloads, torch, np):
         "bound": "hbm", "kernel": name, "launches": ent["launches"],
         "avg_launch_us": ent["total_ms"] * 1e3 / max(ent["launches"], 1),
         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None, "pass": f"second timed pass of {args.steps} steps with per-kernel HIP events "
+        "traffic": None, "traffic_source": None,
+        "pass": f"second timed pass of {args.steps} steps with per-kernel HIP events "
                                    f"({wall_instrumented * 1e3:.1f} ms vs {wall * 1e3:.1f} ms without)",
     }
+    tr = measured_traffic(name)
+    if tr is not None:
+        roofline["traffic"] = tr["bytes"]
+        roofline["traffic_source"] = tr["source"]
     # Frontier-heavy launches: those expanding >= 1% of m edges (by each
     # kernel's per-edge byte model: push 12 B, top-down 8 B, bottom-up 4 B).
     per_edge = {"sssp.push": 12.0, "bfs.top_down": 8.0, "bfs.bottom_up": 4.0}
@@ -183,10 +218,11 @@ def headline(args, gg, _abi, workloads, torch, np):
     config = {"workload": f"rmat{args.scale}_undirected_bfs+sssp", "n": csr.n, "m_csr": csr.m, "edge_factor": 16,
               "weights": "int32 uniform [1,64]", "source": src, "step": "1 BFS + 1 SSSP",
               "teps_counting": "sum of CSR degree over reached vertices",
-              "options": "autotuned (gg_autotune, BFS + SSSP) before timing" if tuned else "defaults",
+              "options": ("fixed " + json.dumps(tuned["fixed"]) if tuned and "fixed" in tuned
+                          else "autotuned (gg_autotune, BFS + SSSP) before timing" if tuned else "defaults"),
               "vertex_order": "caller ids, stored renumbered by degree (GG_GRAPH_REORDER)" if not args.no_reorder
               else "as generated"}
-    if tuned is not None:
+    if tuned is not None and "fixed" not in tuned:
         per_alg["autotune"] = tuned
     # CC on the same graph (label propagation and hooking), outside the timed steps.
     if args.no_extras:
@@ -313,6 +349,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
     ap.add_argument("--no-tune", action="store_true", help="time the default options (no gg_autotune)")
+    ap.add_argument("--options", default=None,
+                    help='fixed options as JSON, e.g. \'{"pull_alpha": 2}\' (no gg_autotune; for profiling runs)')
     ap.add_argument("--no-reorder", action="store_true", help="keep the R-MAT graphs in generated vertex order")
     ap.add_argument("--full-checks", action="store_true", help="FP64 PageRank error at full size")
     ap.add_argument("--copy-probe", action="store_true", help="run only a 1 GiB device copy (PMC calibration)")
