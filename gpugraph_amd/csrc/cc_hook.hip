@@ -66,9 +66,25 @@ struct HookOp {
   __device__ __forceinline__ void finish() {}
 };
 
+// Root without path halving: follows parents only.
+__device__ __forceinline__ int32_t walk_to_root(const int32_t *parent, int32_t x) {
+  for (;;) {
+    const int32_t p = __atomic_load_n(&parent[x], __ATOMIC_RELAXED);
+    if (p == x) return x;
+    x = p;
+  }
+}
+
+// Final labels: parent[v] = root of v.  Each parent[v] is written by v's
+// thread only, with the root, and read by other threads' walks, which accept
+// an old ancestor or the root alike.  (With path halving here, a walk that
+// read parent[v] before v's thread stored the root could store an older
+// ancestor after it, leaving a non-root label.)
 __global__ void k_cc_compress(int32_t *parent, int32_t n) {
-  for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-    parent[v] = find_root(parent, v);
+  for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const int32_t r = walk_to_root(parent, v);
+    __atomic_store_n(&parent[v], r, __ATOMIC_RELAXED);
+  }
 }
 
 }  // namespace
