@@ -201,22 +201,55 @@ k_bits_to_queue(const uint32_t *__restrict__ bits, int32_t words, int32_t *__res
   }
 }
 
-__global__ void k_bits_clear_if(uint32_t *__restrict__ bits, int32_t words, const int32_t *__restrict__ enable) {
-  if (!*enable) return;
-  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (int64_t)gridDim.x * blockDim.x)
-    bits[w] = 0u;
+// Conversions between the two frontier forms, one launch for both (the
+// controller enables at most one).  Queue -> bitmap is rebuilt from the
+// depths (vertices at depth == level form the current frontier): whole
+// words written by a ballot, so no clearing pass and no atomics.
+__global__ void __launch_bounds__(256)
+k_bfs_convert(const int32_t *__restrict__ depth, const int32_t *__restrict__ level, const int32_t *__restrict__ q2b,
+              const int32_t *__restrict__ b2q, int32_t n, uint32_t *__restrict__ bits, int32_t *__restrict__ queue,
+              int32_t *__restrict__ qsize) {
+  const int32_t words = (n + 31) >> 5;
+  if (*q2b) {
+    const int32_t lv = *level;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int lane = lane_id();
+    for (int64_t p = (((int64_t)blockIdx.x * blockDim.x) + threadIdx.x) >> 6; p * 64 < n; p += nwaves) {
+      const int64_t v = p * 64 + lane;
+      const uint64_t mask = ballot(v < n && depth[v] == lv);
+      const int64_t w = 2 * p + lane;
+      if (lane < 2 && w < words) bits[w] = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
+    }
+  } else if (*b2q) {
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < words; base += (int64_t)gridDim.x * blockDim.x) {
+      const int64_t w = base + threadIdx.x;
+      uint32_t word = w < words ? bits[w] : 0u;
+      const int32_t cnt = __popc(word);
+      const int32_t incl = wave_inclusive_scan(cnt);
+      const int32_t total = __shfl(incl, kWave - 1);
+      int32_t start = 0;
+      if (lane_id() == 0 && total) start = atomicAdd(qsize, total);
+      start = __shfl(start, 0) + incl - cnt;
+      while (word) {
+        const int bb = __builtin_ctz(word);
+        word &= word - 1;
+        queue[start++] = (int32_t)(w * 32 + bb);
+      }
+    }
+  }
 }
 
-// State of the device-driven loop: k_bfs_ctrl_pre / k_bfs_ctrl_post take
-// the host loop's per-level decisions (direction switches, end) and keep its
-// statistics.
+// State of the device-driven loop.  One controller launch per level closes
+// the previous level (folds its counters, the host loop's bookkeeping),
+// clears the counters for the new one and takes its direction decision; a
+// last launch at the end of the captured sequence only closes.
 struct BfsCtl {
   int32_t level;      // levels done; the next level writes depth level + 1
-  int32_t bottom_up;  // direction of the current level
+  int32_t bottom_up;  // direction of the level in flight
   int32_t q2b, b2q;   // conversions before this level
   int32_t bu_skip;    // 1 unless this level runs bottom-up
   int32_t push_size;  // frontier items of a top-down level, else 0
-  int32_t done, pull_levels;
+  int32_t done, pull_levels, pending;
   long long nf, prev_nf, mf, mu, reached, reached_edges, examined;
   double bytes;
 };
@@ -231,6 +264,7 @@ __global__ void k_bfs_ctl_init(BfsCtl *c, const unsigned long long *__restrict__
   c->push_size = 0;
   c->done = 0;
   c->pull_levels = 0;
+  c->pending = 0;
   c->nf = 1;
   c->prev_nf = 0;
   c->mf = mf;
@@ -241,47 +275,64 @@ __global__ void k_bfs_ctl_init(BfsCtl *c, const unsigned long long *__restrict__
   c->bytes = bytes0;
 }
 
-__global__ void k_bfs_ctrl_pre(BfsCtl *c, int32_t n, int32_t alpha, int32_t beta, int32_t *__restrict__ qsize_cur) {
-  c->q2b = c->b2q = 0;
-  c->bu_skip = 1;
-  c->push_size = 0;
-  if (c->done) return;
-  if (!c->bottom_up && c->mf > c->mu / alpha && c->nf > c->prev_nf) {
-    c->q2b = 1;
-    c->bottom_up = 1;
-  } else if (c->bottom_up && c->nf < n / beta && c->nf < c->prev_nf) {
-    c->b2q = 1;
-    c->bottom_up = 0;
-    *qsize_cur = 0;  // k_bits_to_queue appends
+// qsize_cur: the queue the level in flight wrote (its next queue) and the
+// new level reads; qsize_nxt: the new level's output queue.
+__global__ void __launch_bounds__(64)
+k_bfs_ctrl(BfsCtl *c, unsigned long long *__restrict__ ctr, int32_t *__restrict__ qsize_cur,
+           int32_t *__restrict__ qsize_nxt, int32_t n, int32_t alpha, int32_t beta, int32_t max_levels,
+           int close_only) {
+  __shared__ int s_go;
+  if (threadIdx.x == 0) {
+    if (c->pending && !c->done) {  // close the level in flight
+      const long long E = (long long)stripe_sum(ctr + 2);
+      long long N;
+      if (c->bottom_up) {
+        N = (long long)stripe_sum(ctr + 0);
+        c->bytes += 8.0 * (double)stripe_sum(ctr + 3) + 4.0 * E + n / 8.0 + 8.0 * N;
+        c->pull_levels++;
+      } else {
+        N = *qsize_cur;
+        c->bytes += 12.0 * c->nf + 8.0 * E + 8.0 * N;
+      }
+      c->examined += E;
+      c->prev_nf = c->nf;
+      c->nf = N;
+      c->mf = (long long)stripe_sum(ctr + 1);
+      c->mu -= c->mf;
+      c->reached += N;
+      c->reached_edges += c->mf;
+      c->level++;
+      if (N == 0 || c->level >= max_levels) c->done = 1;
+    }
+    c->pending = 0;
+    c->q2b = c->b2q = 0;
+    c->bu_skip = 1;
+    c->push_size = 0;
+    s_go = !close_only && !c->done;
   }
-  if (c->bottom_up)
-    c->bu_skip = 0;
-  else
-    c->push_size = (int32_t)c->nf;
-}
-
-__global__ void k_bfs_ctrl_post(BfsCtl *c, const unsigned long long *__restrict__ ctr,
-                                const int32_t *__restrict__ qsize_nxt, int32_t n, int32_t max_levels) {
-  if (c->done) return;
-  const long long E = (long long)stripe_sum(ctr + 2);
-  long long N;
-  if (c->bottom_up) {
-    N = (long long)stripe_sum(ctr + 0);
-    c->bytes += 8.0 * (double)stripe_sum(ctr + 3) + 4.0 * E + n / 8.0 + 8.0 * N;
-    c->pull_levels++;
-  } else {
-    N = *qsize_nxt;
-    c->bytes += 12.0 * c->nf + 8.0 * E + 8.0 * N;
+  __syncthreads();
+  if (!s_go) return;
+  // counters of the new level: [0..3] in every stripe, and its output queue
+  {
+    const int q = threadIdx.x;  // 64 threads: 16 stripes x 4 counters
+    ctr[(q >> 2) * kCounterSlots + (q & 3)] = 0ull;
   }
-  c->examined += E;
-  c->prev_nf = c->nf;
-  c->nf = N;
-  c->mf = (long long)stripe_sum(ctr + 1);
-  c->mu -= c->mf;
-  c->reached += N;
-  c->reached_edges += c->mf;
-  c->level++;
-  if (N == 0 || c->level >= max_levels) c->done = 1;
+  if (threadIdx.x == 0) {
+    *qsize_nxt = 0;
+    if (!c->bottom_up && c->mf > c->mu / alpha && c->nf > c->prev_nf) {
+      c->q2b = 1;
+      c->bottom_up = 1;
+    } else if (c->bottom_up && c->nf < n / beta && c->nf < c->prev_nf) {
+      c->b2q = 1;
+      c->bottom_up = 0;
+      *qsize_cur = 0;  // k_bfs_convert appends
+    }
+    if (c->bottom_up)
+      c->bu_skip = 0;
+    else
+      c->push_size = (int32_t)c->nf;
+    c->pending = 1;
+  }
 }
 
 }  // namespace
@@ -338,10 +389,11 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
   if (!g->prof.enabled) {
     // Device-driven: kLoopLevels levels per replay of a captured launch
     // sequence (an even count, so level parity and the ping-pong buffers
-    // agree across replays); per level k_bfs_ctrl_pre picks the direction,
-    // the conversions, bottom-up kernel and LBS top-down (over
-    // ctl->push_size) run or skip on its flags, and k_bfs_ctrl_post folds
-    // the counters.  Per-kernel events (GG_INSTRUMENT) need the host loop.
+    // agree across replays).  Per level: k_bfs_ctrl (closes the previous
+    // level, picks the direction), k_bfs_convert, the bottom-up kernel and
+    // the LBS top-down (over ctl->push_size), each running or returning on
+    // the controller's flags; a last k_bfs_ctrl closes the replay's final
+    // level.  Per-kernel events (GG_INSTRUMENT) need the host loop below.
     GG_TRY(g->loop_ctl.ensure(sizeof(BfsCtl)));
     BfsCtl *ctl = g->loop_ctl.as<BfsCtl>();
     hipLaunchKernelGGL(k_bfs_ctl_init, dim3(1), dim3(1), 0, s, ctl, ctr, g->m, bytes);
@@ -355,24 +407,23 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
       g->bfs_loop.reset();
       hipGraph_t graph = nullptr;
       GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const int conv_grid = grid_for((int64_t)n, 256, 4096);
       for (int lv = 0; lv < kLoopLevels; lv++) {
         const int c = lv & 1, x = c ^ 1;
-        hipLaunchKernelGGL(k_bfs_ctrl_pre, dim3(1), dim3(1), 0, s, ctl, n, alpha, beta, qsz[c]);
-        hipLaunchKernelGGL(k_bits_clear_if, dim3(grid_for(words, 256, 4096)), dim3(256), 0, s, fbits[c], words,
-                           (const int32_t *)&ctl->q2b);
-        hipLaunchKernelGGL(k_queue_to_bits, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, queue[c], qsz[c], fbits[c],
-                           (const int32_t *)&ctl->q2b);
-        hipLaunchKernelGGL(k_bits_to_queue, dim3(grid_for(words, 256, 4096)), dim3(256), 0, s, fbits[c], words,
-                           queue[c], qsz[c], (const int32_t *)&ctl->b2q);
-        (void)slot_resets_async(ctr, {{0, 0ull}, {1, 0ull}, {2, 0ull}, {3, 0ull}, {8 + x, 0ull}}, s);
+        hipLaunchKernelGGL(k_bfs_ctrl, dim3(1), dim3(64), 0, s, ctl, ctr, qsz[c], qsz[x], n, alpha, beta,
+                           g->opt.iter_num, 0);
+        hipLaunchKernelGGL(k_bfs_convert, dim3(conv_grid), dim3(256), 0, s, depth, (const int32_t *)&ctl->level,
+                           (const int32_t *)&ctl->q2b, (const int32_t *)&ctl->b2q, n, fbits[c], queue[c], qsz[c]);
         hipLaunchKernelGGL(k_bfs_bottom_up, dim3(grid_for((int64_t)words * 32, 256, 2048)), dim3(256), 0, s,
                            g->dg.column_offsets, g->dg.row_indices, g->directed ? g->dg.row_offsets : nullptr, depth,
                            visited, fbits[c], fbits[x], n, 0, g->opt.wave_degree, ctr, (const int32_t *)&ctl->bu_skip,
                            (const int32_t *)&ctl->level);
         BfsPushOp op{g->dg.row_offsets, g->dg.column_indices, depth, visited, ctr, 0, &ctl->level, 0ull};
         lbs_launch(op, false, queue[c], &ctl->push_size, n, (int64_t)n + g->m, lb, queue[x], qsz[x], s, [](int) {});
-        hipLaunchKernelGGL(k_bfs_ctrl_post, dim3(1), dim3(1), 0, s, ctl, ctr, qsz[x], n, g->opt.iter_num);
       }
+      // close the last level of the replay (kLoopLevels is even: its output queue is qsz[0])
+      hipLaunchKernelGGL(k_bfs_ctrl, dim3(1), dim3(64), 0, s, ctl, ctr, qsz[0], qsz[1], n, alpha, beta,
+                         g->opt.iter_num, 1);
       const hipError_t le = hipGetLastError();
       const hipError_t ce = hipStreamEndCapture(s, &graph);
       hipError_t ie = hipErrorUnknown;

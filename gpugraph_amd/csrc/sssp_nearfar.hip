@@ -143,8 +143,20 @@ __global__ void k_sssp_init(int32_t *dist, uint8_t *dirty, int32_t src) {
   dirty[src] = 1;
 }
 
+// Clears the collect's counters (queue sizes, dirty left, min dist left in
+// every stripe, frontier degree sum) for the next collect.
+__device__ __forceinline__ void sssp_reset_collect(unsigned long long *ctr) {
+  ctr[8] = 0ull;  // both int32 queue sizes
+  for (int q = 0; q < kStripes; q++) {
+    ctr[q * kCounterSlots + 16] = 0ull;
+    ctr[q * kCounterSlots + 17] = (unsigned long long)INT_MAX;
+    ctr[q * kCounterSlots + 18] = 0ull;
+  }
+}
+
 // State of the device-driven loop: k_sssp_ctrl takes, after each collect,
-// the decisions the host loop takes (bucket advance, push or pull, end).
+// the decisions the host loop takes (bucket advance, push or pull, end),
+// after copying what it needs out of the counters and clearing them.
 struct SsspCtl {
   int32_t bound;      // frontier bound of the current bucket
   int32_t push_size;  // frontier items the LBS push sees: the queue size when pushing, else 0
@@ -155,7 +167,8 @@ struct SsspCtl {
   double bytes;       // algorithmic bytes (model in the header comment)
 };
 
-__global__ void k_sssp_ctl_init(SsspCtl *c, int32_t bound, double bytes0) {
+__global__ void k_sssp_ctl_init(SsspCtl *c, unsigned long long *ctr, int32_t bound, double bytes0) {
+  sssp_reset_collect(ctr);
   c->bound = bound;
   c->push_size = 0;
   c->pull_skip = 1;
@@ -166,8 +179,8 @@ __global__ void k_sssp_ctl_init(SsspCtl *c, int32_t bound, double bytes0) {
   c->bytes = bytes0;
 }
 
-__global__ void k_sssp_ctrl(SsspCtl *c, const unsigned long long *__restrict__ ctr, int32_t n, int64_t m,
-                            int64_t delta, int64_t pull_alpha, int32_t max_iter) {
+__global__ void k_sssp_ctrl(SsspCtl *c, unsigned long long *__restrict__ ctr, int32_t n, int64_t m, int64_t delta,
+                            int64_t pull_alpha, int32_t max_iter) {
   c->push_size = 0;
   c->pull_skip = 1;
   if (c->done) return;
@@ -179,6 +192,7 @@ __global__ void k_sssp_ctrl(SsspCtl *c, const unsigned long long *__restrict__ c
   const unsigned long long remain = stripe_sum(ctr + 16), degsum = stripe_sum(ctr + 18);
   unsigned long long dmin = ctr[17];
   for (int q = 1; q < kStripes; q++) dmin = min(dmin, ctr[17 + q * kCounterSlots]);
+  sssp_reset_collect(ctr);
   c->bytes += n + 4.0 * (double)(nf + (int64_t)remain) + 4.0 * nf;
   if (nf == 0) {
     if (remain == 0)
@@ -323,12 +337,14 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
                  dirty, ctr, {}};
   if (!g->prof.enabled) {
     // Device-driven: kLoopIters iterations per replay of a captured launch
-    // sequence (reset, collect, k_sssp_ctrl, LBS push over ctl->push_size,
-    // pull unless ctl->pull_skip); the host reads the loop state once per
-    // replay.  Per-kernel events (GG_INSTRUMENT) need the host loop below.
+    // sequence (collect, k_sssp_ctrl (which also resets the collect's
+    // counters), LBS push over ctl->push_size, pull unless ctl->pull_skip);
+    // the host reads the loop state once per replay.  Per-kernel events
+    // (GG_INSTRUMENT) need the host loop below.
     GG_TRY(g->loop_ctl.ensure(sizeof(SsspCtl)));
     SsspCtl *ctl = g->loop_ctl.as<SsspCtl>();
-    hipLaunchKernelGGL(k_sssp_ctl_init, dim3(1), dim3(1), 0, s, ctl, (int32_t)std::min<int64_t>(delta, kInf), bytes);
+    hipLaunchKernelGGL(k_sssp_ctl_init, dim3(1), dim3(1), 0, s, ctl, ctr, (int32_t)std::min<int64_t>(delta, kInf),
+                       bytes);
     GG_HIP(hipGetLastError());
     const std::vector<uint64_t> key = {
         (uint64_t)dist, (uint64_t)dirty, (uint64_t)queue, (uint64_t)ctr, (uint64_t)ctl, (uint64_t)wcv,
@@ -340,8 +356,6 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       hipGraph_t graph = nullptr;
       GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       for (int it = 0; it < kLoopIters; it++) {
-        (void)slot_resets_async(ctr, {{8, 0ull}, {9, 0ull}, {16, 0ull}, {17, (unsigned long long)INT_MAX}, {18, 0ull}},
-                                s);
         hipLaunchKernelGGL(k_sssp_collect, dim3(collect_grid), dim3(256), 0, s, dirty, dist, g->dg.row_offsets, n, 0,
                            queue, qsize, ctr, (const SsspCtl *)ctl);
         hipLaunchKernelGGL(k_sssp_ctrl, dim3(1), dim3(1), 0, s, ctl, ctr, n, m, delta, pull_alpha, g->opt.iter_num);
