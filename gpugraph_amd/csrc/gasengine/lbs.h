@@ -508,13 +508,20 @@ k_pull_reduce(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
 //
 // k_pull_reduce waits on the tile's offsets, then its edges' loads, then
 // their gathers, then each vertex's current state.  Here every load that
-// does not depend on a gather is issued at once at the start of a tile (the
-// tile's edges are the contiguous slots [j0, j1), known from the partition
-// alone): the row offsets, each thread's VT edges (Edge loads) and the
-// current state of the tile's vertices; the next tile's partition bounds are
-// fetched a tile ahead.  The merge-path slot map is then built in LDS, the
-// VT gathers go out together, and the results are combined and handed to
-// item_done with the prefetched state.
+// does not depend on a gather is issued at once (stage 1; the tile's edges
+// are the contiguous slots [j0, j1), known from the partition alone): the
+// row offsets, each thread's VT edges (Edge loads) and the current state of
+// the tile's vertices.  The merge-path slot map is then built in LDS, the VT
+// gathers go out together (stage 2), and the results are combined and
+// handed to item_done with the prefetched state.
+//
+// The loop is software-pipelined: a workgroup issues the stage-1 loads of
+// its next tile (and the partition bounds of the one after) before the
+// item_done writes of the current tile.  gfx950 counts loads and stores in
+// one in-order counter, so loads issued after a tile's stores would wait
+// for those stores as well.  The next tile (t + gridDim.x) shares no vertex
+// that the current tile writes with a plain store: such a vertex has all of
+// its in-edges in the current tile.
 //
 // StagedPullOp concept:
 //   using Edge = ...                        per-edge loads (e.g. source and weight)
@@ -541,37 +548,43 @@ k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
   __shared__ int32_t s_slot[TILE];
   const int tid = threadIdx.x;
   const int64_t path = (int64_t)n + m;
-  int64_t t = blockIdx.x;
-  int32_t i0 = 0, i1 = 0;
-  if (t * TILE < path) {
-    i0 = parts[t];
-    i1 = parts[t + 1];
-  }
-  for (; t * TILE < path; t += gridDim.x) {
-    const int64_t d0 = t * TILE;
-    const int64_t d1 = d0 + TILE < path ? d0 + TILE : path;
-    const int32_t j0 = (int32_t)(d0 - i0), j1 = (int32_t)(d1 - i1);
-    const int32_t ni = i1 - i0, ne = j1 - j0;
-    const int64_t tn = t + gridDim.x;
-    int32_t n0 = 0, n1 = 0;
-    if (tn * TILE < path) {  // next tile's bounds, a tile ahead
-      n0 = parts[tn];
-      n1 = parts[tn + 1];
-    }
-    // Stage 1: edges, offsets and vertex state, all independent.
-    typename Op::Edge ed[VT];
-    if (ne > 0) {
+  const int64_t G = gridDim.x;
+  typename Op::Edge ed[VT];
+  int32_t ov[Q], sv[Q];
+  // Stage 1 of tile tt with partition bounds p0, p1 into ed / ov / sv.
+  auto stage1 = [&](int64_t tt, int32_t p0, int32_t p1) {
+    const int64_t e0 = tt * TILE;
+    const int64_t e1 = e0 + TILE < path ? e0 + TILE : path;
+    const int32_t jj0 = (int32_t)(e0 - p0), nne = (int32_t)(e1 - p1) - jj0, nni = p1 - p0;
+    if (nne > 0) {
 #pragma unroll
-      for (int r = 0; r < VT; r++) ed[r] = op.load_edge(j0 + min(tid + r * kExpandBlock, ne - 1));
+      for (int r = 0; r < VT; r++) ed[r] = op.load_edge(jj0 + min(tid + r * kExpandBlock, nne - 1));
     }
-    int32_t ov[Q], sv[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-      const int32_t k = min(tid + q * kExpandBlock, ni + 1);
-      const int32_t i = i0 - 1 + k;  // <= i1 <= n: offs has n + 1 entries
+      const int32_t k = min(tid + q * kExpandBlock, nni + 1);
+      const int32_t i = p0 - 1 + k;  // <= p1 <= n: offs has n + 1 entries
       ov[q] = i < 0 ? 0 : offs[i];
-      sv[q] = op.vertex_state(i < 0 ? 0 : i < n ? i : n - 1);
+      sv[q] = (i < 0 || i >= n) ? 0 : op.vertex_state(i);
     }
+  };
+  int64_t t = blockIdx.x;
+  int32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;  // partition bounds of tiles t and t + G
+  if (t * TILE < path) {
+    a0 = parts[t];
+    a1 = parts[t + 1];
+    stage1(t, a0, a1);
+  }
+  if ((t + G) * TILE < path) {
+    b0 = parts[t + G];
+    b1 = parts[t + G + 1];
+  }
+  for (; t * TILE < path; t += G) {
+    const int64_t d0 = t * TILE;
+    const int64_t d1 = d0 + TILE < path ? d0 + TILE : path;
+    const int32_t i0 = a0, i1 = a1;
+    const int32_t j0 = (int32_t)(d0 - i0), j1 = (int32_t)(d1 - i1);
+    const int32_t ni = i1 - i0, ne = j1 - j0;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
       const int32_t k = tid + q * kExpandBlock;
@@ -619,6 +632,16 @@ k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
       }
     }
     __syncthreads();
+    // Next tile's stage 1 and the bounds of the one after, ahead of this
+    // tile's writes.
+    int32_t c0 = 0, c1 = 0;
+    if ((t + G) * TILE < path) {
+      stage1(t + G, b0, b1);
+      if ((t + 2 * G) * TILE < path) {
+        c0 = parts[t + 2 * G];
+        c1 = parts[t + 2 * G + 1];
+      }
+    }
     for (int k = tid; k <= ni; k += kExpandBlock) {
       const int32_t v = i0 - 1 + k;
       if (v < 0) continue;
@@ -626,8 +649,10 @@ k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
       op.item_done(v, s_acc[k], whole, s_state[k]);
     }
     __syncthreads();
-    i0 = n0;
-    i1 = n1;
+    a0 = b0;
+    a1 = b1;
+    b0 = c0;
+    b1 = c1;
   }
   op.finish();
 }
