@@ -24,9 +24,21 @@
 // Every fold runs in a fixed order that depends only on the graph and
 // kSegTile, so results are identical run to run and for any grid size.
 //
-// RowOp concept (rowreduce.h): begin(), edge(v, k), item_done(v, sum),
-// finish() (all threads of every workgroup, once per launch).  item_done is
-// called exactly once per vertex, by k_seg_rows or by k_seg_fixup.
+// SegRowOp concept (device, by value):
+//   void  begin()                         per-thread setup at kernel start
+//   using Edge = ...                      per-edge loads (e.g. source, weight)
+//   Edge  load_edge(int32_t k)            loads only
+//   float gather(const Edge &e)           the edge's contribution (dependent loads)
+//   using State = ...                     per-vertex values item_done needs
+//   State vertex_state(int32_t v)         loads only; read-only data during a launch
+//   void  item_done(int32_t v, float sum, const State &st)
+//   void  finish()                        all threads of every workgroup, once
+// item_done is called exactly once per vertex, by k_seg_rows or k_seg_fixup.
+// Per tile, the first stage (row offsets, the tile's Edge loads and its
+// vertices' State) is issued together, a tile ahead: the workgroup loads its
+// next tile's first stage before it writes the current tile's results
+// (gfx950 counts loads and stores in one in-order counter).  The second
+// stage is the VT gathers per thread.
 #pragma once
 
 #include "lbs.h"
@@ -65,9 +77,12 @@ __global__ void __launch_bounds__(kRowBlock)
 k_seg_rows(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const int32_t *__restrict__ parts,
            SegCarry *__restrict__ carry, const int32_t *__restrict__ done) {
   constexpr int TILE = kSegTile, VT = kSegVt, W = kRowBlock / kWave;
+  constexpr int Q = (TILE + 2 + kRowBlock - 1) / kRowBlock;  // per-thread share of the offsets
+  using State = typename Op::State;
   __shared__ int32_t s_offs[TILE + 2];  // s_offs[k] = offs[i0 - 1 + k], k = 0..ni+1
   __shared__ __align__(16) int32_t s_slot[TILE];
   __shared__ __align__(16) float s_val[TILE];
+  __shared__ State s_state[TILE + 1];   // s_state[k]: vertex i0 - 1 + k
   __shared__ float2 s_incl[kRowBlock];
   __shared__ float2 s_wend[W];
   __shared__ int32_t s_whead[W];
@@ -76,15 +91,49 @@ k_seg_rows(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const 
   op.begin();
   const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
   const int64_t path = (int64_t)n + m;
-  for (int64_t t = blockIdx.x; t * TILE < path; t += gridDim.x) {
+  const int64_t G = gridDim.x;
+  typename Op::Edge ed[VT];
+  int32_t ov[Q];
+  State sv[Q];
+  // First stage of tile tt (partition bounds p0, p1) into ed / ov / sv.
+  auto stage1 = [&](int64_t tt, int32_t p0, int32_t p1) {
+    const int64_t e0 = tt * TILE;
+    const int64_t e1 = e0 + TILE < path ? e0 + TILE : path;
+    const int32_t jj0 = (int32_t)(e0 - p0), nne = (int32_t)(e1 - p1) - jj0, nni = p1 - p0;
+    if (nne > 0) {
+#pragma unroll
+      for (int r = 0; r < VT; r++) ed[r] = op.load_edge(jj0 + min(tid + r * kRowBlock, nne - 1));
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int32_t k = min(tid + q * kRowBlock, nni + 1);
+      const int32_t i = p0 - 1 + k;  // <= p1 <= n: offs has n + 1 entries
+      ov[q] = i < 0 ? 0 : offs[i];
+      sv[q] = op.vertex_state(i < 0 ? 0 : i < n ? i : n - 1);
+    }
+  };
+  int64_t t = blockIdx.x;
+  int32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;  // partition bounds of tiles t and t + G
+  if (t * TILE < path) {
+    a0 = parts[t];
+    a1 = parts[t + 1];
+    stage1(t, a0, a1);
+  }
+  if ((t + G) * TILE < path) {
+    b0 = parts[t + G];
+    b1 = parts[t + G + 1];
+  }
+  for (; t * TILE < path; t += G) {
     const int64_t d0 = t * TILE;
     const int64_t d1 = d0 + TILE < path ? d0 + TILE : path;
-    const int32_t i0 = parts[t], i1 = parts[t + 1];
+    const int32_t i0 = a0, i1 = a1;
     const int32_t j0 = (int32_t)(d0 - i0), j1 = (int32_t)(d1 - i1);
     const int32_t ni = i1 - i0, ne = j1 - j0;
-    for (int k = tid; k <= ni + 1; k += kRowBlock) {
-      const int32_t i = i0 - 1 + k;
-      s_offs[k] = i < 0 ? 0 : offs[i];  // i <= i1 <= n: offs has n + 1 entries
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int32_t k = tid + q * kRowBlock;
+      if (k <= ni + 1) s_offs[k] = ov[q];
+      if (k <= ni) s_state[k] = sv[q];
     }
     if (tid == 0) {
       s_carry.open_v = -1;
@@ -117,17 +166,10 @@ k_seg_rows(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const 
         }
       }
     }
-    __syncthreads();
-    if (ne > 0) {  // ne <= TILE: one pass, VT gathers in flight per thread
-      int32_t sl[VT];
+    if (ne > 0) {  // second stage: the gathers, VT in flight per thread (ne <= TILE)
       float val[VT];
 #pragma unroll
-      for (int r = 0; r < VT; r++) sl[r] = s_slot[min(tid + r * kRowBlock, ne - 1)];
-#pragma unroll
-      for (int r = 0; r < VT; r++) {
-        const int32_t k = min(tid + r * kRowBlock, ne - 1);
-        val[r] = op.edge(i0 - 1 + sl[r], j0 + k);
-      }
+      for (int r = 0; r < VT; r++) val[r] = op.gather(ed[r]);
 #pragma unroll
       for (int r = 0; r < VT; r++)
         if (tid + r * kRowBlock < ne) s_val[tid + r * kRowBlock] = val[r];
@@ -159,7 +201,7 @@ k_seg_rows(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const 
             // in-tile sum.  Rows strictly inside one thread start in the tile.
             const int32_t s = cs;
             const bool cont = s_offs[s + 1] > j1;
-            if (!cont && s >= 1) op.item_done(i0 - 1 + s, run.value());
+            if (!cont && s >= 1) op.item_done(i0 - 1 + s, run.value(), s_state[s]);
           }
           run = KSum();
           cs = sl[k];
@@ -201,6 +243,16 @@ k_seg_rows(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const 
     }
     s_incl[tid] = make_float2(ss, sc);
     __syncthreads();
+    // Next tile's first stage (and the bounds of the one after), ahead of
+    // this tile's writes.
+    int32_t c0 = 0, c1 = 0;
+    if ((t + G) * TILE < path) {
+      stage1(t + G, b0, b1);
+      if ((t + 2 * G) * TILE < path) {
+        c0 = parts[t + 2 * G];
+        c1 = parts[t + 2 * G + 1];
+      }
+    }
 
     // Step 5: complete the first and last runs of this thread.
     if (has) {
@@ -221,7 +273,7 @@ k_seg_rows(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const 
           s_carry.open = make_float2(tot.s, tot.c);
           s_carry.open_v = i0 - 1 + s;
         } else {
-          op.item_done(i0 - 1 + s, tot.value());
+          op.item_done(i0 - 1 + s, tot.value(), s_state[s]);
         }
       };
       if (fs != ls) {
@@ -235,7 +287,7 @@ k_seg_rows(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const 
     // at the tile's end (open with an empty partial).
     for (int k = 1 + tid; k <= ni; k += kRowBlock) {
       if (s_offs[k + 1] == s_offs[k]) {
-        op.item_done(i0 - 1 + k, 0.0f);
+        op.item_done(i0 - 1 + k, 0.0f, s_state[k]);
       } else if (s_offs[k] == j1) {
         s_carry.open = make_float2(0.f, 0.f);
         s_carry.open_v = i0 - 1 + k;
@@ -243,6 +295,10 @@ k_seg_rows(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const 
     }
     __syncthreads();
     if (tid == 0) carry[t] = s_carry;
+    a0 = b0;
+    a1 = b1;
+    b0 = c0;
+    b1 = c1;
   }
   op.finish();
 }
@@ -262,7 +318,7 @@ k_seg_fixup(Op op, const SegCarry *__restrict__ carry, int64_t ntiles, const int
     KSum acc;
     for (int64_t q = ts; q < t; q++) acc.merge(carry[q].open.x, carry[q].open.y);
     acc.merge(carry[t].tail.x, carry[t].tail.y);
-    op.item_done(v, acc.value());
+    op.item_done(v, acc.value(), op.vertex_state(v));
   }
   op.finish();
 }

@@ -117,19 +117,30 @@ struct PrForwardOp {
   }
   __device__ __forceinline__ int32_t row_begin(int32_t v) { return off[v]; }
   __device__ __forceinline__ int32_t row_end(int32_t v) { return off[v + 1]; }
-  __device__ __forceinline__ float edge(int32_t, int32_t k) {
-    const int32_t u = src[k];
-    return pcsc ? rc[u] * pcsc[k] : xc[u];
-  }
-  __device__ __forceinline__ void item_done(int32_t v, float sum) {
+  // Staged form (segreduce.h): source and P_uv load with the tile, the
+  // contribution r_u P_uv (or x_u) is gathered after; r_v and 1/outdeg_v
+  // (read-only during a sweep) are prefetched for item_done.
+  struct Edge {
+    int32_t u;
+    float p;
+  };
+  struct State {
+    float r_old, io;
+  };
+  __device__ __forceinline__ Edge load_edge(int32_t k) { return {src[k], pcsc ? pcsc[k] : 0.0f}; }
+  __device__ __forceinline__ float gather(const Edge &e) { return pcsc ? rc[e.u] * e.p : xc[e.u]; }
+  __device__ __forceinline__ State vertex_state(int32_t v) { return {rc[v], inv_out[v]}; }
+  __device__ __forceinline__ void item_done(int32_t v, float sum, const State &st) {
     const float tv = tele ? (1.0f - d) * tele[v] : uniform_tele;
     const float rv = d * (sum + dn) + tv;
     rn[v] = rv;
-    const float io = inv_out[v];
-    if (!pcsc) xn[v] = rv * io;
-    resid += fabs((double)rv - (double)rc[v]);
-    if (io == 0.0f) dang += rv;
+    if (!pcsc) xn[v] = rv * st.io;
+    resid += fabs((double)rv - (double)st.r_old);
+    if (st.io == 0.0f) dang += rv;
   }
+  // Per-edge form (XCD-sliced sweeps).
+  __device__ __forceinline__ float edge(int32_t, int32_t k) { return gather(load_edge(k)); }
+  __device__ __forceinline__ void item_done(int32_t v, float sum) { item_done(v, sum, vertex_state(v)); }
   __device__ __forceinline__ void finish() {
     block_add_f64<kRowBlock>(st->resid_acc, resid);
     block_add_f64<kRowBlock>(st->aux_acc, dang);
@@ -162,19 +173,33 @@ struct PrAdjointOp {
   }
   __device__ __forceinline__ int32_t row_begin(int32_t u) { return off[u]; }
   __device__ __forceinline__ int32_t row_end(int32_t u) { return off[u + 1]; }
-  __device__ __forceinline__ float edge(int32_t, int32_t e) {
-    const float l = lc[dst[e]];
-    return pcsr ? pcsr[e] * l : l;
+  // Staged form (segreduce.h): target and P_uv load with the tile, lambda_v
+  // is gathered after; lambda_u, 1/outdeg_u and c_u (read-only during a
+  // sweep) are prefetched for item_done.
+  struct Edge {
+    int32_t v;
+    float p;
+  };
+  struct State {
+    float l_old, io, cu;
+  };
+  __device__ __forceinline__ Edge load_edge(int32_t e) { return {dst[e], pcsr ? pcsr[e] : 0.0f}; }
+  __device__ __forceinline__ float gather(const Edge &e) {
+    const float l = lc[e.v];
+    return pcsr ? e.p * l : l;
   }
-  __device__ __forceinline__ void item_done(int32_t u, float sum) {
-    const float io = inv_out[u];
-    const float su = pcsr ? sum : sum * io;
-    const float ln_u = c[u] + d * (su + (io == 0.0f ? ln : 0.0f));
+  __device__ __forceinline__ State vertex_state(int32_t u) { return {lc[u], inv_out[u], c[u]}; }
+  __device__ __forceinline__ void item_done(int32_t u, float sum, const State &st) {
+    const float su = pcsr ? sum : sum * st.io;
+    const float ln_u = st.cu + d * (su + (st.io == 0.0f ? ln : 0.0f));
     lnx[u] = ln_u;
     rowsum[u] = su;
-    resid += fabs((double)ln_u - (double)lc[u]);
+    resid += fabs((double)ln_u - (double)st.l_old);
     lsum += ln_u;
   }
+  // Per-edge form (XCD-sliced sweeps).
+  __device__ __forceinline__ float edge(int32_t, int32_t e) { return gather(load_edge(e)); }
+  __device__ __forceinline__ void item_done(int32_t u, float sum) { item_done(u, sum, vertex_state(u)); }
   __device__ __forceinline__ void finish() {
     block_add_f64<kRowBlock>(st->resid_acc, resid);
     block_add_f64<kRowBlock>(st->aux_acc, lsum);
