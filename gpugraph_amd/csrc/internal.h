@@ -200,6 +200,11 @@ struct gg_graph {
   // carries of the segmented row sums.
   gg::DevBuf csr_parts, seg_carry;
   bool csr_parts_ready = false;
+  // Packed CSC in-edges for the SSSP pull (source << pk_wbits | weight),
+  // built per values version when every weight and source id fits.
+  gg::DevBuf pk_edges;
+  int64_t pk_version = -1;
+  int32_t pk_wbits = 0;  // 0: not packable
   int64_t values_version = 0, w_csc_version = -1;
   // Undirected graphs: rows sorted and symmetry confirmed (mirror_ok); the
   // mirrored entries' values differ (values_asymmetric; mirror permutation

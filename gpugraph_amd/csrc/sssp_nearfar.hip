@@ -24,8 +24,12 @@
 //            tile seams).
 //
 // Byte model: collect n + 4(dirty) + 4F; push 16F + 12E + 8N;
-// pull 4n + 12m (offsets; source, weight and distance per in-edge).
+// pull 4n + 12m (offsets; source, weight and distance per in-edge: the same
+// 12 bytes per inspected edge as the push, whichever in-edge layout is read;
+// the packed layout moves 8 of them).
 #include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
 
 #include <climits>
 #include <cstring>
@@ -143,6 +147,102 @@ __global__ void k_sssp_init(int32_t *dist, uint8_t *dirty, int32_t src) {
   dirty[src] = 1;
 }
 
+// The pull over packed in-edges: one 32-bit word per edge holds the source
+// (high bits) and the weight (low wbits bits), so an edge is one load of the
+// stream instead of two.
+struct PackedPullOp {
+  const uint32_t *pk;
+  int32_t wbits;
+  uint32_t wmask;
+  int32_t *dist;
+  uint8_t *dirty;
+  unsigned long long *counters;
+  Tally tally;
+
+  struct Edge {
+    int32_t u, w;
+  };
+  __device__ __forceinline__ Edge load_edge(int32_t slot) {
+    const uint32_t e = pk[slot];
+    return {(int32_t)(e >> wbits), (int32_t)(e & wmask)};
+  }
+  __device__ __forceinline__ int32_t gather(const Edge &e) {
+    const int64_t c = (int64_t)dist[e.u] + e.w;
+    return c < kInf ? (int32_t)c : INT_MAX;
+  }
+  __device__ __forceinline__ int32_t identity() { return INT_MAX; }
+  __device__ __forceinline__ void combine(int32_t *acc, int32_t val) {
+    if (val < *acc) atomicMin(acc, val);
+  }
+  __device__ __forceinline__ int32_t vertex_state(int32_t v) { return dist[v]; }
+  __device__ __forceinline__ void item_done(int32_t v, int32_t acc, bool whole, int32_t cur) {
+    if (acc == INT_MAX || acc >= cur) return;
+    int32_t old = cur;
+    if (whole) {
+      dist[v] = acc;
+    } else {
+      old = atomicMin(&dist[v], acc);
+      if (acc >= old) return;
+    }
+    tally.add();
+    dirty[v] = 1;
+  }
+  __device__ __forceinline__ void finish() { tally.flush(counters); }
+};
+
+__global__ void k_pack_in_edges(const int32_t *__restrict__ src, const int32_t *__restrict__ w, int64_t m,
+                                int32_t wbits, uint32_t *__restrict__ pk) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x)
+    pk[k] = ((uint32_t)src[k] << wbits) | (uint32_t)w[k];
+}
+
+int bits_for(int64_t x) {  // bits to hold 0..x
+  int b = 1;
+  while (b < 40 && (((int64_t)1 << b) <= x)) b++;
+  return b;
+}
+
+// Packed in-edges of the graph for weights wc (CSC order), rebuilt when the
+// values change; *pk = nullptr when a weight is negative or source id and
+// weight do not fit one 32-bit word together.
+int packed_in_edges(gg_graph *g, const int32_t *wc, const uint32_t **pk, int32_t *wbits) {
+  *pk = nullptr;
+  *wbits = 0;
+  if (g->m == 0) return GG_OK;
+  if (g->pk_version != g->values_version) {
+    g->pk_version = g->values_version;
+    g->pk_wbits = 0;
+    hipStream_t s = g->stream;
+    DevBuf mm, tmp;
+    GG_TRY(mm.ensure(2 * sizeof(int32_t)));
+    int32_t *d_mm = mm.as<int32_t>();
+    size_t tb = 0, tb2 = 0;
+    GG_HIP(hipcub::DeviceReduce::Min(nullptr, tb, wc, d_mm, (int)g->m, s));
+    GG_HIP(hipcub::DeviceReduce::Max(nullptr, tb2, wc, d_mm + 1, (int)g->m, s));
+    GG_TRY(tmp.ensure(std::max(tb, tb2)));
+    GG_HIP(hipcub::DeviceReduce::Min(tmp.p, tb, wc, d_mm, (int)g->m, s));
+    GG_HIP(hipcub::DeviceReduce::Max(tmp.p, tb2, wc, d_mm + 1, (int)g->m, s));
+    int32_t h[2];
+    GG_HIP(hipMemcpyAsync(h, d_mm, sizeof(h), hipMemcpyDeviceToHost, s));
+    GG_HIP(hipStreamSynchronize(s));
+    const int wb = bits_for(h[1]), ub = bits_for((int64_t)g->n - 1);
+    if (h[0] >= 0 && wb + ub <= 32) {
+      GG_TRY(g->pk_edges.ensure(4 * (size_t)g->m));
+      hipLaunchKernelGGL(k_pack_in_edges, dim3(grid_for(g->m, 256, 16384)), dim3(256), 0, s, g->dg.row_indices, wc,
+                         g->m, wb, g->pk_edges.as<uint32_t>());
+      GG_HIP(hipGetLastError());
+      g->pk_wbits = wb;
+    } else {
+      g->pk_edges.reset();
+    }
+  }
+  if (g->pk_wbits > 0) {
+    *pk = g->pk_edges.as<uint32_t>();
+    *wbits = g->pk_wbits;
+  }
+  return GG_OK;
+}
+
 // Clears the collect's counters (queue sizes, dirty left, min dist left in
 // every stripe, frontier degree sum) for the next collect.
 __device__ __forceinline__ void sssp_reset_collect(unsigned long long *ctr) {
@@ -180,7 +280,7 @@ __global__ void k_sssp_ctl_init(SsspCtl *c, unsigned long long *ctr, int32_t bou
 }
 
 __global__ void k_sssp_ctrl(SsspCtl *c, unsigned long long *__restrict__ ctr, int32_t n, int64_t m, int64_t delta,
-                            int64_t pull_alpha, int32_t max_iter) {
+                            int64_t pull_alpha, int32_t max_iter, double pull_bytes) {
   c->push_size = 0;
   c->pull_skip = 1;
   if (c->done) return;
@@ -203,7 +303,7 @@ __global__ void k_sssp_ctrl(SsspCtl *c, unsigned long long *__restrict__ ctr, in
   }
   if (pull_alpha > 0 && (int64_t)degsum > m / pull_alpha) {
     c->pull_skip = 0;
-    c->bytes += 4.0 * n + 12.0 * m;
+    c->bytes += pull_bytes;
     c->edges += m;
     c->pulls++;
   } else {
@@ -298,6 +398,10 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       pull_alpha = 0;
     }
   }
+  const uint32_t *pk = nullptr;
+  int32_t pk_bits = 0;
+  if (pull_alpha > 0) GG_TRY(packed_in_edges(g, static_cast<const int32_t *>(wcv), &pk, &pk_bits));
+  const double pull_bytes = 4.0 * n + 12.0 * m;  // per inspected in-edge, as for the push
   int32_t *dist;
   if (flags & GG_OUT_DEVICE) {
     GG_CHECK_ARG(dist_out, "dist is NULL");
@@ -333,6 +437,16 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   double bytes = 5.0 * n;
   const int collect_grid = grid_for(n, kCollectStep, 2048);
   PullOp pull_op{g->dg.row_indices, static_cast<const int32_t *>(wcv), dist, dirty, ctr, {}};
+  PackedPullOp pk_op{pk, pk_bits, pk_bits ? (uint32_t)((1ull << pk_bits) - 1) : 0u, dist, dirty, ctr, {}};
+  // One pull launch (packed in-edges when the graph has them).
+  auto launch_pull = [&](const int32_t *skip) {
+    if (pk)
+      hipLaunchKernelGGL((k_pull_staged<PackedPullOp, int32_t>), dim3(2048), dim3(kExpandBlock), 0, s, pk_op,
+                         g->dg.column_offsets, n, m, pparts, skip);
+    else
+      hipLaunchKernelGGL((k_pull_staged<PullOp, int32_t>), dim3(2048), dim3(kExpandBlock), 0, s, pull_op,
+                         g->dg.column_offsets, n, m, pparts, skip);
+  };
   PushOp push_op{g->dg.row_offsets, g->dg.column_indices, static_cast<const int32_t *>(g->dg.edge_values), dist,
                  dirty, ctr, {}};
   if (!g->prof.enabled) {
@@ -350,7 +464,8 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
         (uint64_t)dist, (uint64_t)dirty, (uint64_t)queue, (uint64_t)ctr, (uint64_t)ctl, (uint64_t)wcv,
         (uint64_t)pparts, (uint64_t)lb.scan_local, (uint64_t)lb.partials, (uint64_t)lb.info, (uint64_t)lb.parts,
         (uint64_t)lb.edge_counter, (uint64_t)lb.expand_grid, (uint64_t)lb.vt, (uint64_t)delta,
-        (uint64_t)pull_alpha, (uint64_t)g->opt.iter_num, (uint64_t)g->frontier[1].p};
+        (uint64_t)pull_alpha, (uint64_t)g->opt.iter_num, (uint64_t)g->frontier[1].p, (uint64_t)pk,
+        (uint64_t)pk_bits};
     if (!g->sssp_loop.exec || g->sssp_loop.key != key) {
       g->sssp_loop.reset();
       hipGraph_t graph = nullptr;
@@ -358,12 +473,11 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       for (int it = 0; it < kLoopIters; it++) {
         hipLaunchKernelGGL(k_sssp_collect, dim3(collect_grid), dim3(256), 0, s, dirty, dist, g->dg.row_offsets, n, 0,
                            queue, qsize, ctr, (const SsspCtl *)ctl);
-        hipLaunchKernelGGL(k_sssp_ctrl, dim3(1), dim3(1), 0, s, ctl, ctr, n, m, delta, pull_alpha, g->opt.iter_num);
+        hipLaunchKernelGGL(k_sssp_ctrl, dim3(1), dim3(1), 0, s, ctl, ctr, n, m, delta, pull_alpha, g->opt.iter_num,
+                           pull_bytes);
         lbs_launch(push_op, false, queue, &ctl->push_size, n, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(),
                    qsize + 1, s, [](int) {});
-        if (pull_alpha > 0)
-          hipLaunchKernelGGL((k_pull_staged<PullOp, int32_t>), dim3(2048), dim3(kExpandBlock), 0, s, pull_op,
-                             g->dg.column_offsets, n, m, pparts, (const int32_t *)&ctl->pull_skip);
+        if (pull_alpha > 0) launch_pull((const int32_t *)&ctl->pull_skip);
       }
       const hipError_t le = hipGetLastError();
       const hipError_t ce = hipStreamEndCapture(s, &graph);
@@ -418,11 +532,10 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     if (pull_alpha > 0 && degsum > m / pull_alpha) {
       {
         TimedRegion t(g->prof, "sssp.pull");
-        hipLaunchKernelGGL((k_pull_staged<PullOp, int32_t>), dim3(2048), dim3(kExpandBlock), 0, s, pull_op,
-                           g->dg.column_offsets, n, m, pparts, (const int32_t *)nullptr);
+        launch_pull((const int32_t *)nullptr);
         GG_HIP(hipGetLastError());
       }
-      const double pb = 4.0 * n + 12.0 * m;
+      const double pb = pull_bytes;
       g->prof.add_launch_bytes("sssp.pull", pb);
       bytes += pb;
       edges += m;

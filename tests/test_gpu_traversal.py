@@ -203,6 +203,29 @@ def test_device_driven_loops_match_host_loops(gpu_available, undirected):
             assert res[0][1] == res[1][1] == 2, prog
 
 
+@pytest.mark.parametrize("hi", [64, 1 << 24])
+def test_sssp_pull_packed_and_wide_weights(gpu_available, hi):
+    """Pull steps read packed in-edges (source and weight in one word) when
+    ids and weights fit 32 bits together (weights up to 64 on 2^14 vertices),
+    and the two-stream in-edges otherwise (weights up to 2^24); both against
+    the oracle, with and without renumbering."""
+    for undirected in (True, False):
+        c = gg.gen_rmat(14, 16, undirected=undirected, seed=28).with_weights("i32", 1, hi, symmetric=undirected, seed=29)
+        src = workloads.max_degree_vertex(c)
+        exp = oracle.sssp(c.row_offsets, c.col_indices, c.values, src)
+        for reorder in (False, True):
+            pulls = 0
+            with gg.Graph(c, directed=not undirected, reorder=reorder) as g:
+                for alpha in (2, 4, 16):  # pull when the frontier's out-edges exceed m / alpha
+                    # buckets as wide as a few edges, so frontiers grow large enough to pull
+                    g.set_options(pull_alpha=alpha, sssp_delta=4 * hi)
+                    for fl in (0, _abi.GG_INSTRUMENT):
+                        d, st = g.sssp(src, flags=fl)
+                        np.testing.assert_array_equal(d, exp, err_msg=str((hi, undirected, reorder, alpha, fl)))
+                        pulls += st["pull_iterations"]
+            assert pulls > 0
+
+
 def test_error_paths(gpu_available):
     c = workloads.config1_er()
     with gg.Graph(c) as g:
