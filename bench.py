@@ -46,7 +46,7 @@ def log(msg):
 # rocprofv3 --pmc summary (FETCH_SIZE x2 + WRITE_SIZE x1, calibrated by a
 # 1 GiB copy), for `roofline.traffic`.
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "headline_rmat22_hbm_traffic_pmc.md")
-TRAFFIC_ROWS = {"sssp.pull": "k_pull_staged<PullOp, int>", "sssp.push": "k_lbs_expand<PushOp, 4>",
+TRAFFIC_ROWS = {"sssp.pull": "k_pull_staged<PackedPullOp, int>", "sssp.push": "k_lbs_expand<PushOp, 4>",
                 "bfs.bottom_up": "k_bfs_bottom_up", "bfs.top_down": "k_lbs_expand<BfsPushOp, 4>"}
 
 
