@@ -314,12 +314,38 @@ __global__ void k_sssp_ctrl(SsspCtl *c, unsigned long long *__restrict__ ctr, in
   c->iters++;
 }
 
-// A workgroup takes 1024 consecutive vertices per step, four coalesced passes
-// of 256.
-// One global atomic per step for the queue, one per workgroup per counter.
+// A workgroup takes 4096 consecutive vertices per step, 16 per thread: the
+// thread's 16 dirty bytes arrive in one 16-byte load (the dirty array is
+// padded with zeros to a multiple of 16), and a step where no byte is set
+// ends there, after one barrier.  For the set bytes the thread reads the 16
+// distances and 17 row offsets as 16-byte vectors (VEC: dist and the offsets
+// 16-byte aligned and the 16 vertices inside n; otherwise one load per set
+// byte).  Selected
+// vertices go to the queue in vertex order within a step, one global atomic
+// per step; the counters take one atomic per workgroup.
 constexpr int kCollectBlock = 256;
-constexpr int kCollectStep = 4 * kCollectBlock;
+constexpr int kCollectPer = 16;
+constexpr int kCollectStep = kCollectPer * kCollectBlock;
 
+// Bits j of the result: byte j of the 16 bytes is nonzero.
+__device__ __forceinline__ uint32_t nonzero_bytes(uint4 d) {
+  const uint32_t w[4] = {d.x, d.y, d.z, d.w};
+  uint32_t m = 0;
+#pragma unroll
+  for (int j = 0; j < 16; j++) m |= ((w[j >> 2] >> (8 * (j & 3))) & 0xffu) ? (1u << j) : 0u;
+  return m;
+}
+
+// The 16 bytes with byte j cleared for every bit j of `clear`.
+__device__ __forceinline__ uint4 clear_bytes(uint4 d, uint32_t clear) {
+  uint32_t w[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+  for (int j = 0; j < 16; j++)
+    if (clear & (1u << j)) w[j >> 2] &= ~(0xffu << (8 * (j & 3)));
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+template <bool VEC>
 __global__ void __launch_bounds__(kCollectBlock)
 k_sssp_collect(uint8_t *__restrict__ dirty, const int32_t *__restrict__ dist, const int32_t *__restrict__ ro,
                int32_t n, int32_t bound, int32_t *__restrict__ queue, int32_t *__restrict__ qsize,
@@ -335,33 +361,71 @@ k_sssp_collect(uint8_t *__restrict__ dirty, const int32_t *__restrict__ dist, co
   unsigned long long remain = 0, degsum = 0;
   int32_t dmin = INT_MAX;
   for (int64_t base = (int64_t)blockIdx.x * kCollectStep; base < n; base += (int64_t)gridDim.x * kCollectStep) {
-    uint32_t selmask = 0;  // bit k: vertex of pass k selected (no dynamically indexed arrays)
+    const int64_t v0 = base + (int64_t)threadIdx.x * kCollectPer;
+    uint4 d4 = make_uint4(0u, 0u, 0u, 0u);
+    if (v0 < n) d4 = *reinterpret_cast<const uint4 *>(dirty + v0);
+    const uint32_t nz = nonzero_bytes(d4);
+    if (!__syncthreads_or(nz != 0)) continue;
+    uint32_t sel = 0;  // bit j: vertex v0 + j selected
+    if (nz) {
+      if (VEC && v0 + kCollectPer <= n) {
+        int32_t dv[16], rv[17];
+        const int4 *dp = reinterpret_cast<const int4 *>(dist + v0);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int64_t v = base + k * kCollectBlock + threadIdx.x;
-      bool sel = false;
-      if (v < n && dirty[v]) {
-        const int32_t d = dist[v];
-        if (d < bound) {
-          sel = true;
-          dirty[v] = 0;
-          degsum += (unsigned long long)(ro[v + 1] - ro[v]);
-        } else {
-          remain++;
-          dmin = min(dmin, d);
+        for (int q = 0; q < 4; q++) {
+          const int4 x = dp[q];
+          dv[4 * q] = x.x;
+          dv[4 * q + 1] = x.y;
+          dv[4 * q + 2] = x.z;
+          dv[4 * q + 3] = x.w;
+        }
+        const int4 *rp = reinterpret_cast<const int4 *>(ro + v0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int4 x = rp[q];
+          rv[4 * q] = x.x;
+          rv[4 * q + 1] = x.y;
+          rv[4 * q + 2] = x.z;
+          rv[4 * q + 3] = x.w;
+        }
+        rv[16] = ro[v0 + 16];  // v0 + 16 <= n: offsets has n + 1 entries
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          if (!(nz & (1u << j))) continue;
+          if (dv[j] < bound) {
+            sel |= 1u << j;
+            degsum += (unsigned long long)(rv[j + 1] - rv[j]);
+          } else {
+            remain++;
+            dmin = min(dmin, dv[j]);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          if (!(nz & (1u << j))) continue;
+          const int64_t v = v0 + j;  // a set byte lies below n
+          const int32_t d = dist[v];
+          if (d < bound) {
+            sel |= 1u << j;
+            degsum += (unsigned long long)(ro[v + 1] - ro[v]);
+          } else {
+            remain++;
+            dmin = min(dmin, d);
+          }
         }
       }
-      selmask |= sel ? (1u << k) : 0u;
+      if (sel) *reinterpret_cast<uint4 *>(dirty + v0) = clear_bytes(d4, sel);
     }
     int32_t total;
-    const int32_t off = block_exclusive_scan<kCollectBlock>(__popc(selmask), s_scan, &total);
+    const int32_t off = block_exclusive_scan<kCollectBlock>(__popc(sel), s_scan, &total);
     if (total) {
       if (threadIdx.x == 0) s_base = atomicAdd(qsize, total);
       __syncthreads();
       int32_t at = s_base + off;
 #pragma unroll
-      for (int k = 0; k < 4; k++)
-        if (selmask & (1u << k)) queue[at++] = (int32_t)(base + k * kCollectBlock + threadIdx.x);
+      for (int j = 0; j < 16; j++)
+        if (sel & (1u << j)) queue[at++] = (int32_t)(v0 + j);
       __syncthreads();
     }
   }
@@ -387,7 +451,8 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   hipStream_t s = g->stream;
   LbsBuffers lb;
   GG_TRY(lbs_buffers(g, &lb));
-  GG_TRY(g->vtx[3].ensure((size_t)n));
+  const int64_t n_dirty = ((int64_t)n + kCollectPer - 1) / kCollectPer * kCollectPer;  // zero padding for the collect
+  GG_TRY(g->vtx[3].ensure((size_t)n_dirty));
   const int32_t *pparts = nullptr;
   const void *wcv = nullptr;
   int64_t pull_alpha = g->opt.pull_alpha;
@@ -425,7 +490,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   {
     TimedRegion t(g->prof, "sssp.init");
     GG_HIP(fill_async<int32_t>(dist, kInf, n, s));
-    GG_HIP(fill_async<uint8_t>(dirty, (uint8_t)0, n, s));
+    GG_HIP(fill_async<uint8_t>(dirty, (uint8_t)0, n_dirty, s));
     GG_HIP(fill_async<unsigned long long>(ctr, 0ull, kCounterWords, s));
     hipLaunchKernelGGL(k_sssp_init, dim3(1), dim3(1), 0, s, dist, dirty, src);
     GG_HIP(hipGetLastError());
@@ -435,7 +500,10 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   int iterations = 0, pulls = 0;
   int64_t edges = 0, prev_improved = 0;
   double bytes = 5.0 * n;
-  const int collect_grid = grid_for(n, kCollectStep, 2048);
+  const int collect_grid = grid_for(n, kCollectStep, 4096);
+  // The collect reads dist and the offsets as 16-byte vectors when both arrays allow.
+  const bool collect_vec = ((reinterpret_cast<uintptr_t>(dist) | reinterpret_cast<uintptr_t>(g->dg.row_offsets)) & 15) == 0;
+  const auto collect = collect_vec ? k_sssp_collect<true> : k_sssp_collect<false>;
   PullOp pull_op{g->dg.row_indices, static_cast<const int32_t *>(wcv), dist, dirty, ctr, {}};
   PackedPullOp pk_op{pk, pk_bits, pk_bits ? (uint32_t)((1ull << pk_bits) - 1) : 0u, dist, dirty, ctr, {}};
   // One pull launch (packed in-edges when the graph has them).
@@ -471,7 +539,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       hipGraph_t graph = nullptr;
       GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       for (int it = 0; it < kLoopIters; it++) {
-        hipLaunchKernelGGL(k_sssp_collect, dim3(collect_grid), dim3(256), 0, s, dirty, dist, g->dg.row_offsets, n, 0,
+        hipLaunchKernelGGL(collect, dim3(collect_grid), dim3(kCollectBlock), 0, s, dirty, dist, g->dg.row_offsets, n, 0,
                            queue, qsize, ctr, (const SsspCtl *)ctl);
         hipLaunchKernelGGL(k_sssp_ctrl, dim3(1), dim3(1), 0, s, ctl, ctr, n, m, delta, pull_alpha, g->opt.iter_num,
                            pull_bytes);
@@ -510,7 +578,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     GG_HIP(slot_resets_async(ctr, {{8, 0ull}, {9, 0ull}, {16, 0ull}, {17, (unsigned long long)INT_MAX}, {18, 0ull}}, s));
     {
       TimedRegion t(g->prof, "sssp.collect");
-      hipLaunchKernelGGL(k_sssp_collect, dim3(collect_grid), dim3(256), 0, s, dirty, dist, g->dg.row_offsets, n,
+      hipLaunchKernelGGL(collect, dim3(collect_grid), dim3(kCollectBlock), 0, s, dirty, dist, g->dg.row_offsets, n,
                          (int32_t)std::min<int64_t>(bound, kInf), queue, qsize, ctr, (const SsspCtl *)nullptr);
       GG_HIP(hipGetLastError());
     }
