@@ -178,11 +178,12 @@ struct CcPullOp {
   };
   __device__ __forceinline__ Edge load_edge(int32_t slot) { return {nbr[slot]}; }
   __device__ __forceinline__ int32_t gather(const Edge &e) { return label[e.u]; }
-  __device__ __forceinline__ int32_t identity() { return INT_MAX; }
   __device__ __forceinline__ void combine(int32_t *acc, int32_t val) {
     if (val < *acc) atomicMin(acc, val);
   }
   __device__ __forceinline__ int32_t vertex_state(int32_t v) { return label[v]; }
+  // A row starts from its own current value: only smaller gathers touch LDS.
+  __device__ __forceinline__ int32_t acc_start(int32_t state) { return state; }
   // k_pull_staged calls this with consecutive vertices on consecutive lanes
   // (v = first + lane for the active lanes), so the changed marks of a wave
   // are OR-ed into each bitmap word with one atomic; only bits the atomic

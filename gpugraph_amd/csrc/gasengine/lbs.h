@@ -527,9 +527,12 @@ k_pull_reduce(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
 //   using Edge = ...                        per-edge loads (e.g. source and weight)
 //   Edge    load_edge(int32_t slot)         loads only
 //   T       gather(const Edge &e)           dependent loads (e.g. dist[src] + w)
-//   T       identity()
 //   void    combine(T *lds_acc, T val)
 //   int32_t vertex_state(int32_t v)         prefetched per-vertex value
+//   T       acc_start(int32_t state)        a row's accumulator before its
+//                                           gathers (for a minimum: the
+//                                           vertex's own value, so gathers
+//                                           that cannot improve it skip LDS)
 //   void    item_done(int32_t v, T acc, bool whole, int32_t state)
 //           whole: every in-edge of v lies in this tile, so no other
 //           workgroup writes v during the launch and `state` is exact
@@ -591,7 +594,7 @@ k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
       if (k <= ni + 1) s_offs[k] = ov[q];
       if (k <= ni) {
         s_state[k] = sv[q];
-        s_acc[k] = op.identity();
+        s_acc[k] = op.acc_start(sv[q]);
       }
     }
     __syncthreads();

@@ -120,11 +120,12 @@ struct PullOp {
     const int64_t c = (int64_t)dist[e.u] + e.w;
     return c < kInf ? (int32_t)c : INT_MAX;
   }
-  __device__ __forceinline__ int32_t identity() { return INT_MAX; }
   __device__ __forceinline__ void combine(int32_t *acc, int32_t val) {
     if (val < *acc) atomicMin(acc, val);
   }
   __device__ __forceinline__ int32_t vertex_state(int32_t v) { return dist[v]; }
+  // A row starts from its own current value: only smaller gathers touch LDS.
+  __device__ __forceinline__ int32_t acc_start(int32_t state) { return state; }
   // cur: dist[v] at the start of the tile; exact when `whole`, otherwise
   // possibly stale (larger), and the atomicMin decides.
   __device__ __forceinline__ void item_done(int32_t v, int32_t acc, bool whole, int32_t cur) {
@@ -170,11 +171,12 @@ struct PackedPullOp {
     const int64_t c = (int64_t)dist[e.u] + e.w;
     return c < kInf ? (int32_t)c : INT_MAX;
   }
-  __device__ __forceinline__ int32_t identity() { return INT_MAX; }
   __device__ __forceinline__ void combine(int32_t *acc, int32_t val) {
     if (val < *acc) atomicMin(acc, val);
   }
   __device__ __forceinline__ int32_t vertex_state(int32_t v) { return dist[v]; }
+  // A row starts from its own current value: only smaller gathers touch LDS.
+  __device__ __forceinline__ int32_t acc_start(int32_t state) { return state; }
   __device__ __forceinline__ void item_done(int32_t v, int32_t acc, bool whole, int32_t cur) {
     if (acc == INT_MAX || acc >= cur) return;
     int32_t old = cur;
