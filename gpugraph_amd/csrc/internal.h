@@ -216,7 +216,9 @@ struct gg_graph {
 
   // Device-driven BFS / SSSP loops: captured launch sequences and their
   // device-side loop state.
-  gg::CapturedLoop bfs_loop, sssp_loop;
+  gg::CapturedLoop bfs_loop;
+  gg::CapturedLoop sssp_loops[17];  // by iterations per replay (1..16)
+  int sssp_steps_hint = 0;          // loop iterations the last device-driven SSSP took
   gg::DevBuf loop_ctl;
 
   gg::EventProfiler prof;

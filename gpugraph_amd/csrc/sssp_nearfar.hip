@@ -42,7 +42,8 @@ namespace {
 
 using namespace GASengine;
 constexpr int32_t kInf = 100000000;
-constexpr int kLoopIters = 12;  // iterations per replay of the captured loop
+constexpr int kLoopIters = 12;     // iterations per replay of the captured loop
+constexpr int kLoopItersMax = 16;  // longest replay captured (internal.h sizes the cache)
 
 // counters (striped, GASengine::block_counter_add): [2] LBS edge counter,
 // [4] improvements, [16] dirty left, [17] min dist left (minimum over the
@@ -265,6 +266,7 @@ struct SsspCtl {
   int32_t pull_skip;  // 0 when this iteration pulls
   int32_t done;
   int32_t iters, pulls;
+  int32_t steps;      // controller calls that found the run unfinished (the last one ends it)
   long long edges;    // edges inspected
   double bytes;       // algorithmic bytes (model in the header comment)
 };
@@ -277,6 +279,7 @@ __global__ void k_sssp_ctl_init(SsspCtl *c, unsigned long long *ctr, int32_t bou
   c->done = 0;
   c->iters = 0;
   c->pulls = 0;
+  c->steps = 0;
   c->edges = 0;
   c->bytes = bytes0;
 }
@@ -286,6 +289,7 @@ __global__ void k_sssp_ctrl(SsspCtl *c, unsigned long long *__restrict__ ctr, in
   c->push_size = 0;
   c->pull_skip = 1;
   if (c->done) return;
+  c->steps++;
   if (c->iters >= max_iter) {
     c->done = 1;
     return;
@@ -305,6 +309,7 @@ __global__ void k_sssp_ctrl(SsspCtl *c, unsigned long long *__restrict__ ctr, in
   }
   if (pull_alpha > 0 && (int64_t)degsum > m / pull_alpha) {
     c->pull_skip = 0;
+    c->bound = kInf;  // after a pull, buckets end: every dirty vertex is collected
     c->bytes += pull_bytes;
     c->edges += m;
     c->pulls++;
@@ -520,7 +525,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   PushOp push_op{g->dg.row_offsets, g->dg.column_indices, static_cast<const int32_t *>(g->dg.edge_values), dist,
                  dirty, ctr, {}};
   if (!g->prof.enabled) {
-    // Device-driven: kLoopIters iterations per replay of a captured launch
+    // Device-driven: a replay of a captured launch
     // sequence (collect, k_sssp_ctrl (which also resets the collect's
     // counters), LBS push over ctl->push_size, pull unless ctl->pull_skip);
     // the host reads the loop state once per replay.  Per-kernel events
@@ -536,33 +541,46 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
         (uint64_t)lb.edge_counter, (uint64_t)lb.expand_grid, (uint64_t)lb.vt, (uint64_t)delta,
         (uint64_t)pull_alpha, (uint64_t)g->opt.iter_num, (uint64_t)g->frontier[1].p, (uint64_t)pk,
         (uint64_t)pk_bits};
-    if (!g->sssp_loop.exec || g->sssp_loop.key != key) {
-      g->sssp_loop.reset();
-      hipGraph_t graph = nullptr;
-      GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      for (int it = 0; it < kLoopIters; it++) {
-        hipLaunchKernelGGL(collect, dim3(collect_grid), dim3(kCollectBlock), 0, s, dirty, dist, g->dg.row_offsets, n, 0,
-                           queue, qsize, ctr, (const SsspCtl *)ctl);
-        hipLaunchKernelGGL(k_sssp_ctrl, dim3(1), dim3(1), 0, s, ctl, ctr, n, m, delta, pull_alpha, g->opt.iter_num,
-                           pull_bytes);
-        lbs_launch(push_op, false, queue, &ctl->push_size, n, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(),
-                   qsize + 1, s, [](int) {});
-        if (pull_alpha > 0) launch_pull((const int32_t *)&ctl->pull_skip);
+    // The replay of `iters` iterations, captured on first use for this graph
+    // and these buffers / options.
+    auto loop_exec = [&](int iters, hipGraphExec_t *out) -> int {
+      CapturedLoop &L = g->sssp_loops[iters];
+      if (!L.exec || L.key != key) {
+        L.reset();
+        hipGraph_t graph = nullptr;
+        GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int it = 0; it < iters; it++) {
+          hipLaunchKernelGGL(collect, dim3(collect_grid), dim3(kCollectBlock), 0, s, dirty, dist, g->dg.row_offsets, n,
+                             0, queue, qsize, ctr, (const SsspCtl *)ctl);
+          hipLaunchKernelGGL(k_sssp_ctrl, dim3(1), dim3(1), 0, s, ctl, ctr, n, m, delta, pull_alpha, g->opt.iter_num,
+                             pull_bytes);
+          lbs_launch(push_op, false, queue, &ctl->push_size, n, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(),
+                     qsize + 1, s, [](int) {});
+          if (pull_alpha > 0) launch_pull((const int32_t *)&ctl->pull_skip);
+        }
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipStreamEndCapture(s, &graph);
+        hipError_t ie = hipErrorUnknown;
+        if (le == hipSuccess && ce == hipSuccess) ie = hipGraphInstantiate(&L.exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+          L.exec = nullptr;
+          return set_error(GG_ERR_HIP, "SSSP loop capture failed: %s / %s / %s", hipGetErrorString(le),
+                           hipGetErrorString(ce), hipGetErrorString(ie));
+        }
+        L.key = key;
       }
-      const hipError_t le = hipGetLastError();
-      const hipError_t ce = hipStreamEndCapture(s, &graph);
-      hipError_t ie = hipErrorUnknown;
-      if (le == hipSuccess && ce == hipSuccess) ie = hipGraphInstantiate(&g->sssp_loop.exec, graph, nullptr, nullptr, 0);
-      if (graph) (void)hipGraphDestroy(graph);
-      if (ie != hipSuccess) {
-        g->sssp_loop.exec = nullptr;
-        return set_error(GG_ERR_HIP, "SSSP loop capture failed: %s / %s / %s", hipGetErrorString(le),
-                         hipGetErrorString(ce), hipGetErrorString(ie));
-      }
-      g->sssp_loop.key = key;
-    }
+      *out = L.exec;
+      return GG_OK;
+    };
+    // The first replay runs as many iterations as the previous run on this
+    // graph took (a repeated query then needs one replay and no idle
+    // iterations); later replays run kLoopIters.
+    int iters = g->sssp_steps_hint > 0 ? std::min(g->sssp_steps_hint, kLoopItersMax) : kLoopIters;
     for (;;) {
-      GG_HIP(hipGraphLaunch(g->sssp_loop.exec, s));
+      hipGraphExec_t exec = nullptr;
+      GG_TRY(loop_exec(iters, &exec));
+      GG_HIP(hipGraphLaunch(exec, s));
       GG_HIP(hipMemcpyAsync(g->h_pinned, ctl, sizeof(SsspCtl), hipMemcpyDeviceToHost, s));
       GG_HIP(hipStreamSynchronize(s));
       SsspCtl h;
@@ -572,8 +590,10 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
         pulls = h.pulls;
         edges = h.edges;
         bytes = h.bytes;
+        g->sssp_steps_hint = h.steps;
         break;
       }
+      iters = kLoopIters;
     }
   }
   while (g->prof.enabled && iterations < g->opt.iter_num) {
@@ -600,6 +620,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     (void)N;
     prev_improved = improved_so_far;
     if (pull_alpha > 0 && degsum > m / pull_alpha) {
+      bound = kInf;  // as k_sssp_ctrl
       {
         TimedRegion t(g->prof, "sssp.pull");
         launch_pull((const int32_t *)nullptr);
