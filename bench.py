@@ -46,21 +46,16 @@ def log(msg):
 # rocprofv3 --pmc summary (FETCH_SIZE x2 + WRITE_SIZE x1, calibrated by a
 # 1 GiB copy), for `roofline.traffic`.
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "headline_rmat22_hbm_traffic_pmc.md")
-TRAFFIC_ROWS = {"sssp.pull": "k_pull_staged<PackedPullOp, int>", "sssp.push": "k_lbs_expand<PushOp, 4>",
-                "bfs.bottom_up": "k_bfs_bottom_up", "bfs.top_down": "k_lbs_expand<BfsPushOp, 4>"}
-
-
 def measured_traffic(kernel_class):
-    """HBM bytes per launch of a kernel class from the committed counter
-    summary (fetch + write per working dispatch), or None."""
-    row = TRAFFIC_ROWS.get(kernel_class)
-    if row is None or not os.path.exists(TRAFFIC_SUMMARY):
+    """HBM bytes per launch of a kernel class (fetch + write per launch of the
+    instrumented pass) from the committed counter summary, or None."""
+    if not os.path.exists(TRAFFIC_SUMMARY):
         return None
     for line in open(TRAFFIC_SUMMARY):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 4 and cells[0].startswith(row):
+        if len(cells) == 6 and cells[1] == kernel_class:
             try:
-                return {"bytes": (float(cells[2]) + float(cells[3])) * 1e6,
+                return {"bytes": (float(cells[4]) + float(cells[5])) * 1e6, "kernel": cells[0],
                         "source": "profiles/" + os.path.basename(TRAFFIC_SUMMARY)}
             except ValueError:
                 return None
