@@ -156,6 +156,47 @@ def test_device_outputs_and_instrumentation(gpu_available):
         assert all(v["launches"] > 0 and v["total_ms"] >= 0 for v in prof.values())
 
 
+def test_sssp_replay_sizes_follow_previous_runs(gpu_available):
+    """The device-driven SSSP sizes its first replay by the previous run on
+    the same graph.  A short query, then one needing many more iterations
+    than that (a 60-vertex unit-weight chain with unit buckets: about two
+    loop iterations per vertex, several replays), then the short one with a
+    replay longer than it needs: every result must match the oracle, and
+    with pushes only the iteration counts must match the host-driven loop."""
+    r = gg.gen_rmat(12, 8, undirected=True, seed=41).with_weights("i32", 1, 64, symmetric=True, seed=42)
+    L = 60
+    n = r.n + L
+    rows = [np.repeat(np.arange(r.n), np.diff(r.row_offsets))]
+    cols = [r.col_indices.astype(np.int64)]
+    vals = [r.values.astype(np.int64)]
+    a = np.arange(r.n, r.n + L - 1)
+    rows += [a, a + 1]
+    cols += [a + 1, a]
+    vals += [np.ones(L - 1, np.int64)] * 2
+    row, col, val = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    order = np.lexsort((col, row))
+    row, col, val = row[order], col[order], val[order]
+    ro = np.zeros(n + 1, np.int64)
+    np.add.at(ro, row + 1, 1)
+    c = csr_of(np.cumsum(ro), col, val)
+    hub = workloads.max_degree_vertex(c)
+    queries = (hub, r.n, hub, r.n + L // 2)
+    with gg.Graph(c) as g:
+        for opts in ({"pull_alpha": 2, "sssp_delta": 1}, {"pull_alpha": 0, "sssp_delta": 1}):
+            g.set_options(**gg.default_options().as_dict())
+            g.set_options(**opts)
+            for src in queries:
+                exp = oracle.sssp(c.row_offsets, c.col_indices, c.values, int(src))
+                d, st = g.sssp(int(src))
+                np.testing.assert_array_equal(d, exp, err_msg=str((opts, src)))
+                if opts["pull_alpha"] == 0:
+                    d2, st2 = g.sssp(int(src), flags=_abi.GG_INSTRUMENT)
+                    np.testing.assert_array_equal(d2, exp)
+                    assert st["iterations"] == st2["iterations"], (src, st["iterations"], st2["iterations"])
+                    if src == r.n:
+                        assert st["iterations"] > 40  # the chain query ran past several replays
+
+
 @pytest.mark.parametrize("undirected", [True, False])
 def test_device_driven_loops_match_host_loops(gpu_available, undirected):
     """BFS and SSSP run their iteration loops on the device (captured
