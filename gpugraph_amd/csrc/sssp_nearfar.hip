@@ -21,7 +21,10 @@
 //            of dist[u] + w over all in-neighbours u reduced in LDS (each
 //            thread's edges loaded back to back), written with a plain store
 //            when the tile holds all of the vertex's in-edges (atomicMin at
-//            tile seams).
+//            tile seams).  A pull ends the buckets: the bound becomes
+//            unlimited, so the next collects take every dirty vertex (the
+//            pull relaxed every in-edge, so bucket order saves no more work
+//            and the tail needs fewer iterations).
 //
 // Byte model: collect n + 4(dirty) + 4F; push 16F + 12E + 8N;
 // pull 4n + 12m (offsets; source, weight and distance per in-edge: the same
