@@ -156,6 +156,31 @@ def test_device_outputs_and_instrumentation(gpu_available):
         assert all(v["launches"] > 0 and v["total_ms"] >= 0 for v in prof.values())
 
 
+@pytest.mark.parametrize("offset", [0, 1, 3])
+def test_sssp_collect_alignment_and_tail(gpu_available, offset):
+    """The collect reads 16 dirty bytes per thread (the dirty array is padded
+    to a multiple of 16) and, when the distance array is 16-byte aligned,
+    16 distances as vectors; a caller's device array at another offset takes
+    the one-load-per-vertex path.  n = 1000 + 13 is not a multiple of 16."""
+    import torch
+
+    c = gg.gen_erdos_renyi(1013, 1013 * 6, seed=51).with_weights("i32", 1, 30, seed=52)
+    exp = {}
+    buf = torch.empty(c.n + 8, dtype=torch.int32, device="cuda")
+    out = buf[offset:offset + c.n]
+    with gg.Graph(c, directed=True) as g:
+        for opts in ({}, {"pull_alpha": 2, "sssp_delta": 8}):
+            g.set_options(**gg.default_options().as_dict())
+            g.set_options(**opts)
+            for src in (0, c.n - 1, 517):
+                if src not in exp:
+                    exp[src] = oracle.sssp(c.row_offsets, c.col_indices, c.values, src)
+                for fl in (_abi.GG_OUT_DEVICE, _abi.GG_OUT_DEVICE | _abi.GG_INSTRUMENT):
+                    g.sssp(src, flags=fl, out=out.data_ptr())
+                    g.synchronize()
+                    np.testing.assert_array_equal(out.cpu().numpy(), exp[src], err_msg=str((offset, opts, src, fl)))
+
+
 def test_sssp_replay_sizes_follow_previous_runs(gpu_available):
     """The device-driven SSSP sizes its first replay by the previous run on
     the same graph.  A short query, then one needing many more iterations
