@@ -386,6 +386,7 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
   bool bottom_up = false;
   int cur = 0;  // index of the current queue / frontier bitmap
   int levels = 0, pull_levels = 0;
+  bool timed = false;  // the device-driven loop recorded e1 and synchronised past it
   if (!g->prof.enabled) {
     // Device-driven: kLoopLevels levels per replay of a captured launch
     // sequence (an even count, so level parity and the ping-pong buffers
@@ -438,6 +439,7 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
     }
     for (;;) {
       GG_HIP(hipGraphLaunch(g->bfs_loop.exec, s));
+      GG_HIP(hipEventRecord(e1, s));  // the last replay's record times the run
       GG_HIP(hipMemcpyAsync(g->h_pinned, ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, s));
       GG_HIP(hipStreamSynchronize(s));
       BfsCtl h;
@@ -450,6 +452,7 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
         examined_total = h.examined;
         bytes = h.bytes;
         nf = 0;
+        timed = true;
         break;
       }
     }
@@ -521,8 +524,10 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
     levels++;
     cur = nxt;
   }
-  GG_HIP(hipEventRecord(e1, s));
-  GG_HIP(hipEventSynchronize(e1));
+  if (!timed) {
+    GG_HIP(hipEventRecord(e1, s));
+    GG_HIP(hipEventSynchronize(e1));
+  }
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);

@@ -167,8 +167,28 @@ __device__ __forceinline__ bool bit_test_and_set(uint32_t *bits, int32_t v) {
 
 template <class T>
 __global__ void k_fill(T *p, T value, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = value;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+  if constexpr (16 % sizeof(T) == 0) {
+    // 16-byte stores between a head and a tail of fewer than E elements each
+    // (the grid has more than E threads).
+    constexpr int E = 16 / sizeof(T);
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    int64_t head = a % sizeof(T) ? n : (int64_t)(((16 - (a & 15)) & 15) / sizeof(T));
+    head = head < n ? head : n;
+    const int64_t body = (n - head) / E, t0 = head + body * E;
+    union {
+      T t[E];
+      uint4 v;
+    } pat;
+#pragma unroll
+    for (int k = 0; k < E; k++) pat.t[k] = value;
+    uint4 *q = reinterpret_cast<uint4 *>(p + head);
+    for (int64_t i = tid; i < body; i += stride) q[i] = pat.v;
+    for (int64_t i = tid; i < head; i += stride) p[i] = value;
+    for (int64_t i = t0 + tid; i < n; i += stride) p[i] = value;
+  } else {
+    for (int64_t i = tid; i < n; i += stride) p[i] = value;
+  }
 }
 
 template <class T>
@@ -214,7 +234,8 @@ inline int grid_for(int64_t n, int block, int max_grid = 65536) {
 template <class T>
 inline hipError_t fill_async(T *p, T value, int64_t n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, p, value, n);
+  const int64_t per = 16 % sizeof(T) == 0 ? 16 / sizeof(T) : 1;  // elements per thread store
+  hipLaunchKernelGGL(k_fill<T>, dim3(grid_for((n + per - 1) / per, 256, 8192)), dim3(256), 0, s, p, value, n);
   return hipGetLastError();
 }
 

@@ -58,28 +58,46 @@ __global__ void k_reach(const int32_t *__restrict__ res, int32_t sentinel, const
   GASengine::block_counter_add<256>(&out[1], deg);
 }
 
-int reach_stats(gg_graph *g, const int32_t *result, int32_t sentinel, gg_stats *st) {
-  if (!st) return GG_OK;
+int reach_launch(gg_graph *g, const int32_t *result, int32_t sentinel) {
   GG_TRY(g->counters.ensure(GASengine::kCounterWords * sizeof(unsigned long long)));
   auto *c = g->counters.as<unsigned long long>();
   GG_HIP(GASengine::slot_resets_async(c, {{32, 0ull}, {33, 0ull}}, g->stream));
   hipLaunchKernelGGL(k_reach, dim3(GASengine::grid_for(g->n, 256, 2048)), dim3(256), 0, g->stream, result, sentinel,
                      g->dg.row_offsets, g->n, c + 32);
   GG_HIP(hipGetLastError());
+  return GG_OK;
+}
+
+int reach_stats(gg_graph *g, const int32_t *result, int32_t sentinel, gg_stats *st) {
+  if (!st) return GG_OK;
+  GG_TRY(reach_launch(g, result, sentinel));
   GG_TRY(read_counters(g, 34));
   st->reached_vertices = g->h64()[32];
   st->reached_edges = g->h64()[33];
   return GG_OK;
 }
 
-int read_counters(gg_graph *g, int count, uint64_t min_mask) {
+int copy_counters_async(gg_graph *g, int count) {
   using GASengine::kCounterSlots;
   using GASengine::kStripes;
   GG_CHECK_ARG(count > 0 && count <= kCounterSlots, "counter range");
   uint64_t *raw = reinterpret_cast<uint64_t *>(g->h64() + kCounterSlots);
   GG_HIP(hipMemcpy2DAsync(raw, count * sizeof(uint64_t), g->counters.p, kCounterSlots * sizeof(uint64_t),
                           count * sizeof(uint64_t), kStripes, hipMemcpyDeviceToHost, g->stream));
+  return GG_OK;
+}
+
+int read_counters(gg_graph *g, int count, uint64_t min_mask) {
+  GG_TRY(copy_counters_async(g, count));
   GG_HIP(hipStreamSynchronize(g->stream));
+  fold_counters(g, count, min_mask);
+  return GG_OK;
+}
+
+void fold_counters(gg_graph *g, int count, uint64_t min_mask) {
+  using GASengine::kCounterSlots;
+  using GASengine::kStripes;
+  const uint64_t *raw = reinterpret_cast<const uint64_t *>(g->h64() + kCounterSlots);
   for (int k = 0; k < count; k++) {
     const bool is_min = (min_mask >> k) & 1u;
     uint64_t t = raw[k];
@@ -89,7 +107,6 @@ int read_counters(gg_graph *g, int count, uint64_t min_mask) {
     }
     g->h64()[k] = (int64_t)t;
   }
-  return GG_OK;
 }
 
 void finish_profile(gg_graph *g) {

@@ -236,11 +236,19 @@ int copy_out(gg_graph *g, void *dst, const void *src, size_t bytes, uint32_t fla
 // reached vertices / reached-edge (degree) sums for a result array: counts
 // entries != sentinel.  Synchronises the stream.
 int reach_stats(gg_graph *g, const int32_t *result, int32_t sentinel, gg_stats *st);
+// The same count enqueued only: counters 32 (vertices) and 33 (degree sum)
+// hold it once the stream reaches that point.
+int reach_launch(gg_graph *g, const int32_t *result, int32_t sentinel);
 // Copy counters [0, count) of the graph's striped counter array
 // (GASengine::kStripes stripes of kCounterSlots words) to h64()[0, count),
 // each folded over its stripes: a sum, or a minimum for the bits set in
 // min_mask.  Synchronises the stream.
 int read_counters(gg_graph *g, int count, uint64_t min_mask = 0);
+// read_counters in two halves: enqueue the copy of the raw stripes into the
+// pinned area (no synchronisation), and, once the stream has passed it,
+// fold them into h64()[0, count).
+int copy_counters_async(gg_graph *g, int count);
+void fold_counters(gg_graph *g, int count, uint64_t min_mask = 0);
 // Finish a run: resolve profiler events.
 void finish_profile(gg_graph *g);
 // Frontier queues, scan/partition buffers and counters of the specialised

@@ -527,6 +527,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   };
   PushOp push_op{g->dg.row_offsets, g->dg.column_indices, static_cast<const int32_t *>(g->dg.edge_values), dist,
                  dirty, ctr, {}};
+  bool device_tail = false;  // the device-driven loop read the counters and reach statistics itself
   if (!g->prof.enabled) {
     // Device-driven: a replay of a captured launch
     // sequence (collect, k_sssp_ctrl (which also resets the collect's
@@ -580,11 +581,17 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     // graph took (a repeated query then needs one replay and no idle
     // iterations); later replays run kLoopIters.
     int iters = g->sssp_steps_hint > 0 ? std::min(g->sssp_steps_hint, kLoopItersMax) : kLoopIters;
+    // After each replay, one synchronisation reads the loop state, the
+    // counters and (when asked for) the reach statistics, enqueued on the
+    // chance that the replay ends the run (otherwise they are dropped).
     for (;;) {
       hipGraphExec_t exec = nullptr;
       GG_TRY(loop_exec(iters, &exec));
       GG_HIP(hipGraphLaunch(exec, s));
+      GG_HIP(hipEventRecord(e1, s));
+      if (stats) GG_TRY(reach_launch(g, dist, kInf));
       GG_HIP(hipMemcpyAsync(g->h_pinned, ctl, sizeof(SsspCtl), hipMemcpyDeviceToHost, s));
+      GG_TRY(copy_counters_async(g, 34));
       GG_HIP(hipStreamSynchronize(s));
       SsspCtl h;
       std::memcpy(&h, g->h_pinned, sizeof(h));
@@ -594,6 +601,8 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
         edges = h.edges;
         bytes = h.bytes;
         g->sssp_steps_hint = h.steps;
+        fold_counters(g, 34, 1ull << 17);
+        device_tail = true;
         break;
       }
       iters = kLoopIters;
@@ -647,8 +656,10 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     }
     iterations++;
   }
-  GG_HIP(hipEventRecord(e1, s));
-  GG_TRY(read_counters(g, 8));
+  if (!device_tail) {
+    GG_HIP(hipEventRecord(e1, s));
+    GG_TRY(read_counters(g, 8));
+  }
   const int64_t improved = g->h64()[4];
   bytes += 8.0 * improved;  // dist + dirty writes of every improvement
   float ms = 0.f;
@@ -658,7 +669,12 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   finish_profile(g);
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
-    GG_TRY(reach_stats(g, dist, kInf, stats));
+    if (device_tail) {
+      stats->reached_vertices = g->h64()[32];
+      stats->reached_edges = g->h64()[33];
+    } else {
+      GG_TRY(reach_stats(g, dist, kInf, stats));
+    }
     stats->iterations = iterations;
     stats->pull_iterations = pulls;
     stats->edges_inspected = edges;
