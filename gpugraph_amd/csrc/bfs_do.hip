@@ -253,6 +253,7 @@ struct BfsCtl {
   long long nf, prev_nf, mf, mu, reached, reached_edges, examined;
   double bytes;
 };
+static_assert(sizeof(BfsCtl) % 8 == 0, "read back in 8-byte words");
 
 // One thread, after k_bfs_init (counters[1] = out-degree of the source).
 __global__ void k_bfs_ctl_init(BfsCtl *c, const unsigned long long *__restrict__ ctr, int64_t m, double bytes0) {
@@ -366,8 +367,7 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
   LbsPhaseTimer lbs_timer(g->prof, "bfs", "top_down");
 
   hipEvent_t e0, e1;
-  GG_HIP(hipEventCreate(&e0));
-  GG_HIP(hipEventCreate(&e1));
+  GG_HIP(g->bfs_ev.get(&e0, &e1));
   GG_HIP(hipEventRecord(e0, s));
   {
     TimedRegion t(g->prof, "bfs.init");
@@ -440,10 +440,10 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
     for (;;) {
       GG_HIP(hipGraphLaunch(g->bfs_loop.exec, s));
       GG_HIP(hipEventRecord(e1, s));  // the last replay's record times the run
-      GG_HIP(hipMemcpyAsync(g->h_pinned, ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, s));
+      GG_TRY(counters_to_host_async(g, 0, 0, ctl, sizeof(BfsCtl)));
       GG_HIP(hipStreamSynchronize(s));
       BfsCtl h;
-      std::memcpy(&h, g->h_pinned, sizeof(h));
+      std::memcpy(&h, g->h64() + kCounterSlots, sizeof(h));
       if (h.done) {
         levels = h.level;
         pull_levels = h.pull_levels;
@@ -530,8 +530,6 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
   }
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   finish_profile(g);
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));

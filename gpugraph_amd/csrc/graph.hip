@@ -77,36 +77,46 @@ int reach_stats(gg_graph *g, const int32_t *result, int32_t sentinel, gg_stats *
   return GG_OK;
 }
 
-int copy_counters_async(gg_graph *g, int count) {
+// One workgroup of kCounterSlots threads: counter k < count folded over its
+// stripes into host[k] (a sum, or a minimum for the bits of min_mask), and
+// `words` 8-byte words of `extra` into host[kCounterSlots + j].  `host` is
+// the graph's pinned area, written by the kernel itself: no copy-engine
+// transfer per read-back.
+__global__ void k_counters_to_host(const unsigned long long *__restrict__ ctr, int count, uint64_t min_mask,
+                                   const int64_t *__restrict__ extra, int words, int64_t *host) {
   using GASengine::kCounterSlots;
   using GASengine::kStripes;
-  GG_CHECK_ARG(count > 0 && count <= kCounterSlots, "counter range");
-  uint64_t *raw = reinterpret_cast<uint64_t *>(g->h64() + kCounterSlots);
-  GG_HIP(hipMemcpy2DAsync(raw, count * sizeof(uint64_t), g->counters.p, kCounterSlots * sizeof(uint64_t),
-                          count * sizeof(uint64_t), kStripes, hipMemcpyDeviceToHost, g->stream));
+  const int t = threadIdx.x;
+  if (t < count) {
+    const bool is_min = (min_mask >> t) & 1u;
+    unsigned long long v = ctr[t];
+    for (int q = 1; q < kStripes; q++) {
+      const unsigned long long x = ctr[(size_t)q * kCounterSlots + t];
+      v = is_min ? (x < v ? x : v) : v + x;
+    }
+    host[t] = (int64_t)v;
+  }
+  for (int j = t; j < words; j += blockDim.x) host[kCounterSlots + j] = extra[j];
+}
+
+int counters_to_host_async(gg_graph *g, int count, uint64_t min_mask, const void *extra, size_t extra_bytes) {
+  using GASengine::kCounterSlots;
+  GG_CHECK_ARG(count >= 0 && count <= kCounterSlots, "counter range");
+  GG_CHECK_ARG(extra_bytes % 8 == 0 && extra_bytes / 8 <= (size_t)GASengine::kCounterWords &&
+                   (extra_bytes == 0 || (extra && reinterpret_cast<uintptr_t>(extra) % 8 == 0)),
+               "read-back block");
+  if (count > 0) GG_TRY(g->counters.ensure(GASengine::kCounterWords * sizeof(unsigned long long)));
+  hipLaunchKernelGGL(k_counters_to_host, dim3(1), dim3(kCounterSlots), 0, g->stream,
+                     g->counters.as<unsigned long long>(), count, min_mask, static_cast<const int64_t *>(extra),
+                     (int)(extra_bytes / 8), g->h_dev);
+  GG_HIP(hipGetLastError());
   return GG_OK;
 }
 
 int read_counters(gg_graph *g, int count, uint64_t min_mask) {
-  GG_TRY(copy_counters_async(g, count));
+  GG_TRY(counters_to_host_async(g, count, min_mask, nullptr, 0));
   GG_HIP(hipStreamSynchronize(g->stream));
-  fold_counters(g, count, min_mask);
   return GG_OK;
-}
-
-void fold_counters(gg_graph *g, int count, uint64_t min_mask) {
-  using GASengine::kCounterSlots;
-  using GASengine::kStripes;
-  const uint64_t *raw = reinterpret_cast<const uint64_t *>(g->h64() + kCounterSlots);
-  for (int k = 0; k < count; k++) {
-    const bool is_min = (min_mask >> k) & 1u;
-    uint64_t t = raw[k];
-    for (int q = 1; q < kStripes; q++) {
-      const uint64_t v = raw[(size_t)q * count + k];
-      t = is_min ? (v < t ? v : t) : t + v;
-    }
-    g->h64()[k] = (int64_t)t;
-  }
 }
 
 void finish_profile(gg_graph *g) {
@@ -384,6 +394,7 @@ int gg_graph_create(int32_t n, int64_t m, const int32_t *row_offsets, const int3
   g->value_dtype = value_dtype;
   gg_default_options(&g->opt);
   GG_HIP(hipHostMalloc((void **)&g->h_pinned, (GASengine::kCounterSlots + GASengine::kCounterWords) * sizeof(int64_t)));
+  GG_HIP(hipHostGetDevicePointer((void **)&g->h_dev, g->h_pinned, 0));
 
   const hipMemcpyKind kind = dev_in ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   GG_TRY(g->ro.ensure(4 * ((size_t)n + 1)));

@@ -123,6 +123,29 @@ struct CapturedLoop {
   }
 };
 
+// Two events timing a run, created on first use and kept with the graph
+// (creating and destroying them on every call costs host time per query).
+struct RunEvents {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  RunEvents() = default;
+  RunEvents(const RunEvents &) = delete;
+  RunEvents &operator=(const RunEvents &) = delete;
+  ~RunEvents() {
+    for (hipEvent_t x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+  hipError_t get(hipEvent_t *e0, hipEvent_t *e1) {
+    for (hipEvent_t &x : e)
+      if (!x) {
+        const hipError_t r = hipEventCreate(&x);
+        if (r != hipSuccess) return r;
+      }
+    *e0 = e[0];
+    *e1 = e[1];
+    return hipSuccess;
+  }
+};
+
 // Scoped helper for kernels launched outside the generic enactor.
 struct TimedRegion {
   EventProfiler &p;
@@ -188,7 +211,8 @@ struct gg_graph {
   int32_t sl_mid_base[2][9] = {}, sl_large_base[2][9] = {};
   bool sl_ready[2] = {false, false};
   int64_t sl_val_version[2] = {-1, -1};
-  int32_t *h_pinned = nullptr;  // pinned readback area (64 x int64, then the raw stripes of read_counters)
+  int32_t *h_pinned = nullptr;  // pinned readback area (64 x int64 counters, then a block of read-back words)
+  int64_t *h_dev = nullptr;     // the same area as kernels address it
   bool pr_cache_valid = false;
 
   // Graph-derived caches: the static merge-path partition of the CSC (pull
@@ -216,6 +240,7 @@ struct gg_graph {
 
   // Device-driven BFS / SSSP loops: captured launch sequences and their
   // device-side loop state.
+  gg::RunEvents bfs_ev, sssp_ev;
   gg::CapturedLoop bfs_loop;
   gg::CapturedLoop sssp_loops[17];  // by iterations per replay (1..16)
   int sssp_steps_hint = 0;          // loop iterations the last device-driven SSSP took
@@ -244,11 +269,11 @@ int reach_launch(gg_graph *g, const int32_t *result, int32_t sentinel);
 // each folded over its stripes: a sum, or a minimum for the bits set in
 // min_mask.  Synchronises the stream.
 int read_counters(gg_graph *g, int count, uint64_t min_mask = 0);
-// read_counters in two halves: enqueue the copy of the raw stripes into the
-// pinned area (no synchronisation), and, once the stream has passed it,
-// fold them into h64()[0, count).
-int copy_counters_async(gg_graph *g, int count);
-void fold_counters(gg_graph *g, int count, uint64_t min_mask = 0);
+// read_counters without the synchronisation, plus `extra_bytes` (a multiple
+// of 8, 8-byte aligned) copied from device memory `extra` to
+// h64() + kCounterSlots: one kernel writes both into the pinned area, valid
+// once the stream has passed it.
+int counters_to_host_async(gg_graph *g, int count, uint64_t min_mask, const void *extra, size_t extra_bytes);
 // Finish a run: resolve profiler events.
 void finish_profile(gg_graph *g);
 // Frontier queues, scan/partition buffers and counters of the specialised

@@ -273,6 +273,7 @@ struct SsspCtl {
   long long edges;    // edges inspected
   double bytes;       // algorithmic bytes (model in the header comment)
 };
+static_assert(sizeof(SsspCtl) % 8 == 0, "read back in 8-byte words");
 
 __global__ void k_sssp_ctl_init(SsspCtl *c, unsigned long long *ctr, int32_t bound, double bytes0) {
   sssp_reset_collect(ctr);
@@ -494,8 +495,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   LbsPhaseTimer lbs_timer(g->prof, "sssp", "push");
 
   hipEvent_t e0, e1;
-  GG_HIP(hipEventCreate(&e0));
-  GG_HIP(hipEventCreate(&e1));
+  GG_HIP(g->sssp_ev.get(&e0, &e1));
   GG_HIP(hipEventRecord(e0, s));
   {
     TimedRegion t(g->prof, "sssp.init");
@@ -590,18 +590,16 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       GG_HIP(hipGraphLaunch(exec, s));
       GG_HIP(hipEventRecord(e1, s));
       if (stats) GG_TRY(reach_launch(g, dist, kInf));
-      GG_HIP(hipMemcpyAsync(g->h_pinned, ctl, sizeof(SsspCtl), hipMemcpyDeviceToHost, s));
-      GG_TRY(copy_counters_async(g, 34));
+      GG_TRY(counters_to_host_async(g, 34, 1ull << 17, ctl, sizeof(SsspCtl)));
       GG_HIP(hipStreamSynchronize(s));
       SsspCtl h;
-      std::memcpy(&h, g->h_pinned, sizeof(h));
+      std::memcpy(&h, g->h64() + kCounterSlots, sizeof(h));
       if (h.done) {
         iterations = h.iters;
         pulls = h.pulls;
         edges = h.edges;
         bytes = h.bytes;
         g->sssp_steps_hint = h.steps;
-        fold_counters(g, 34, 1ull << 17);
         device_tail = true;
         break;
       }
@@ -664,8 +662,6 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   bytes += 8.0 * improved;  // dist + dirty writes of every improvement
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   finish_profile(g);
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
