@@ -63,11 +63,19 @@ struct Tally {
   }
 };
 
-struct PushOp {
+// PACKED: the CSR slot's target and weight come from one word of the packed
+// in-edge array, which is the CSR's own slot order on an undirected graph
+// whose in-edge weights are its out-edge weights: one coalesced 4-byte load
+// per edge instead of two.
+template <bool PACKED>
+struct PushOpT {
   static constexpr bool kItemValues = true;
   const int32_t *ro;
   const int32_t *col;
   const int32_t *w;
+  const uint32_t *pk;  // PACKED: (target << wbits) | weight per CSR slot
+  int32_t wbits;
+  uint32_t wmask;
   int32_t *dist;
   uint8_t *dirty;
   unsigned long long *counters;
@@ -86,7 +94,14 @@ struct PushOp {
     int32_t nb;
     int64_t nd;
   };
-  __device__ __forceinline__ Loads load(int32_t, int32_t dv, int32_t slot) { return {col[slot], (int64_t)dv + w[slot]}; }
+  __device__ __forceinline__ Loads load(int32_t, int32_t dv, int32_t slot) {
+    if constexpr (PACKED) {
+      const uint32_t e = pk[slot];
+      return {(int32_t)(e >> wbits), (int64_t)dv + (int32_t)(e & wmask)};
+    } else {
+      return {col[slot], (int64_t)dv + w[slot]};
+    }
+  }
   __device__ __forceinline__ bool apply(int32_t, const Loads &l, int32_t &) {
     if (l.nd >= dist[l.nb]) return false;
     const int32_t nd = (int32_t)l.nd;
@@ -101,6 +116,8 @@ struct PushOp {
   }
   __device__ __forceinline__ void finish() { tally.flush(counters); }
 };
+using PushOp = PushOpT<false>;
+using PackedPushOp = PushOpT<true>;
 
 // Relaxes every in-edge, not only those of frontier vertices: the pass reads
 // all in-edges anyway, dist[u] + w is a path length whatever u's state, and
@@ -525,8 +542,20 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       hipLaunchKernelGGL((k_pull_staged<PullOp, int32_t>), dim3(2048), dim3(kExpandBlock), 0, s, pull_op,
                          g->dg.column_offsets, n, m, pparts, skip);
   };
-  PushOp push_op{g->dg.row_offsets, g->dg.column_indices, static_cast<const int32_t *>(g->dg.edge_values), dist,
-                 dirty, ctr, {}};
+  const int32_t *w_out = static_cast<const int32_t *>(g->dg.edge_values);
+  PushOp push_op{g->dg.row_offsets, g->dg.column_indices, w_out, nullptr, 0, 0u, dist, dirty, ctr, {}};
+  // The push reads the packed words too when they are the CSR's own slots.
+  const bool push_packed = pk && g->dg.row_indices == g->dg.column_indices && wcv == g->dg.edge_values;
+  PackedPushOp push_pk{g->dg.row_offsets, g->dg.column_indices, w_out, pk, pk_bits,
+                       pk_bits ? (uint32_t)((1ull << pk_bits) - 1) : 0u, dist, dirty, ctr, {}};
+  auto launch_push = [&](const int32_t *fsize, int64_t nf_host, auto &&timer) {
+    if (push_packed)
+      lbs_launch(push_pk, false, queue, fsize, nf_host, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(), qsize + 1,
+                 s, timer);
+    else
+      lbs_launch(push_op, false, queue, fsize, nf_host, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(), qsize + 1,
+                 s, timer);
+  };
   bool device_tail = false;  // the device-driven loop read the counters and reach statistics itself
   if (!g->prof.enabled) {
     // Device-driven: a replay of a captured launch
@@ -558,8 +587,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
                              0, queue, qsize, ctr, (const SsspCtl *)ctl);
           hipLaunchKernelGGL(k_sssp_ctrl, dim3(1), dim3(1), 0, s, ctl, ctr, n, m, delta, pull_alpha, g->opt.iter_num,
                              pull_bytes);
-          lbs_launch(push_op, false, queue, &ctl->push_size, n, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(),
-                     qsize + 1, s, [](int) {});
+          launch_push(&ctl->push_size, n, [](int) {});
           if (pull_alpha > 0) launch_pull((const int32_t *)&ctl->pull_skip);
         }
         const hipError_t le = hipGetLastError();
@@ -643,8 +671,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       pulls++;
     } else {
       // PushOp never enqueues: the output queue stays empty.
-      lbs_launch(push_op, false, queue, qsize, nf, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(), qsize + 1, s,
-                 lbs_timer);
+      launch_push(qsize, nf, lbs_timer);
       GG_HIP(hipGetLastError());
       const double pb = 16.0 * nf + 12.0 * degsum;
       g->prof.add_launch_bytes("sssp.lbs_count", 16.0 * nf);
