@@ -191,7 +191,7 @@ std::vector<Knob> knob_table() {
       {kFieldWave, GG_TUNE_BFS | GG_TUNE_CC | GG_TUNE_GAS, {32, 64, 128, 256, 512}},
       {kFieldBlock, GG_TUNE_CC | GG_TUNE_GAS, {512, 1024, 2048, 4096, 16384}},
       {kFieldThreshold, GG_TUNE_CC | GG_TUNE_GAS, {0, 1000, 10000, 100000, INT_MAX}},
-      {kFieldDelta, GG_TUNE_SSSP, {16, 32, 64, 128, 256, 512, 1024}},
+      {kFieldDelta, GG_TUNE_SSSP, {12, 16, 24, 32, 64, 128, 256, 512, 1024}},
       {kFieldPull, GG_TUNE_SSSP, {0, 2, 4, 8, 16}},
       {kFieldSlices, GG_TUNE_PAGERANK, {0, 1}},
   };
