@@ -16,7 +16,9 @@
 //            vertices left: bound jumps to the bucket of that distance.
 //   push     frontier edges <= m / pull_alpha: LBS expand over the queue,
 //            nd = dist[v] + w, plain pre-check then atomicMin(dist[nb]); an
-//            improvement stores dirty[nb] = 1 (no atomic, no enqueue).
+//            improvement stores dirty[nb] = 1 (no atomic, no enqueue).  On an
+//            undirected graph whose in-edges are packed (below) the push
+//            reads the same packed words: they are the CSR's own slots.
 //   pull     otherwise: merge-path pass over every vertex's in-edges, min
 //            of dist[u] + w over all in-neighbours u reduced in LDS (each
 //            thread's edges loaded back to back), written with a plain store
@@ -28,8 +30,8 @@
 //
 // Byte model: collect n + 4(dirty) + 4F; push 16F + 12E + 8N;
 // pull 4n + 12m (offsets; source, weight and distance per in-edge: the same
-// 12 bytes per inspected edge as the push, whichever in-edge layout is read;
-// the packed layout moves 8 of them).
+// 12 bytes per inspected edge as the push).  Both count 12 bytes per edge
+// whichever layout is read; the packed words move 8 of them.
 #include <hip/hip_runtime.h>
 
 #include <hipcub/hipcub.hpp>
