@@ -109,6 +109,10 @@ class EventProfiler : public GASengine::KernelTimer {
 // An instantiated capture of a device-driven iteration loop, cached on a
 // graph handle with the kernel arguments it was captured with (`key`); a
 // call whose arguments differ captures again.
+// Longest device-driven SSSP replay that is captured (iterations); the graph
+// keeps one captured replay per length 1..kSsspReplayMax.
+constexpr int kSsspReplayMax = 16;
+
 struct CapturedLoop {
   hipGraphExec_t exec = nullptr;
   std::vector<uint64_t> key;
@@ -242,7 +246,7 @@ struct gg_graph {
   // device-side loop state.
   gg::RunEvents bfs_ev, sssp_ev;
   gg::CapturedLoop bfs_loop;
-  gg::CapturedLoop sssp_loops[17];  // by iterations per replay (1..16)
+  gg::CapturedLoop sssp_loops[gg::kSsspReplayMax + 1];  // by iterations per replay
   int sssp_steps_hint = 0;          // loop iterations the last device-driven SSSP took
   gg::DevBuf loop_ctl;
 

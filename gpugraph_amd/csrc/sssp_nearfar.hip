@@ -47,8 +47,8 @@ namespace {
 
 using namespace GASengine;
 constexpr int32_t kInf = 100000000;
-constexpr int kLoopIters = 12;     // iterations per replay of the captured loop
-constexpr int kLoopItersMax = 16;  // longest replay captured (internal.h sizes the cache)
+constexpr int kLoopIters = 12;  // iterations per replay after the first
+static_assert(kLoopIters <= kSsspReplayMax, "replay lengths are cached up to kSsspReplayMax");
 
 // counters (striped, GASengine::block_counter_add): [2] LBS edge counter,
 // [4] improvements, [16] dirty left, [17] min dist left (minimum over the
@@ -610,7 +610,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     // The first replay runs as many iterations as the previous run on this
     // graph took (a repeated query then needs one replay and no idle
     // iterations); later replays run kLoopIters.
-    int iters = g->sssp_steps_hint > 0 ? std::min(g->sssp_steps_hint, kLoopItersMax) : kLoopIters;
+    int iters = g->sssp_steps_hint > 0 ? std::min(g->sssp_steps_hint, kSsspReplayMax) : kLoopIters;
     // After each replay, one synchronisation reads the loop state, the
     // counters and (when asked for) the reach statistics, enqueued on the
     // chance that the replay ends the run (otherwise they are dropped).
