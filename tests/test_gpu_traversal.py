@@ -179,6 +179,11 @@ def test_sssp_collect_alignment_and_tail(gpu_available, offset):
                     g.sssp(src, flags=fl, out=out.data_ptr())
                     g.synchronize()
                     np.testing.assert_array_equal(out.cpu().numpy(), exp[src], err_msg=str((offset, opts, src, fl)))
+                    # the C ABI allows a NULL statistics pointer
+                    out.fill_(-1)
+                    _abi.check(g.lib.gg_sssp(g.h, src, out.data_ptr(), fl, None), "gg_sssp")
+                    g.synchronize()
+                    np.testing.assert_array_equal(out.cpu().numpy(), exp[src], err_msg=str((offset, opts, src, fl, "no stats")))
 
 
 def test_sssp_replay_sizes_follow_previous_runs(gpu_available):
