@@ -408,7 +408,7 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
       g->bfs_loop.reset();
       hipGraph_t graph = nullptr;
       GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      const int conv_grid = grid_for((int64_t)n, 256, 4096);
+      const int conv_grid = grid_for((int64_t)n, 256, 2048);  // strides by waves; idle at most levels
       for (int lv = 0; lv < kLoopLevels; lv++) {
         const int c = lv & 1, x = c ^ 1;
         hipLaunchKernelGGL(k_bfs_ctrl, dim3(1), dim3(64), 0, s, ctl, ctr, qsz[c], qsz[x], n, alpha, beta,

@@ -67,6 +67,10 @@ struct StagedEdgeLoads<Op, true> {
 constexpr int kCountBlock = 256;
 constexpr int kCountIpt = 4;
 constexpr int kCountTile = kCountBlock * kCountIpt;  // items per scan partial
+// k_lbs_count strides over its tiles; captured loops launch it at its largest
+// size every iteration, and a gated launch of 2048 workgroups costs less than
+// one of 4096.
+constexpr int kCountGridMax = 2048;
 constexpr int kScanBlock = 1024;
 constexpr int kExpandBlock = 256;
 constexpr int kExpandVt = 4;  // default 1024-position tiles: ~21 KB LDS, 7 workgroups per CU
@@ -692,10 +696,10 @@ inline void lbs_launch(const Op &op, bool prologue, const int32_t *frontier, con
                        Timer &&timer) {
   timer(0);
   if (prologue)
-    hipLaunchKernelGGL((k_lbs_count<Op, true>), dim3(grid_for(nf_host, kCountTile)), dim3(kCountBlock), 0, s, op,
+    hipLaunchKernelGGL((k_lbs_count<Op, true>), dim3(grid_for(nf_host, kCountTile, kCountGridMax)), dim3(kCountBlock), 0, s, op,
                        frontier, fsize, b.scan_local, b.partials);
   else
-    hipLaunchKernelGGL((k_lbs_count<Op, false>), dim3(grid_for(nf_host, kCountTile)), dim3(kCountBlock), 0, s, op,
+    hipLaunchKernelGGL((k_lbs_count<Op, false>), dim3(grid_for(nf_host, kCountTile, kCountGridMax)), dim3(kCountBlock), 0, s, op,
                        frontier, fsize, b.scan_local, b.partials);
   timer(1);
   hipLaunchKernelGGL(k_lbs_scan<kScanBlock>, dim3(1), dim3(kScanBlock), 0, s, fsize, b.partials, b.info,
