@@ -61,15 +61,6 @@ struct BfsPushOp {
   __device__ __forceinline__ void finish() { block_counter_add<kExpandBlock>(&counters[1], mf); }
 };
 
-__global__ void k_bfs_init(int32_t *depth, uint32_t *visited, int32_t *queue, int32_t *qsize, const int32_t *ro,
-                           int32_t src, unsigned long long *counters) {
-  depth[src] = 0;
-  visited[src >> 5] = 1u << (src & 31);
-  queue[0] = src;
-  *qsize = 1;
-  counters[1] = (unsigned long long)(ro[src + 1] - ro[src]);
-}
-
 __device__ __forceinline__ uint32_t fbit(const uint32_t *__restrict__ b, int32_t u) {
   return (b[u >> 5] >> (u & 31)) & 1u;
 }
@@ -255,25 +246,42 @@ struct BfsCtl {
 };
 static_assert(sizeof(BfsCtl) % 8 == 0, "read back in 8-byte words");
 
-// One thread, after k_bfs_init (counters[1] = out-degree of the source).
-__global__ void k_bfs_ctl_init(BfsCtl *c, const unsigned long long *__restrict__ ctr, int64_t m, double bytes0) {
-  const long long mf = (long long)ctr[1];
-  c->level = 0;
-  c->bottom_up = 0;
-  c->q2b = c->b2q = 0;
-  c->bu_skip = 1;
-  c->push_size = 0;
-  c->done = 0;
-  c->pull_levels = 0;
-  c->pending = 0;
-  c->nf = 1;
-  c->prev_nf = 0;
-  c->mf = mf;
-  c->mu = m - mf;
-  c->reached = 1;
-  c->reached_edges = mf;
-  c->examined = 0;
-  c->bytes = bytes0;
+// The start of a run in one launch, every element at its final initial value
+// (no fill then patch, so no ordering between threads is needed): depth = -1
+// except depth[src] = 0; visited clear except src's bit; the counters zero
+// except [1] = out-degree of the source and the first queue size (low half
+// of slot 8) = 1; queue[0] = src; and, for the device-driven loop, its state
+// block.
+__global__ void k_bfs_start(int32_t *depth, int32_t n, uint32_t *visited, int32_t words, unsigned long long *ctr,
+                            int32_t *queue, const int32_t *ro, int32_t src, BfsCtl *c, int64_t m, double bytes0) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long mf = (long long)(ro[src + 1] - ro[src]);
+  for (int64_t i = t0; i < n; i += stride) depth[i] = i == src ? 0 : -1;
+  for (int64_t i = t0; i < words; i += stride) visited[i] = i == (src >> 5) ? 1u << (src & 31) : 0u;
+  for (int64_t i = t0; i < kCounterWords; i += stride)
+    ctr[i] = i == 1 ? (unsigned long long)mf : i == 8 ? 1ull : 0ull;  // slot 8: int32 queue sizes {1, 0}
+  if (t0 == 0) {
+    queue[0] = src;
+    if (c) {
+      c->level = 0;
+      c->bottom_up = 0;
+      c->q2b = c->b2q = 0;
+      c->bu_skip = 1;
+      c->push_size = 0;
+      c->done = 0;
+      c->pull_levels = 0;
+      c->pending = 0;
+      c->nf = 1;
+      c->prev_nf = 0;
+      c->mf = mf;
+      c->mu = m - mf;
+      c->reached = 1;
+      c->reached_edges = mf;
+      c->examined = 0;
+      c->bytes = bytes0;
+    }
+  }
 }
 
 // qsize_cur: the queue the level in flight wrote (its next queue) and the
@@ -369,13 +377,16 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
   hipEvent_t e0, e1;
   GG_HIP(g->bfs_ev.get(&e0, &e1));
   GG_HIP(hipEventRecord(e0, s));
+  BfsCtl *ctl = nullptr;  // state block of the device-driven loop
+  if (!g->prof.enabled) {
+    GG_TRY(g->loop_ctl.ensure(sizeof(BfsCtl)));
+    ctl = g->loop_ctl.as<BfsCtl>();
+  }
   {
     TimedRegion t(g->prof, "bfs.init");
-    GG_HIP(fill_async<int32_t>(depth, -1, n, s));
-    GG_HIP(fill_async<uint32_t>(visited, 0u, words, s));
-    GG_HIP(fill_async<unsigned long long>(ctr, 0ull, kCounterWords, s));
-    hipLaunchKernelGGL(k_bfs_init, dim3(1), dim3(1), 0, s, depth, visited, queue[0], qsz[0], g->dg.row_offsets, src,
-                       ctr);
+    hipLaunchKernelGGL(k_bfs_start, dim3(grid_for(std::max<int64_t>(n, kCounterWords), 256, 8192)), dim3(256), 0, s,
+                       depth, n, visited, words, ctr, queue[0], g->dg.row_offsets, src, ctl, g->m,
+                       4.0 * n + words * 4.0);
     GG_HIP(hipGetLastError());
   }
   g->prof.add_bytes("bfs.init", 4.0 * n + words * 4.0);
@@ -395,10 +406,6 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
     // the LBS top-down (over ctl->push_size), each running or returning on
     // the controller's flags; a last k_bfs_ctrl closes the replay's final
     // level.  Per-kernel events (GG_INSTRUMENT) need the host loop below.
-    GG_TRY(g->loop_ctl.ensure(sizeof(BfsCtl)));
-    BfsCtl *ctl = g->loop_ctl.as<BfsCtl>();
-    hipLaunchKernelGGL(k_bfs_ctl_init, dim3(1), dim3(1), 0, s, ctl, ctr, g->m, bytes);
-    GG_HIP(hipGetLastError());
     const std::vector<uint64_t> key = {
         (uint64_t)depth, (uint64_t)visited, (uint64_t)fbits[0], (uint64_t)fbits[1], (uint64_t)queue[0],
         (uint64_t)queue[1], (uint64_t)ctr, (uint64_t)ctl, (uint64_t)lb.scan_local, (uint64_t)lb.partials,

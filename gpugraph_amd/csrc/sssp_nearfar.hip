@@ -166,11 +166,6 @@ struct PullOp {
   __device__ __forceinline__ void finish() { tally.flush(counters); }
 };
 
-__global__ void k_sssp_init(int32_t *dist, uint8_t *dirty, int32_t src) {
-  dist[src] = 0;
-  dirty[src] = 1;
-}
-
 // The pull over packed in-edges: one 32-bit word per edge holds the source
 // (high bits) and the weight (low wbits bits), so an edge is one load of the
 // stream instead of two.
@@ -293,19 +288,6 @@ struct SsspCtl {
   double bytes;       // algorithmic bytes (model in the header comment)
 };
 static_assert(sizeof(SsspCtl) % 8 == 0, "read back in 8-byte words");
-
-__global__ void k_sssp_ctl_init(SsspCtl *c, unsigned long long *ctr, int32_t bound, double bytes0) {
-  sssp_reset_collect(ctr);
-  c->bound = bound;
-  c->push_size = 0;
-  c->pull_skip = 1;
-  c->done = 0;
-  c->iters = 0;
-  c->pulls = 0;
-  c->steps = 0;
-  c->edges = 0;
-  c->bytes = bytes0;
-}
 
 __global__ void k_sssp_ctrl(SsspCtl *c, unsigned long long *__restrict__ ctr, int32_t n, int64_t m, int64_t delta,
                             int64_t pull_alpha, int32_t max_iter, double pull_bytes) {
@@ -472,6 +454,40 @@ k_sssp_collect(uint8_t *__restrict__ dirty, const int32_t *__restrict__ dist, co
   }
 }
 
+// The start of a run in one launch: every element gets its final initial
+// value (no fill then patch, so no ordering between threads is needed):
+// dist = kInf except dist[src] = 0; the dirty bytes (16 per thread, the array
+// padded to n16 x 16 bytes) clear except src's; the counters zero except the
+// collect's minimum slot, at its identity in every stripe; and, for the
+// device-driven loop, its state block.
+__global__ void k_sssp_start(int32_t *dist, int32_t n, uint4 *dirty16, int64_t n16, unsigned long long *ctr,
+                             int32_t src, SsspCtl *ctl, int32_t bound, double bytes0) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t i = t0; i < n; i += stride) dist[i] = i == src ? 0 : kInf;
+  const int64_t src16 = src / kCollectPer;
+  const int sb = src % kCollectPer;  // byte of src within its 16
+  const uint32_t bit = 1u << (8 * (sb & 3));
+  for (int64_t i = t0; i < n16; i += stride) {
+    const bool here = i == src16;
+    dirty16[i] = make_uint4(here && (sb >> 2) == 0 ? bit : 0u, here && (sb >> 2) == 1 ? bit : 0u,
+                            here && (sb >> 2) == 2 ? bit : 0u, here && (sb >> 2) == 3 ? bit : 0u);
+  }
+  for (int64_t i = t0; i < kCounterWords; i += stride)
+    ctr[i] = i % kCounterSlots == 17 ? (unsigned long long)INT_MAX : 0ull;
+  if (ctl && t0 == 0) {
+    ctl->bound = bound;
+    ctl->push_size = 0;
+    ctl->pull_skip = 1;
+    ctl->done = 0;
+    ctl->iters = 0;
+    ctl->pulls = 0;
+    ctl->steps = 0;
+    ctl->edges = 0;
+    ctl->bytes = bytes0;
+  }
+}
+
 }  // namespace
 
 int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, gg_stats *stats) {
@@ -516,12 +532,17 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   hipEvent_t e0, e1;
   GG_HIP(g->sssp_ev.get(&e0, &e1));
   GG_HIP(hipEventRecord(e0, s));
+  SsspCtl *ctl = nullptr;  // state block of the device-driven loop
+  if (!g->prof.enabled) {
+    GG_TRY(g->loop_ctl.ensure(sizeof(SsspCtl)));
+    ctl = g->loop_ctl.as<SsspCtl>();
+  }
   {
     TimedRegion t(g->prof, "sssp.init");
-    GG_HIP(fill_async<int32_t>(dist, kInf, n, s));
-    GG_HIP(fill_async<uint8_t>(dirty, (uint8_t)0, n_dirty, s));
-    GG_HIP(fill_async<unsigned long long>(ctr, 0ull, kCounterWords, s));
-    hipLaunchKernelGGL(k_sssp_init, dim3(1), dim3(1), 0, s, dist, dirty, src);
+    const int64_t n16 = n_dirty / kCollectPer;
+    hipLaunchKernelGGL(k_sssp_start, dim3(grid_for(std::max<int64_t>(n, kCounterWords), 256, 8192)), dim3(256), 0, s,
+                       dist, n, reinterpret_cast<uint4 *>(dirty), n16, ctr, src, ctl,
+                       (int32_t)std::min<int64_t>(delta, kInf), 5.0 * n);
     GG_HIP(hipGetLastError());
   }
   g->prof.add_bytes("sssp.init", 5.0 * n);
@@ -565,11 +586,6 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     // counters), LBS push over ctl->push_size, pull unless ctl->pull_skip);
     // the host reads the loop state once per replay.  Per-kernel events
     // (GG_INSTRUMENT) need the host loop below.
-    GG_TRY(g->loop_ctl.ensure(sizeof(SsspCtl)));
-    SsspCtl *ctl = g->loop_ctl.as<SsspCtl>();
-    hipLaunchKernelGGL(k_sssp_ctl_init, dim3(1), dim3(1), 0, s, ctl, ctr, (int32_t)std::min<int64_t>(delta, kInf),
-                       bytes);
-    GG_HIP(hipGetLastError());
     const std::vector<uint64_t> key = {
         (uint64_t)dist, (uint64_t)dirty, (uint64_t)queue, (uint64_t)ctr, (uint64_t)ctl, (uint64_t)wcv,
         (uint64_t)pparts, (uint64_t)lb.scan_local, (uint64_t)lb.partials, (uint64_t)lb.info, (uint64_t)lb.parts,

@@ -240,9 +240,12 @@ def test_device_driven_loops_match_host_loops(gpu_available, undirected):
     src = workloads.max_degree_vertex(c)
     exp_b = oracle.bfs(c.row_offsets, c.col_indices, src)
     exp_s = oracle.sssp(c.row_offsets, c.col_indices, c.values, src)
-    # BFS statistics are fixed by the graph; an SSSP pull reads distances
-    # other workgroups lower during the same pass, so its iteration count
-    # (not its result) depends on timing once pulls are on.
+    # BFS statistics are fixed by the graph.  SSSP's are not: a pull reads
+    # distances other workgroups lower during the same pass, and a push can
+    # lower the distance of a vertex of its own frontier, whose edges then go
+    # out with the old or the new value depending on which workgroup ran
+    # first (the vertex is expanded again in the first case).  The results,
+    # and the reached counts, are fixed.
     keys = {"bfs": ("iterations", "pull_iterations", "reached_vertices", "reached_edges", "edges_inspected"),
             "sssp": ("reached_vertices", "reached_edges")}
     outs = [torch.empty(c.n, dtype=torch.int32, device="cuda") for _ in range(2)]
@@ -259,8 +262,6 @@ def test_device_driven_loops_match_host_loops(gpu_available, undirected):
                         g.synchronize()
                         np.testing.assert_array_equal(out.cpu().numpy(), exp, err_msg=str((prog, opts, fl)))
                         stats[fl] = {k: st[k] for k in keys[prog]}
-                        if prog == "sssp" and g.options().pull_alpha == 0:
-                            stats[fl].update(iterations=st["iterations"], edges_inspected=st["edges_inspected"])
                     assert stats[_abi.GG_OUT_DEVICE] == stats[_abi.GG_OUT_DEVICE | _abi.GG_INSTRUMENT], (prog, opts)
         # iteration cap: both loops stop after the same number of levels / iterations
         g.set_options(**gg.default_options().as_dict())
