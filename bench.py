@@ -42,9 +42,10 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-# Kernel classes of the bench's HIP-event profile -> rows of the committed
-# rocprofv3 --pmc summary (FETCH_SIZE x2 + WRITE_SIZE x1, calibrated by a
-# 1 GiB copy), for `roofline.traffic`.
+# `roofline.traffic`: HBM bytes per launch of a kernel class, from the
+# committed rocprofv3 --pmc summary (a table row per class: FETCH_SIZE x2 +
+# WRITE_SIZE x1, calibrated by a 1 GiB copy, per launch of the instrumented
+# pass).
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "headline_rmat22_hbm_traffic_pmc.md")
 def measured_traffic(kernel_class):
     """HBM bytes per launch of a kernel class (fetch + write per launch of the
