@@ -47,6 +47,8 @@ def log(msg):
 # WRITE_SIZE x1, calibrated by a 1 GiB copy, per launch of the instrumented
 # pass).
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "headline_rmat22_hbm_traffic_pmc.md")
+
+
 def measured_traffic(kernel_class):
     """HBM bytes per launch of a kernel class (fetch + write per launch of the
     instrumented pass) from the committed counter summary, or None."""
