@@ -153,6 +153,9 @@ class Graph:
             else:
                 vals, dt = np.ascontiguousarray(csr.values, dtype=np.float32), _abi.GG_VALUES_F32
         self.value_dtype = dt
+        # Bumped by every update_values: callers that keep results computed
+        # from the values (autograd) check it before reusing the graph.
+        self.values_version = 0
         h = C.c_void_p()
         _abi.check(self.lib.gg_graph_create(self.n, self.m, _ptr(ro), _ptr(col), _ptr(vals) if vals is not None else None,
                                             dt, (_abi.GG_GRAPH_DIRECTED if directed else 0)
@@ -323,6 +326,7 @@ class Graph:
             dt = np.int32 if self.value_dtype == _abi.GG_VALUES_I32 else np.float32
             v = np.ascontiguousarray(values, dtype=dt)
             _abi.check(self.lib.gg_graph_update_values(self.h, _ptr(v), 0), "gg_graph_update_values")
+        self.values_version += 1
 
     def batch(self, sssp_sources=(), ppr_seeds=(), damping: float = 0.85, tol_l1: float = 1e-6,
               max_iter: int = 1000, flags: int = 0, dist_out=None, rank_out=None):

@@ -8,9 +8,12 @@
 
 forward() loads the raw edge weights into the resident graph (the library
 normalises each source's out-weights), runs the compensated FP32 forward and
-saves the ranks; backward() hands the saved ranks and the incoming gradient
-to the adjoint kernel, which returns dL/dw (CSR order, through the
-normalisation) and dL/dt.  Both run on the graph's stream, which is
+saves the ranks and the weights; backward() hands the saved ranks and the
+incoming gradient to the adjoint kernel, which returns dL/dw (CSR order,
+through the normalisation) and dL/dt.  The adjoint reads the transition
+weights resident in the graph, so when another forward replaced them
+between this forward and its backward, backward loads the saved weights
+again first.  Both run on the graph's stream, which is
 ordered after torch's pending work (see _join_torch_stream).
 """
 from __future__ import annotations
@@ -51,17 +54,22 @@ class PageRankFunction(torch.autograd.Function):
         ctx.adj_tol = adj_tol
         ctx.max_iter = max_iter
         ctx.has_teleport = teleport is not None
-        ctx.save_for_backward(rank)
+        ctx.values_version = graph.values_version
+        ctx.save_for_backward(rank, w)
         return rank
 
     @staticmethod
     def backward(ctx, grad_rank):
-        (rank,) = ctx.saved_tensors
+        rank, w = ctx.saved_tensors
         graph: Graph = ctx.graph
         c = grad_rank.detach().to(torch.float32).contiguous()
         gw = torch.empty(graph.m, dtype=torch.float32, device=rank.device)
         gt = torch.empty(graph.n, dtype=torch.float32, device=rank.device)
         _join_torch_stream(graph)
+        if graph.values_version != ctx.values_version:
+            # Another forward loaded other weights since this one ran.
+            graph.update_values(w, device=True)
+            ctx.values_version = graph.values_version
         graph.pagerank_adjoint(rank, c, damping=ctx.damping, tol_l1=ctx.adj_tol, max_iter=ctx.max_iter,
                                flags=_abi.GG_OUT_DEVICE, grad_w=gw, grad_t=gt)
         return gw, (gt if ctx.has_teleport else None), None, None, None, None, None

@@ -634,9 +634,9 @@ static int batch_pass(gg_graph *g, const int32_t *h_src, int32_t ns, int32_t *di
     pa.inv_n = 1.0 / (double)n;
   }
 
-  hipEvent_t e0, e1;
-  GG_HIP(hipEventCreate(&e0));
-  GG_HIP(hipEventCreate(&e1));
+  ScopedEvents ev;
+  GG_HIP(ev.create());
+  hipEvent_t e0 = ev.e0, e1 = ev.e1;
   GG_HIP(hipEventRecord(e0, s));
   {
     TimedRegion t(g->prof, "batch.init");
@@ -653,8 +653,7 @@ static int batch_pass(gg_graph *g, const int32_t *h_src, int32_t ns, int32_t *di
     GG_HIP(hipGetLastError());
   }
   // Capture kIters iterations of both parts into one graph.
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  ScopedGraph cg;
   GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int it = 0; it < kIters; it++) {
     if (ns) {
@@ -681,9 +680,9 @@ static int batch_pass(gg_graph *g, const int32_t *h_src, int32_t ns, int32_t *di
       hipLaunchKernelGGL(k_ppr_finalize, dim3(1), dim3(kPprMax), 0, s, st, np, tol, max_iter);
     }
   }
-  hipError_t cap = hipStreamEndCapture(s, &graph);
+  hipError_t cap = hipStreamEndCapture(s, &cg.graph);
   if (cap != hipSuccess) return set_error(GG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(cap));
-  GG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  GG_HIP(hipGraphInstantiate(&cg.exec, cg.graph, nullptr, nullptr, 0));
   struct H {
     int32_t sssp_iters, sssp_done;
     unsigned long long edges;
@@ -692,7 +691,7 @@ static int batch_pass(gg_graph *g, const int32_t *h_src, int32_t ns, int32_t *di
   for (;;) {
     {
       TimedRegion t(g->prof, "batch.replay");
-      GG_HIP(hipGraphLaunch(exec, s));
+      GG_HIP(hipGraphLaunch(cg.exec, s));
     }
     replays++;
     GG_HIP(hipMemcpyAsync(g->h_pinned, &st->sssp_iters, sizeof(H), hipMemcpyDeviceToHost, s));
@@ -702,8 +701,6 @@ static int batch_pass(gg_graph *g, const int32_t *h_src, int32_t ns, int32_t *di
     const int sd = g->h_pinned[1], pd = g->h_pinned[9];
     if (sd && pd) break;
   }
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
   H h;
   std::memcpy(&h, g->h_pinned, sizeof(H));
   const int32_t ppr_iters = g->h_pinned[8];
@@ -742,8 +739,6 @@ static int batch_pass(gg_graph *g, const int32_t *h_src, int32_t ns, int32_t *di
   GG_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   acc->iterations = std::max(acc->iterations, h.sssp_iters);
   acc->pull_iterations = std::max(acc->pull_iterations, ppr_iters);
   acc->edges_inspected += (int64_t)h.edges + (int64_t)ppr_iters * m;
@@ -776,6 +771,7 @@ extern "C" int gg_batch_run(gg_graph *g, const gg_batch_spec *spec, uint32_t fla
   for (int32_t i = 0; i < spec->num_ppr; i++)
     GG_CHECK_ARG(spec->ppr_seeds[i] >= 0 && spec->ppr_seeds[i] < g->n, "seed %d out of range", i);
   GG_TRY(activate(g));
+  if (spec->num_sssp) GG_TRY(require_nonnegative_weights(g, "batched SSSP"));
   g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
   gg_stats acc{};
   const int32_t max_iter = spec->max_iter > 0 ? spec->max_iter : INT_MAX;

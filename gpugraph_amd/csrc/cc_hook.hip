@@ -105,9 +105,9 @@ int cc_hooking(gg_graph *g, int32_t *label_out, uint32_t flags, gg_stats *stats)
   const int32_t *parts = nullptr;
   GG_TRY(pull_partition(g, &parts));  // merge-path tiles over (vertices, adjacency), cached per graph
   g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
-  hipEvent_t e0, e1;
-  GG_HIP(hipEventCreate(&e0));
-  GG_HIP(hipEventCreate(&e1));
+  ScopedEvents ev;
+  GG_HIP(ev.create());
+  hipEvent_t e0 = ev.e0, e1 = ev.e1;
   GG_HIP(hipEventRecord(e0, s));
   {
     TimedRegion t(g->prof, "cc.hook");
@@ -126,8 +126,6 @@ int cc_hooking(gg_graph *g, int32_t *label_out, uint32_t flags, gg_stats *stats)
   GG_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   g->prof.add_bytes("cc.hook", 8.0 * n + 8.0 * (double)g->m);
   g->prof.add_bytes("cc.compress", 8.0 * n);
   finish_profile(g);

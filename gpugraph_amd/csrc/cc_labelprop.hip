@@ -311,9 +311,9 @@ int cc_label_propagation(gg_graph *g, int32_t *label_out, uint32_t flags, gg_sta
   const int grid = g->opt.expand_grid;
   g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
 
-  hipEvent_t e0, e1;
-  GG_HIP(hipEventCreate(&e0));
-  GG_HIP(hipEventCreate(&e1));
+  ScopedEvents ev;
+  GG_HIP(ev.create());
+  hipEvent_t e0 = ev.e0, e1 = ev.e1;
   GG_HIP(hipEventRecord(e0, s));
   {
     TimedRegion t(g->prof, "cc.init");
@@ -328,8 +328,7 @@ int cc_label_propagation(gg_graph *g, int32_t *label_out, uint32_t flags, gg_sta
   const int32_t *parts = nullptr;
   GG_TRY(pull_partition(g, &parts));  // merge-path tiles over (vertices, adjacency), cached per graph
   CcPullOp pull_op{g->dg.column_indices, label, st, 0ull};
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  ScopedGraph cg;
   GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int it = 0; it < kIters; it++) {
     hipLaunchKernelGGL(k_lp_prep, dim3(grid_for((int64_t)words, kBlock, 1024)), dim3(kBlock), 0, s, st,
@@ -340,15 +339,9 @@ int cc_label_propagation(gg_graph *g, int32_t *label_out, uint32_t flags, gg_sta
                        g->dg.row_offsets, n, g->m, parts, &st->pull_skip);
     hipLaunchKernelGGL(k_lp_swap, dim3(1), dim3(1), 0, s, st, n, g->m, max_iter);
   }
-  hipError_t cap = hipStreamEndCapture(s, &graph);
+  hipError_t cap = hipStreamEndCapture(s, &cg.graph);
   if (cap != hipSuccess) return set_error(GG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(cap));
-  {
-    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ie != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      return set_error(GG_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
-    }
-  }
+  GG_HIP(hipGraphInstantiate(&cg.exec, cg.graph, nullptr, nullptr, 0));
   struct H {
     int32_t iters, done;
     unsigned long long edges, frontier_total, push_frontier;
@@ -359,7 +352,7 @@ int cc_label_propagation(gg_graph *g, int32_t *label_out, uint32_t flags, gg_sta
   for (;;) {
     {
       TimedRegion t(g->prof, "cc.labelprop");
-      if (hipGraphLaunch(exec, s) != hipSuccess) {
+      if (hipGraphLaunch(cg.exec, s) != hipSuccess) {
         rc = set_error(GG_ERR_HIP, "graph launch failed");
         break;
       }
@@ -372,15 +365,11 @@ int cc_label_propagation(gg_graph *g, int32_t *label_out, uint32_t flags, gg_sta
     std::memcpy(&h, g->h_pinned, sizeof(H));
     if (h.done) break;
   }
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
   GG_TRY(rc);
   GG_HIP(hipEventRecord(e1, s));
   GG_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   const double F = (double)h.frontier_total;
   const double bytes = 12.0 * n + words * 8.0 + 16.0 * (double)h.push_frontier + 8.0 * (double)h.edges +
                        (double)h.pulls * (12.0 * n + n / 8.0) + 8.0 * (F - n);

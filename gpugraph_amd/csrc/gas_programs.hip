@@ -132,9 +132,9 @@ int run_program(gg_graph *g, int num_srcs, int *srcs, typename P::DataType *out,
   if (g->prof.enabled) enactor.timer = &timer;
   const int threshold = g->opt.threshold >= 0 ? g->opt.threshold : Slot<P>::default_threshold;
 
-  hipEvent_t e0, e1;
-  GG_HIP(hipEventCreate(&e0));
-  GG_HIP(hipEventCreate(&e1));
+  ScopedEvents ev;
+  GG_HIP(ev.create());
+  hipEvent_t e0 = ev.e0, e1 = ev.e1;
   GG_HIP(hipEventRecord(e0, g->stream));
   hipError_t err = enactor.EnactIterativeSearch(*prob, nullptr, g->directed, num_srcs, srcs, g->opt.iter_num,
                                                 threshold);
@@ -142,8 +142,6 @@ int run_program(gg_graph *g, int num_srcs, int *srcs, typename P::DataType *out,
   (void)hipEventSynchronize(e1);
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (err != hipSuccess)
     return set_error(err == hipErrorOutOfMemory ? GG_ERR_OOM : GG_ERR_HIP, "%s: %s", Slot<P>::name,
                      hipGetErrorString(err));
@@ -317,6 +315,8 @@ int gg_sssp(gg_graph *g, int32_t src, int32_t *dist, uint32_t flags, gg_stats *s
   GG_CHECK_ARG(g, "graph is NULL");
   GG_CHECK_ARG(src >= 0 && src < g->n, "src %d out of range [0, %d)", src, g->n);
   GG_CHECK_ARG(g->value_dtype == GG_VALUES_I32, "SSSP needs int32 edge weights");
+  GG_TRY(activate(g));
+  GG_TRY(require_nonnegative_weights(g, "SSSP"));
   if (!g->reordered) return sssp_core(g, src, dist, flags, stats);
   const int32_t s = map_vertex(g, src);
   return with_mapped_output(g, dist, flags,

@@ -219,6 +219,63 @@ int csc_values(gg_graph *g, const void **vals) {
   return GG_OK;
 }
 
+// Smallest and largest int32 edge value: per-thread running min / max,
+// wave shuffles, then one atomicMin / atomicMax per wave.
+__global__ void __launch_bounds__(256) k_minmax_i32(const int32_t *__restrict__ w, int64_t m, int32_t *__restrict__ mm) {
+  int32_t lo = INT_MAX, hi = INT_MIN;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t x = w[k];
+    lo = min(lo, x);
+    hi = max(hi, x);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, __shfl_xor(lo, off));
+    hi = max(hi, __shfl_xor(hi, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(mm, lo);
+    atomicMax(mm + 1, hi);
+  }
+}
+
+__global__ void k_minmax_init(int32_t *mm) {
+  mm[0] = INT_MAX;
+  mm[1] = INT_MIN;
+}
+
+int weight_range(gg_graph *g, int32_t *min_w, int32_t *max_w) {
+  if (g->value_dtype != GG_VALUES_I32 || g->m == 0) {
+    *min_w = 0;
+    *max_w = 0;
+    return GG_OK;
+  }
+  if (g->wr_version != g->values_version) {
+    hipStream_t s = g->stream;
+    DevBuf mm;
+    GG_TRY(mm.ensure(2 * sizeof(int32_t)));
+    int32_t *d = mm.as<int32_t>();
+    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(1), 0, s, d);
+    hipLaunchKernelGGL(k_minmax_i32, dim3(GASengine::grid_for(g->m, 256, 4096)), dim3(256), 0, s,
+                       g->val.as<int32_t>(), g->m, d);
+    GG_HIP(hipGetLastError());
+    GG_HIP(hipMemcpyAsync(g->wr, d, sizeof(g->wr), hipMemcpyDeviceToHost, s));
+    GG_HIP(hipStreamSynchronize(s));
+    g->wr_version = g->values_version;
+  }
+  *min_w = g->wr[0];
+  *max_w = g->wr[1];
+  return GG_OK;
+}
+
+int require_nonnegative_weights(gg_graph *g, const char *what) {
+  int32_t lo = 0, hi = 0;
+  GG_TRY(weight_range(g, &lo, &hi));
+  if (lo < 0)
+    return set_error(GG_ERR_INVALID, "%s needs non-negative edge weights (smallest weight %d)", what, lo);
+  return GG_OK;
+}
+
 // Undirected graphs: confirm the CSR is symmetric (every u->v has a v->u)
 // and record whether mirrored entries carry equal values; keeps the mirror
 // permutation in g->eid when they do not.  Rows that are not sorted cannot

@@ -150,6 +150,35 @@ struct RunEvents {
   }
 };
 
+// Two events owned by one call: destroyed on every return path.
+struct ScopedEvents {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ScopedEvents() = default;
+  ScopedEvents(const ScopedEvents &) = delete;
+  ScopedEvents &operator=(const ScopedEvents &) = delete;
+  ~ScopedEvents() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+  hipError_t create() {
+    hipError_t r = hipEventCreate(&e0);
+    return r != hipSuccess ? r : hipEventCreate(&e1);
+  }
+};
+
+// A captured graph and its executable owned by one call.
+struct ScopedGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ScopedGraph() = default;
+  ScopedGraph(const ScopedGraph &) = delete;
+  ScopedGraph &operator=(const ScopedGraph &) = delete;
+  ~ScopedGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
 // Scoped helper for kernels launched outside the generic enactor.
 struct TimedRegion {
   EventProfiler &p;
@@ -234,6 +263,9 @@ struct gg_graph {
   int64_t pk_version = -1;
   int32_t pk_wbits = 0;  // 0: not packable
   int64_t values_version = 0, w_csc_version = -1;
+  // Smallest / largest int32 edge value, for values version wr_version.
+  int32_t wr[2] = {0, 0};
+  int64_t wr_version = -1;
   // Undirected graphs: rows sorted and symmetry confirmed (mirror_ok); the
   // mirrored entries' values differ (values_asymmetric; mirror permutation
   // then kept in eid).
@@ -301,6 +333,13 @@ int reorder_in_vertices(gg_graph *g, const void *src, uint32_t flags, DevBuf &bu
 int reorder_in_edges(gg_graph *g, const void *src, uint32_t flags, void *dst_dev);
 int reorder_out_edges(gg_graph *g, const void *in, void *dst, uint32_t flags);
 int reorder_out_cc(gg_graph *g, const int32_t *label, int32_t *dst, uint32_t flags);
+// Smallest and largest int32 edge value (0, 0 for other value types),
+// cached per values version.
+int weight_range(gg_graph *g, int32_t *min_w, int32_t *max_w);
+// GG_ERR_INVALID when an int32 edge weight is negative: shortest paths
+// then need not exist (a negative undirected edge is a negative cycle), and
+// runs that go to quiescence would not end.
+int require_nonnegative_weights(gg_graph *g, const char *what);
 // Symmetry / value-symmetry check of an undirected graph (sets mirror_ok,
 // values_asymmetric); GG_ERR_INVALID when the CSR is provably not symmetric.
 int check_undirected(gg_graph *g);

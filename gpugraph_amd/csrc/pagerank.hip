@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 
 #include <hipcub/hipcub.hpp>
@@ -527,7 +528,8 @@ int stage_input(gg_graph *g, const float *p, uint32_t flags, gg::DevBuf &buf, co
 // rows or, with slices, the XCD-sliced partials + combine.
 template <class Op>
 int iterate(gg_graph *g, Op op, int dir, const SliceLists *slices, PrState *st, double tol, int32_t max_iter,
-            const char *kname, double bytes_per_sweep, int32_t *iters_out, double *resid_out) {
+            const char *kname, double bytes_per_sweep, int32_t *iters_out, double *resid_out,
+            double *scale_out = nullptr) {
   hipStream_t s = g->stream;
   const int32_t *done = &st->done;
   float2 *part = slices ? g->sl_part.as<float2>() : nullptr;
@@ -539,10 +541,15 @@ int iterate(gg_graph *g, Op op, int dir, const SliceLists *slices, PrState *st, 
     GG_TRY(g->seg_carry.ensure(sizeof(SegCarry) * (size_t)seg_tiles(g->n, g->m)));
     carry = g->seg_carry.as<SegCarry>();
   }
+  // Read back from PrState::scale on: scale, resid_last, iters, done.
   struct H {
-    double resid_last;
+    double scale, resid_last;
     int32_t iters, done;
   };
+  static_assert(offsetof(PrState, resid_last) == offsetof(PrState, scale) + offsetof(H, resid_last) &&
+                    offsetof(PrState, iters) == offsetof(PrState, scale) + offsetof(H, iters) &&
+                    offsetof(PrState, done) == offsetof(PrState, scale) + offsetof(H, done),
+                "PrState read-back layout");
   for (;;) {
     for (int j = 0; j < kItersPerCheck; j++) {
       {
@@ -555,13 +562,14 @@ int iterate(gg_graph *g, Op op, int dir, const SliceLists *slices, PrState *st, 
       hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(1), 0, s, st, tol, max_iter);
     }
     GG_HIP(hipGetLastError());
-    GG_HIP(hipMemcpyAsync(g->h_pinned, &st->resid_last, sizeof(H), hipMemcpyDeviceToHost, s));
+    GG_HIP(hipMemcpyAsync(g->h_pinned, &st->scale, sizeof(H), hipMemcpyDeviceToHost, s));
     GG_HIP(hipStreamSynchronize(s));
     H h;
     std::memcpy(&h, g->h_pinned, sizeof(H));
     if (h.done) {
       *iters_out = h.iters;
       *resid_out = h.resid_last;
+      if (scale_out) *scale_out = h.scale;
       g->prof.add_bytes(kname, bytes_per_sweep * h.iters);
       return GG_OK;
     }
@@ -599,9 +607,9 @@ int gg_pagerank(gg_graph *g, float damping, const float *teleport, double tol_l1
   float *x0 = g->pr[5].as<float>(), *x1 = x0 + n;
   PrState *st = g->pr[6].as<PrState>();
 
-  hipEvent_t e0, e1;
-  GG_HIP(hipEventCreate(&e0));
-  GG_HIP(hipEventCreate(&e1));
+  ScopedEvents ev;
+  GG_HIP(ev.create());
+  hipEvent_t e0 = ev.e0, e1 = ev.e1;
   GG_HIP(hipEventRecord(e0, s));
   GG_HIP(fill_async<unsigned char>(reinterpret_cast<unsigned char *>(st), 0, sizeof(PrState), s));
   hipLaunchKernelGGL(k_pr_init, dim3(grid_for(n, kRowBlock, 4096)), dim3(kRowBlock), 0, s, r0,
@@ -637,8 +645,6 @@ int gg_pagerank(gg_graph *g, float damping, const float *teleport, double tol_l1
   GG_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   finish_profile(g);
   const float *res = (iters & 1) ? r1 : r0;
   if (stats) {
@@ -693,9 +699,9 @@ int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank, const flo
   float *l0 = g->batch[0].as<float>(), *l1 = l0 + n;
   PrState *st = g->batch[2].as<PrState>();
 
-  hipEvent_t e0, e1;
-  GG_HIP(hipEventCreate(&e0));
-  GG_HIP(hipEventCreate(&e1));
+  ScopedEvents ev;
+  GG_HIP(ev.create());
+  hipEvent_t e0 = ev.e0, e1 = ev.e1;
   GG_HIP(hipEventRecord(e0, s));
   GG_HIP(fill_async<unsigned char>(reinterpret_cast<unsigned char *>(st), 0, sizeof(PrState), s));
   hipLaunchKernelGGL(k_pr_init, dim3(grid_for(n, kRowBlock, 4096)), dim3(kRowBlock), 0, s, l0, (float *)nullptr,
@@ -723,9 +729,9 @@ int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank, const flo
     op.pcsr = weighted ? g->sl_val[1].as<float>() : nullptr;
   }
   int32_t iters = 0;
-  double resid = 0;
+  double resid = 0, scale = 1.0;  // scale = ||c||_1 (1 when c = 0)
   GG_TRY(iterate(g, op, 1, sliced ? &L : nullptr, st, tol_l1, max_iter, "pagerank.adjoint_sweep", per_sweep, &iters,
-                 &resid));
+                 &resid, &scale));
   const float *lam = (iters & 1) ? l1 : l0;
   // Gradients.
   float *gw = nullptr, *gt = nullptr;
@@ -768,8 +774,6 @@ int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank, const flo
   GG_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   finish_profile(g);
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
@@ -788,7 +792,7 @@ int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank, const flo
     if (grad_w && !dev_out) GG_TRY(copy_out(g, grad_w, gw, 4 * (size_t)m, flags));
     if (grad_t && !dev_out) GG_TRY(copy_out(g, grad_t, gt, 4 * (size_t)n, flags));
   }
-  if (iters >= max_iter && resid > tol_l1)
+  if (resid > tol_l1 * scale)
     return set_error(GG_ERR_NOT_CONVERGED, "adjoint: residual %.3e > tol %.3e x ||c||_1 after %d sweeps", resid, tol_l1,
                      iters);
   return GG_OK;

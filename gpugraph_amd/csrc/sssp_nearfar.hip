@@ -34,8 +34,6 @@
 // whichever layout is read; the packed words move 8 of them.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
-
 #include <climits>
 #include <cstring>
 #include <vector>
@@ -233,18 +231,9 @@ int packed_in_edges(gg_graph *g, const int32_t *wc, const uint32_t **pk, int32_t
     g->pk_version = g->values_version;
     g->pk_wbits = 0;
     hipStream_t s = g->stream;
-    DevBuf mm, tmp;
-    GG_TRY(mm.ensure(2 * sizeof(int32_t)));
-    int32_t *d_mm = mm.as<int32_t>();
-    size_t tb = 0, tb2 = 0;
-    GG_HIP(hipcub::DeviceReduce::Min(nullptr, tb, wc, d_mm, (int)g->m, s));
-    GG_HIP(hipcub::DeviceReduce::Max(nullptr, tb2, wc, d_mm + 1, (int)g->m, s));
-    GG_TRY(tmp.ensure(std::max(tb, tb2)));
-    GG_HIP(hipcub::DeviceReduce::Min(tmp.p, tb, wc, d_mm, (int)g->m, s));
-    GG_HIP(hipcub::DeviceReduce::Max(tmp.p, tb2, wc, d_mm + 1, (int)g->m, s));
+    // The CSC-ordered weights are a permutation of the CSR values: same range.
     int32_t h[2];
-    GG_HIP(hipMemcpyAsync(h, d_mm, sizeof(h), hipMemcpyDeviceToHost, s));
-    GG_HIP(hipStreamSynchronize(s));
+    GG_TRY(weight_range(g, &h[0], &h[1]));
     const int wb = bits_for(h[1]), ub = bits_for((int64_t)g->n - 1);
     if (h[0] >= 0 && wb + ub <= 32) {
       GG_TRY(g->pk_edges.ensure(4 * (size_t)g->m));

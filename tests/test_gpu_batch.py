@@ -116,3 +116,29 @@ def test_batch_errors(gpu_available):
             g.batch(sssp_sources=[0])  # no int32 weights
         with pytest.raises(gg.GGError):
             g.batch(ppr_seeds=[c.n])  # seed out of range
+
+
+def test_negative_weights_rejected(gpu_available):
+    """A negative int32 weight makes shortest paths undefined on an undirected
+    graph (a negative cycle); SSSP and batched SSSP refuse it with
+    GG_ERR_INVALID instead of relaxing without end, also after values that
+    were fine are replaced by negative ones."""
+    from gpugraph_amd import _abi
+
+    c = workloads.config1_er().with_weights("i32", 1, 64, symmetric=True, seed=54)
+    bad = c.values.copy()
+    k = int(np.flatnonzero(c.col_indices > 0)[0])
+    bad[k] = -3
+    with gg.Graph(c) as g:
+        dist, _ = g.sssp(0)
+        np.testing.assert_array_equal(dist, oracle.sssp(c.row_offsets, c.col_indices, c.values, 0))
+        g.update_values(bad)
+        for call in (lambda: g.sssp(0), lambda: g.sssp(0, flags=_abi.GG_SSSP_GAS), lambda: g.batch(sssp_sources=[0, 1])):
+            with pytest.raises(gg.GGError) as ei:
+                call()
+            assert ei.value.code == _abi.GG_ERR_INVALID
+        # PageRank-only batches do not read the weights as path lengths.
+        g.batch(ppr_seeds=[0], tol_l1=1e-6)
+        g.update_values(c.values)
+        dist2, _ = g.sssp(0)
+        np.testing.assert_array_equal(dist2, dist)

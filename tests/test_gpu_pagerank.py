@@ -142,6 +142,64 @@ def test_autograd_function(gpu_available):
     np.testing.assert_allclose(t.grad.cpu().numpy(), gt, rtol=1e-5, atol=1e-9)
 
 
+def test_autograd_two_forwards_one_backward(gpu_available):
+    """Two forwards on one graph with different weights, then one backward
+    through both: each call's gradient is its own adjoint (the second
+    forward replaced the graph's resident weights)."""
+    import torch
+
+    from gpugraph_amd.autograd import pagerank
+
+    c = gg.gen_rmat(11, 16, undirected=False, seed=43).with_weights("f32", symmetric=False, seed=44)
+    rng = np.random.default_rng(45)
+    c1 = rng.standard_normal(c.n).astype(np.float32)
+    c2 = rng.standard_normal(c.n).astype(np.float32)
+    w1v = c.values.copy()
+    w2v = c.values.copy()
+    w2v[: c.m // 2] *= 3.0
+    with gg.Graph(c, directed=True) as g:
+        w1 = torch.tensor(w1v, device="cuda", requires_grad=True)
+        w2 = torch.tensor(w2v, device="cuda", requires_grad=True)
+        r1 = pagerank(g, w1, tol_l1=2e-7, adj_tol=2e-7)
+        r2 = pagerank(g, w2, tol_l1=2e-7, adj_tol=2e-7)
+        loss = (torch.tensor(c1, device="cuda") * r1).sum() + (torch.tensor(c2, device="cuda") * r2).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        expected = []
+        for wv, cv in ((w1v, c1), (w2v, c2)):
+            g.update_values(wv)
+            rr, _ = g.pagerank(0.85, tol_l1=2e-7)
+            expected.append(g.pagerank_adjoint(rr, cv, tol_l1=2e-7)[0])
+    np.testing.assert_allclose(w1.grad.cpu().numpy(), expected[0], rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(w2.grad.cpu().numpy(), expected[1], rtol=1e-5, atol=1e-9)
+    assert np.abs(expected[0] - expected[1]).max() > 1e-6  # the two calls really differ
+
+
+@pytest.mark.parametrize("c_scale", [1e-3, 1e3])
+def test_adjoint_convergence_status_scales_with_c(gpu_available, c_scale):
+    """The adjoint's stopping test is residual <= tol_l1 * ||c||_1: scaling c
+    scales the residual and the threshold alike, so the sweep count is the
+    same, a run capped at that count returns GG_OK and one capped a sweep
+    earlier returns GG_ERR_NOT_CONVERGED, whatever ||c||_1 is."""
+    c = gg.gen_rmat(11, 16, undirected=False, seed=46).with_weights("f32", symmetric=False, seed=47)
+    rng = np.random.default_rng(48)
+    cv = rng.standard_normal(c.n)
+    cv *= 1.0 / np.abs(cv).sum()  # ||c||_1 = 1
+    with gg.Graph(c, directed=True) as g:
+        r, _ = g.pagerank(0.85, tol_l1=2e-7)
+        _, _, st1 = g.pagerank_adjoint(r, cv.astype(np.float32), tol_l1=1e-6)
+        k = st1["iterations"]
+        cs = (cv * c_scale).astype(np.float32)
+        _, _, st = g.pagerank_adjoint(r, cs, tol_l1=1e-6)
+        assert abs(st["iterations"] - k) <= 1
+        k = st["iterations"]
+        _, _, st = g.pagerank_adjoint(r, cs, tol_l1=1e-6, max_iter=k)
+        assert st["residual"] <= 1e-6 * np.abs(cs.astype(np.float64)).sum()
+        with pytest.raises(gg.GGError) as ei:
+            g.pagerank_adjoint(r, cs, tol_l1=1e-6, max_iter=k - 1)
+        assert ei.value.code == _abi.GG_ERR_NOT_CONVERGED
+
+
 def test_update_values_changes_ranks(gpu_available):
     c = gg.gen_rmat(11, 8, undirected=False, seed=41).with_weights("f32", symmetric=False, seed=42)
     with gg.Graph(c, directed=True) as g:
