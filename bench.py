@@ -15,12 +15,23 @@ its frontier-heavy launches (>= 1% of m edges) broken out.  `cpu_baseline` is th
 reference's CPUBFS and, for SSSP, a heap Dijkstra with CPUSSSP's output) on
 the same graph and source.
 
-Extras (same JSON line, `extras`): CC on the 2048^2 percolated lattice
-(label propagation and hooking); batched 128 SSSP + 64 PPR on R-MAT scale 20
-directed against 128 + 64 plain launches; PageRank and its adjoint on R-MAT
-scale 22 directed with FP32 weights.  --full-checks adds the FP64 oracle's
-PageRank L1 error at scale 22 (tens of seconds of CPU time).  The R-MAT graphs
-are stored renumbered by degree (GG_GRAPH_REORDER; ids in and out stay the
+Per-config rows (same JSON line, `per_config`), each with the GPU time, its
+GTEPS (the reference's count, the undirected half where the graph is
+symmetric, and work GTEPS = edges inspected / t), the oracle as written on
+1 host core and the OpenMP baselines on all host cores, and a parity check
+of the GPU result against the oracle in the same run:
+  1. BFS on the Erdos-Renyi graph (n = 4096);
+  2. the headline's BFS and SSSP, plus CC on the same graph;
+  3. CC on the 2048^2 percolated lattice (label propagation, hooking, the
+     generic GAS program);
+  4. batched 128 SSSP + 64 PPR on R-MAT scale 20 directed against 128 + 64
+     plain GPU launches and against the CPU's per-instance loop (a sample of
+     instances, scaled);
+  5. PageRank and its adjoint on R-MAT scale 22 directed with FP32 weights,
+     with the L1 error against the FP64 oracle in every run.
+The generic GAS path (the reference's own schedules: push-only BFS, GAS
+SSSP, GAS CC) is timed beside the specialised kernels.  The R-MAT graphs are
+stored renumbered by degree (GG_GRAPH_REORDER; ids in and out stay the
 generator's; --no-reorder keeps the generated order), and the headline's
 BFS / SSSP times on the graph in generated order are reported beside it.
 
@@ -180,11 +191,12 @@ def headline(args, gg, _abi, workloads, torch, np):
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["total_ms"])}
     per_alg = {
         "frontier_heavy": heavy,
-        "bfs": {"gteps": re_bfs / (bfs_ms * 1e-3) / 1e9, "gteps_undirected": re_bfs / 2 / (bfs_ms * 1e-3) / 1e9,
+        "bfs": {**gteps_row(re_bfs, sb_list[-1]["edges_inspected"], bfs_ms, True),
                 "ms": bfs_ms, "levels": sb_list[-1]["iterations"], "pull_levels": sb_list[-1]["pull_iterations"],
                 "edges_inspected": sb_list[-1]["edges_inspected"],
                 "hbm_frac": sb_list[-1]["alg_bytes"] / (bfs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        "sssp": {"gteps": re_sssp / (sssp_ms * 1e-3) / 1e9, "ms": sssp_ms, "iterations": ss_list[-1]["iterations"],
+        "sssp": {**gteps_row(re_sssp, ss_list[-1]["edges_inspected"], sssp_ms, True),
+                 "ms": sssp_ms, "iterations": ss_list[-1]["iterations"],
                  "edges_inspected": ss_list[-1]["edges_inspected"],
                  "hbm_frac": ss_list[-1]["alg_bytes"] / (sssp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
@@ -193,11 +205,14 @@ def headline(args, gg, _abi, workloads, torch, np):
         import oracle
 
         t0 = time.perf_counter()
-        oracle.bfs(csr.row_offsets, csr.col_indices, src)
+        exp_bfs = oracle.bfs(csr.row_offsets, csr.col_indices, src)
         tb = time.perf_counter() - t0
         t0 = time.perf_counter()
-        oracle.sssp(csr.row_offsets, csr.col_indices, csr.values, src)
+        exp_sssp = oracle.sssp(csr.row_offsets, csr.col_indices, csr.values, src)
         ts = time.perf_counter() - t0
+        # Parity of the timed steps' last outputs (still on the device).
+        per_alg["bfs"]["bit_exact"] = bool(np.array_equal(out_bfs.cpu().numpy(), exp_bfs))
+        per_alg["sssp"]["bit_exact"] = bool(np.array_equal(out_sssp.cpu().numpy(), exp_sssp))
         cpu = {"value": (re_bfs + re_sssp) / (tb + ts) / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
                "sample": f"one BFS ({tb:.2f} s) + one heap-Dijkstra SSSP ({ts:.2f} s) from the same source on "
                          f"the same R-MAT s{args.scale} graph, 1 thread"}
@@ -226,10 +241,25 @@ def headline(args, gg, _abi, workloads, torch, np):
     if args.no_extras:
         return value, wall, roofline, kernels, per_alg, cpu, config
     with gg.Graph(csr, reorder=not args.no_reorder) as gc:
+        per_alg["gas_path"] = gas_path_rows(gg, _abi, gc, csr, src, np)
         cc = {}
-        for name, mode in (("labelprop", {}), ("hook", {"hook": True})):
+        exp_cc = None
+        if not args.no_cpu:
+            import oracle
+
+            t0 = time.perf_counter()
+            exp_cc, _ = oracle.cc(csr.row_offsets, csr.col_indices)
+            cc["cpu_1core_ms"] = (time.perf_counter() - t0) * 1e3
+            t0 = time.perf_counter()
+            oracle.cc_omp(csr.row_offsets, csr.col_indices)
+            cc["cpu_all_cores_ms"] = (time.perf_counter() - t0) * 1e3
+            cc["cores"] = oracle.threads()
+        for name, mode in (("labelprop", {}), ("hook", {"hook": True}), ("gas", {"gas": True})):
             ms, st = median_ms(lambda: gc.cc(flags=flags, out=out_bfs, **mode), 5)
-            cc[name] = {"ms": ms, "gteps": st["reached_edges"] / (ms * 1e-3) / 1e9, "iterations": st["iterations"]}
+            cc[name] = {"ms": ms, **gteps_row(st["reached_edges"], st["edges_inspected"], ms, True),
+                        "iterations": st["iterations"]}
+            if exp_cc is not None:
+                cc[name]["bit_exact"] = bool(np.array_equal(out_bfs.cpu().numpy(), exp_cc))
         per_alg["cc"] = cc
     if not args.no_reorder:
         # The same BFS / SSSP on the graph stored in generated order.
@@ -240,18 +270,84 @@ def headline(args, gg, _abi, workloads, torch, np):
     return value, wall, roofline, kernels, per_alg, cpu, config
 
 
-def extra_lattice(gg, workloads, np):
-    lat = workloads.config3_lattice(2048)
+def timed(fn, min_s=0.2, max_reps=200):
+    """Seconds per call of a CPU function: repeated until min_s has passed."""
+    t0 = time.perf_counter()
+    reps = 0
+    out = None
+    while True:
+        out = fn()
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= min_s or reps >= max_reps:
+            return el / reps, out
+
+
+def gteps_row(reached_edges, edges_inspected, ms, symmetric):
+    row = {"gteps": reached_edges / (ms * 1e-3) / 1e9, "work_gteps": edges_inspected / (ms * 1e-3) / 1e9}
+    if symmetric:
+        row["gteps_undirected"] = row["gteps"] / 2
+    return row
+
+
+def extra_config1(gg, workloads, np):
+    """Config 1: BFS from vertex 0 on the Erdos-Renyi graph (n = 4096)."""
+    import oracle
+
+    c = workloads.config1_er()
+    with gg.Graph(c) as g:
+        ms, st = median_ms(lambda: g.bfs(0), 20)
+        d, _ = g.bfs(0)
+    t1, exp = timed(lambda: oracle.bfs(c.row_offsets, c.col_indices, 0))
+    tp, _ = timed(lambda: oracle.bfs_omp(c.row_offsets, c.col_indices, 0))
+    row = {"workload": f"erdos_renyi n={c.n} m_csr={c.m}, BFS from 0", "gpu_ms": ms,
+           **gteps_row(st["reached_edges"], st["edges_inspected"], ms, True),
+           "cpu_1core_ms": t1 * 1e3, "cpu_all_cores_ms": tp * 1e3, "cores": oracle.threads(),
+           "speedup_vs_1core": t1 * 1e3 / ms, "speedup_vs_all_cores": tp * 1e3 / ms,
+           "bit_exact": bool(np.array_equal(d, exp))}
+    return row
+
+
+def gas_path_rows(gg, _abi, g, csr, src, np):
+    """The generic GAS program path (the reference's own schedules) on the
+    headline graph: push-only BFS and GAS SSSP, with their byte models."""
     out = {}
+    for name, fn in (("bfs_push_only", lambda: g.bfs(src, push_only=True)),
+                     ("sssp_gas", lambda: g.sssp(src, flags=_abi.GG_SSSP_GAS))):
+        ms, st = median_ms(fn, 5)
+        out[name] = {"ms": ms, **gteps_row(st["reached_edges"], st["edges_inspected"], ms, True),
+                     "iterations": st["iterations"], "hbm_frac": st["alg_bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    return out
+
+
+def extra_lattice(gg, workloads, np):
+    """Config 3: CC on the 2048^2 percolated lattice, three GPU engines."""
+    import oracle
+
+    lat = workloads.config3_lattice(2048)
+    out = {"workload": f"lattice 2048^2 p=0.6 + 0.1% long-range, n={lat.n} m_csr={lat.m}"}
+    t0 = time.perf_counter()
+    exp, sweeps = oracle.cc(lat.row_offsets, lat.col_indices)
+    t1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.cc_omp(lat.row_offsets, lat.col_indices)
+    tp = time.perf_counter() - t0
     with gg.Graph(lat) as gl:
         for name, mode in (("labelprop", {}), ("labelprop_gas", {"gas": True}), ("hook", {"hook": True})):
             ms, st = median_ms(lambda: gl.cc(**mode), 3)
-            out["cc_lattice_" + name] = {
-                "ms": ms, "gteps": st["reached_edges"] / (ms * 1e-3) / 1e9, "iterations": st["iterations"]}
+            lab, _ = gl.cc(**mode)
+            out[name] = {"ms": ms, **gteps_row(st["reached_edges"], st["edges_inspected"], ms, True),
+                         "iterations": st["iterations"], "bit_exact": bool(np.array_equal(lab, exp)),
+                         "speedup_vs_1core": t1 * 1e3 / ms, "speedup_vs_all_cores": tp * 1e3 / ms}
+    out.update({"cpu_1core_ms": t1 * 1e3, "cpu_1core_sweeps": sweeps, "cpu_all_cores_ms": tp * 1e3,
+                "cores": oracle.threads()})
     return out
 
 
 def extra_batch(gg, workloads, torch, np, scale, reorder):
+    """Config 4: 128 SSSP + 64 PPR batched vs plain GPU launches vs the CPU."""
+    import oracle
+
     c = workloads.config4_rmat(scale)
     src, seeds = workloads.config4_instances(c, 128, 64)
     out = {"graph": f"rmat{scale} directed, n={c.n}, m={c.m}", "vertex_order": "renumbered" if reorder else "generated"}
@@ -272,6 +368,12 @@ def extra_batch(gg, workloads, torch, np, scale, reorder):
             if label == "sssp128":
                 out["sssp128_union_edges"] = st["edges_inspected"]
                 out["sssp128_iterations"] = st["iterations"]
+            if label == "both":
+                out["ppr_residual_l1"] = st["residual"]
+        sample_s = [0, 37, 74, 111]
+        sample_p = [0, 21, 42, 63]
+        dist_rows = dist_dev[sample_s].cpu().numpy()
+        rank_rows = rank_dev[sample_p].cpu().numpy()
         del dist_dev, rank_dev
         dev_i = torch.empty(c.n, dtype=torch.int32, device="cuda")
         dev_f = torch.empty(c.n, dtype=torch.float32, device="cuda")
@@ -292,10 +394,48 @@ def extra_batch(gg, workloads, torch, np, scale, reorder):
                                                 "both": plain_sssp + plain_ppr},
                 "speedup": {"sssp128": plain_sssp / res["sssp128"], "ppr64": plain_ppr / res["ppr64"],
                             "both": (plain_sssp + plain_ppr) / res["both"]}})
+    # Parity of sampled rows (SSSP bit-exact; PPR L1 against FP64 at 1e-14),
+    # and the CPU's per-instance loop on the same sample, scaled to 128 + 64.
+    co, ri, perm = oracle.csc(c.row_offsets, c.col_indices)
+    ok, t1s, tps = True, 0.0, 0.0
+    for k, i in enumerate(sample_s):
+        t0 = time.perf_counter()
+        exp = oracle.sssp(c.row_offsets, c.col_indices, c.values, int(src[i]))
+        t1s += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        oracle.sssp_omp(c.row_offsets, c.col_indices, c.values, int(src[i]))
+        tps += time.perf_counter() - t0
+        ok = ok and bool(np.array_equal(dist_rows[k], exp))
+    errs, t1p, tpp = [], 0.0, 0.0
+    for k, i in enumerate(sample_p):
+        t = np.zeros(c.n)
+        t[seeds[i]] = 1.0
+        exp, _, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, t=t, tol=1e-14)
+        errs.append(float(np.abs(rank_rows[k].astype(np.float64) - exp).sum()))
+        t0 = time.perf_counter()
+        oracle.pagerank_f64(c.row_offsets, c.col_indices, t=t, tol=1e-6)
+        t1p += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        oracle.pagerank_f64_omp(c.row_offsets, co, ri, t=t, tol=1e-6)
+        tpp += time.perf_counter() - t0
+    cpu1 = (t1s * 128 / len(sample_s) + t1p * 64 / len(sample_p)) * 1e3
+    cpup = (tps * 128 / len(sample_s) + tpp * 64 / len(sample_p)) * 1e3
+    out["parity"] = {"sssp_rows_bit_exact": ok, "sssp_rows_checked": len(sample_s),
+                     "ppr_l1_error_vs_fp64_max": max(errs), "ppr_rows_checked": len(sample_p),
+                     "ppr_error_bound_at_tol": 0.85 / 0.15 * 1e-6}
+    out["cpu"] = {"per_instance_loop_1core_ms": cpu1, "per_instance_loop_all_cores_ms": cpup,
+                  "cores": oracle.threads(),
+                  "sample": f"{len(sample_s)} of 128 SSSP sources (heap Dijkstra / OpenMP bucketed SSSP) and "
+                            f"{len(sample_p)} of 64 PPR seeds (FP64 power iteration to L1 1e-6), scaled",
+                  "speedup_batched_vs_1core": cpu1 / res["both"], "speedup_batched_vs_all_cores": cpup / res["both"]}
     return out
 
 
-def extra_pagerank(gg, workloads, np, scale, full_checks, reorder):
+def extra_pagerank(gg, workloads, np, scale, reorder):
+    """Config 5: FP32 PageRank and its adjoint, the FP64 oracle's L1 error and
+    the CPU's FP64 power iteration beside them."""
+    import oracle
+
     c = workloads.config5_rmat(scale)
     out = {"graph": f"rmat{scale} directed f32, n={c.n}, m={c.m}", "vertex_order": "renumbered" if reorder else "generated"}
     cvec = workloads.config5_loss_weights(c.n).astype(np.float32)
@@ -304,7 +444,8 @@ def extra_pagerank(gg, workloads, np, scale, full_checks, reorder):
         ms, st = median_ms(lambda: g.pagerank(tol_l1=2e-7), 3)
         r, _ = g.pagerank(tol_l1=2e-7)
         out["forward"] = {"ms": ms, "sweeps": st["iterations"], "residual_l1": st["residual"],
-                          "hbm_frac": st["alg_bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                          "hbm_frac": st["alg_bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "work_gteps": st["edges_inspected"] / (ms * 1e-3) / 1e9}
         ams, ast = median_ms(lambda: g.pagerank_adjoint(r, cvec, tol_l1=2e-7)[1:], 3)
         out["adjoint"] = {"ms": ams, "sweeps": ast["iterations"],
                           "hbm_frac": ast["alg_bytes"] / (ams * 1e-3) / 1e9 / HBM_PEAK_GBS}
@@ -313,17 +454,27 @@ def extra_pagerank(gg, workloads, np, scale, full_checks, reorder):
         ms1, st1 = median_ms(lambda: g.pagerank(tol_l1=2e-7), 3)
         ams1, _ = median_ms(lambda: g.pagerank_adjoint(r, cvec, tol_l1=2e-7)[1:], 3)
         out["xcd_sliced"] = {"forward_ms": ms1, "forward_sweeps": st1["iterations"], "adjoint_ms": ams1}
-    if full_checks:
-        import oracle
-
-        deg = c.degrees()
-        rows = np.repeat(np.arange(c.n), deg)
-        s = np.bincount(rows, weights=c.values.astype(np.float64), minlength=c.n)
-        p = (c.values / np.repeat(np.where(deg > 0, s, 1.0), deg)).astype(np.float32).astype(np.float64)
-        t0 = time.perf_counter()
-        exp, it, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=p, tol=1e-12)
-        out["forward"]["l1_error_vs_fp64"] = float(np.abs(r.astype(np.float64) - exp).sum())
-        out["forward"]["fp64_oracle_s"] = time.perf_counter() - t0
+    deg = c.degrees()
+    rows = np.repeat(np.arange(c.n), deg)
+    s = np.bincount(rows, weights=c.values.astype(np.float64), minlength=c.n)
+    p = (c.values / np.repeat(np.where(deg > 0, s, 1.0), deg)).astype(np.float32).astype(np.float64)
+    t0 = time.perf_counter()
+    exp, it, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=p, tol=1e-12)
+    out["forward"]["l1_error_vs_fp64"] = float(np.abs(r.astype(np.float64) - exp).sum())
+    out["forward"]["fp64_oracle"] = {"tol_l1": 1e-12, "sweeps": it, "s": time.perf_counter() - t0}
+    # CPU baseline: the same FP64 power iteration to the GPU's tolerance.
+    t0 = time.perf_counter()
+    _, it1, _ = oracle.pagerank_f64(c.row_offsets, c.col_indices, p=p, tol=2e-7)
+    t1 = time.perf_counter() - t0
+    co, ri, perm = oracle.csc(c.row_offsets, c.col_indices)
+    pc = p[perm]
+    t0 = time.perf_counter()
+    oracle.pagerank_f64_omp(c.row_offsets, co, ri, p_csc=pc, tol=2e-7)
+    tp = time.perf_counter() - t0
+    out["cpu"] = {"forward_1core_ms": t1 * 1e3, "forward_all_cores_ms": tp * 1e3, "cores": oracle.threads(),
+                  "sweeps": it1, "sample": "FP64 power iteration to L1 2e-7 on the same graph (push, 1 thread; "
+                                          "pull over the CSC, OpenMP)",
+                  "speedup_vs_1core": t1 * 1e3 / ms, "speedup_vs_all_cores": tp * 1e3 / ms}
     return out
 
 
@@ -350,7 +501,6 @@ def main():
     ap.add_argument("--options", default=None,
                     help='fixed options as JSON, e.g. \'{"pull_alpha": 2}\' (no gg_autotune; for profiling runs)')
     ap.add_argument("--no-reorder", action="store_true", help="keep the R-MAT graphs in generated vertex order")
-    ap.add_argument("--full-checks", action="store_true", help="FP64 PageRank error at full size")
     ap.add_argument("--copy-probe", action="store_true", help="run only a 1 GiB device copy (PMC calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -371,21 +521,26 @@ def main():
     value, wall, roofline, kernels, per_alg, cpu, config = headline(args, gg, _abi, workloads, torch, np)
     cpu_all = cpu.pop("parallel", None) if cpu else None
     cpu_main = cpu
-    extras = {}
+    per_config = {}
     if not args.no_extras:
-        extras.update(extra_lattice(gg, workloads, np))
-        log(f"lattice CC: {extras}")
-        extras["batched"] = extra_batch(gg, workloads, torch, np, 20, not args.no_reorder)
-        log(f"batched: {extras['batched']}")
-        extras["pagerank"] = extra_pagerank(gg, workloads, np, 22, args.full_checks, not args.no_reorder)
-        log(f"pagerank: {extras['pagerank']}")
+        per_config["1_er4096_bfs"] = extra_config1(gg, workloads, np)
+        log(f"config 1: {per_config['1_er4096_bfs']}")
+        per_config["2_rmat22_bfs_sssp_cc"] = {
+            "bfs": per_alg["bfs"], "sssp": per_alg["sssp"], "cc": per_alg.get("cc"),
+            "cpu_1core": cpu_main, "cpu_all_cores": cpu_all}
+        per_config["3_lattice2048_cc"] = extra_lattice(gg, workloads, np)
+        log(f"config 3: {per_config['3_lattice2048_cc']}")
+        per_config["4_batched_rmat20"] = extra_batch(gg, workloads, torch, np, 20, not args.no_reorder)
+        log(f"config 4: {per_config['4_batched_rmat20']}")
+        per_config["5_pagerank_rmat22"] = extra_pagerank(gg, workloads, np, 22, not args.no_reorder)
+        log(f"config 5: {per_config['5_pagerank_rmat22']}")
     line = {
         "metric": "GTEPS (BFS/SSSP/CC, R-MAT s22) + % HBM roofline; batched vs plain speedup",
         "value": value, "unit": "GTEPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int32", "data": "synthetic (seeded R-MAT generator)", "config": config,
         "roofline": roofline, "cpu_baseline": cpu_main, "cpu_baseline_all_cores": cpu_all, "per_algorithm": per_alg,
-        "kernels": kernels, "extras": extras,
+        "kernels": kernels, "per_config": per_config,
     }
     print(json.dumps(line))
 

@@ -106,14 +106,19 @@ int get_problem(gg_graph *g, Problem<P> **out) {
   return GG_OK;
 }
 
-// Algorithmic bytes per iteration (frontier F, edges E, next frontier N).
+// Algorithmic bytes of iterations with frontier F, edges E, next frontier N
+// (per iteration, or summed over a run: the model is linear).
 template <class P>
-double iteration_bytes(const GASengine::IterationRecord &r) {
-  const double F = r.frontier, E = (double)r.edges, N = r.next;
+double iteration_bytes(double F, double E, double N) {
   if (std::is_same<P, sssp>::value) return 16 * F + 12 * E + 8 * N;
   if (std::is_same<P, cc>::value || std::is_same<P, cc_weak>::value) return 16 * F + 8 * E + 8 * N;
   if (std::is_same<P, bfs>::value || std::is_same<P, bfs_reverse>::value) return 12 * F + 8 * E + 8 * N;
   return 12 * F + 12 * E + 8 * N;  // pagerank: gather reads ri + rank + outdeg per in-edge
+}
+
+template <class P>
+double iteration_bytes(const GASengine::IterationRecord &r) {
+  return iteration_bytes<P>(r.frontier, (double)r.edges, r.next);
 }
 
 template <class P>
@@ -159,10 +164,11 @@ int run_program(gg_graph *g, int num_srcs, int *srcs, typename P::DataType *out,
   }
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
-    stats->iterations = (int32_t)enactor.history.size();
+    stats->iterations = enactor.iterations_run;
     stats->edges_inspected = enactor.total_edges;
     stats->gpu_ms = ms;
-    for (auto &r : enactor.history) stats->alg_bytes += iteration_bytes<P>(r);
+    stats->alg_bytes = iteration_bytes<P>((double)enactor.frontier_total, (double)enactor.total_edges,
+                                          (double)enactor.next_total);
     if (count_reach) GG_TRY(reach_stats(g, (const int32_t *)res, *(int32_t *)&sentinel, stats));
   }
   return copy_out(g, out, res, sizeof(typename P::DataType) * (size_t)g->n, flags);

@@ -16,9 +16,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <optional>
 #include <vector>
 
 #include "Enums.h"
+#include "config.h"
 #include "csc_build.h"
 #include "lbs.h"
 #include "program_ops.h"
@@ -53,7 +55,18 @@ struct Workspace {
   int32_t *sizes = nullptr;                   // [0],[1]: frontier sizes
   unsigned long long *counters = nullptr;     // [0]: edges inspected
   int32_t *frontier_values[3] = {nullptr, nullptr, nullptr};
+  int32_t *preds = nullptr;                   // MARK_PREDECESSORS: discovering vertex per vertex
   int32_t *h_sizes = nullptr;                 // pinned readback: [0] size, [2..3] edge counter
+  GasCtl *ctl = nullptr;                      // device loop state
+  GasCtl *h_ctl = nullptr;                    // pinned read-back of ctl, sizes and the edge counter
+  void *gcarry = nullptr;                     // gather tile carries
+  size_t gcarry_bytes = 0;
+  // Device-driven loops: a captured sequence of iterations, the arguments it
+  // was captured with, and the stream it is captured on (launched on the
+  // problem's stream, which may be the null stream).
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t loop_exec = nullptr;
+  std::vector<uint64_t> loop_key;
 
   static int32_t words(int32_t n) { return (n + 31) / 32; }
 
@@ -76,8 +89,31 @@ struct Workspace {
     GAS_RETURN_IF(hipMalloc(&sizes, 4 * sizeof(int32_t)));
     GAS_RETURN_IF(hipMalloc(&counters, 8 * sizeof(unsigned long long)));
     GAS_RETURN_IF(hipHostMalloc(&h_sizes, 8 * sizeof(int32_t)));
+    GAS_RETURN_IF(hipMalloc(&ctl, sizeof(GasCtl)));
+    GAS_RETURN_IF(hipHostMalloc(&h_ctl, 2 * sizeof(GasCtl)));
     capacity = n;
     gather_bytes = gather_elem_bytes * nn;
+    return hipSuccess;
+  }
+
+  hipError_t EnsureCarry(size_t bytes) {
+    if (gcarry_bytes >= bytes && gcarry) return hipSuccess;
+    if (gcarry) (void)hipFree(gcarry);
+    gcarry = nullptr;
+    gcarry_bytes = 0;
+    GAS_RETURN_IF(hipMalloc(&gcarry, bytes));
+    gcarry_bytes = bytes;
+    return hipSuccess;
+  }
+
+  void ResetLoop() {
+    if (loop_exec) (void)hipGraphExecDestroy(loop_exec);
+    loop_exec = nullptr;
+    loop_key.clear();
+  }
+
+  hipError_t AllocatePreds() {
+    if (!preds) GAS_RETURN_IF(hipMalloc(&preds, (size_t)(capacity > 0 ? capacity : 1) * 4));
     return hipSuccess;
   }
 
@@ -99,10 +135,18 @@ struct Workspace {
       if (frontier_values[i]) (void)hipFree(frontier_values[i]);
       frontier_values[i] = nullptr;
     }
-    void *ps[] = {changed, gather_tmp, scan_local, partials, parts, info, sizes, counters};
+    ResetLoop();
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    cap_stream = nullptr;
+    void *ps[] = {changed, gather_tmp, scan_local, partials, parts, info, sizes, counters, preds, ctl, gcarry};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     if (h_sizes) (void)hipHostFree(h_sizes);
+    if (h_ctl) (void)hipHostFree(h_ctl);
+    ctl = nullptr;
+    h_ctl = nullptr;
+    gcarry = nullptr;
+    gcarry_bytes = 0;
     changed = nullptr;
     gather_tmp = nullptr;
     scan_local = nullptr;
@@ -111,17 +155,32 @@ struct Workspace {
     info = nullptr;
     sizes = nullptr;
     counters = nullptr;
+    preds = nullptr;
     h_sizes = nullptr;
     capacity = 0;
     gather_bytes = 0;
   }
 };
 
+namespace detail {
+template <class P, class = void>
+struct has_extract_pred : std::false_type {};
+template <class P>
+struct has_extract_pred<P, std::void_t<decltype(&P::extractPred)>> : std::true_type {};
+}  // namespace detail
+
+// MARK_PREDECESSORS: the engine also records, for every vertex that enters a
+// frontier, the vertex whose edge put it there first (the one that won the
+// frontier's dedup bit); sources are their own predecessor, unreached
+// vertices -1.  ExtractPreds copies them out (through Program::extractPred
+// when the program has one, as csr_problem.cuh:459-483 does).  In BFS this
+// is a BFS tree: each predecessor sits one level up.
 template <class Program, typename VertexId = int, typename SizeT = int,
           typename Value = typename Program::DataType, bool MARK_PREDECESSORS = false, bool WITH_VALUE = true>
 struct CsrProblem {
   typedef Program ProgramType;
   typedef typename Program::DataType DataType;
+  static constexpr bool kMarkPredecessors = MARK_PREDECESSORS;
 
   DeviceGraph graph;
   Workspace ws;
@@ -129,8 +188,15 @@ struct CsrProblem {
   typename Program::EdgeType edge_list;
   hipStream_t stream = nullptr;
   std::vector<void *> owned;  // device copies made by FromHostProblem
+  // The driver's parameters when built from a Config (held, not default
+  // constructed: a Config's first construction runs the driver's
+  // registerParameters(), which only programs that use Config define).
+  std::optional<Config> cfg;
 
   CsrProblem() = default;
+  // Reference constructor (csr_problem.cuh:378): keeps the driver's
+  // parameters; the graph comes with FromHostProblem.
+  explicit CsrProblem(const Config &cfg_) : cfg(cfg_) {}
   CsrProblem(const CsrProblem &) = delete;
   CsrProblem &operator=(const CsrProblem &) = delete;
   ~CsrProblem() {
@@ -142,7 +208,19 @@ struct CsrProblem {
   hipError_t Attach(const DeviceGraph &g, hipStream_t s) {
     graph = g;
     stream = s;
-    return ws.Allocate(g.nodes, g.edges, sizeof(typename Program::GatherType));
+    GAS_RETURN_IF(ws.Allocate(g.nodes, g.edges, sizeof(typename Program::GatherType)));
+    if (MARK_PREDECESSORS) GAS_RETURN_IF(ws.AllocatePreds());
+    return hipSuccess;
+  }
+
+  // Reference form with the device and MPI rank (csr_problem.cuh:512-524):
+  // the graph goes to device `device_id`; rank_id has no use on one GPU.
+  hipError_t FromHostProblem(bool stream_from_host, SizeT nodes, SizeT edges, VertexId *h_column_indices,
+                             SizeT *h_row_offsets, Value *h_edge_values, VertexId *h_row_indices,
+                             SizeT *h_column_offsets, int num_gpus, int directed, int device_id, int /*rank_id*/) {
+    GAS_RETURN_IF(hipSetDevice(device_id));
+    return FromHostProblem(stream_from_host, nodes, edges, h_column_indices, h_row_offsets, h_edge_values,
+                           h_row_indices, h_column_offsets, num_gpus, directed);
   }
 
   // Reference form (csr_problem.cuh:512-524, upstream 10-argument overload):
@@ -213,6 +291,20 @@ struct CsrProblem {
     GAS_RETURN_IF(hipStreamSynchronize(stream));
     engine_stream() = stream;
     Program::extractResult(vertex_list, h_out);
+    return hipStreamSynchronize(stream);
+  }
+
+  // Predecessors of the last run (csr_problem.cuh:459-483); needs
+  // MARK_PREDECESSORS.
+  hipError_t ExtractPreds(int *h_preds) {
+    if (!MARK_PREDECESSORS || !ws.preds) return hipErrorInvalidValue;
+    GAS_RETURN_IF(hipStreamSynchronize(stream));
+    if constexpr (detail::has_extract_pred<Program>::value) {
+      engine_stream() = stream;
+      Program::extractPred(ws.preds, graph.nodes, h_preds);
+    } else {
+      GAS_RETURN_IF(hipMemcpy(h_preds, ws.preds, sizeof(int) * (size_t)graph.nodes, hipMemcpyDeviceToHost));
+    }
     return hipStreamSynchronize(stream);
   }
 };

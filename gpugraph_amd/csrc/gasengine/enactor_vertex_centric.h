@@ -2,17 +2,27 @@
 // GASengine/enactor_vertex_centric.cuh:96-5072; its single-GPU sequence is
 // gather -> apply -> post_apply -> expand -> contract, :3454-3771).
 //
-// Per iteration on the problem's stream:
-//   [gather]  k_gather over in/out edges of the frontier    (gatherOverEdges)
-//   if |F| > threshold  (LBS):
-//     k_lbs_count  (apply + post_apply fused as the per-item prologue)
-//     k_lbs_scan
-//     k_lbs_expand (expand_edge + contract + dedup + enqueue per edge)
-//   else (tiered):
-//     k_frontier_prologue (apply + post_apply), if the program has them
-//     k_tiered_expand
-//   read back |F_next| (one 4-byte D2H per iteration, as the reference does
-//   at enactor_vertex_centric.cuh:3801-3804), stop at 0 or iter_num.
+// One iteration on the problem's stream:
+//   k_gas_ctl       stop test and LBS-or-tiered choice on the device (GasCtl)
+//   [gather]        frontier gather over in- and/or out-edges, load-balanced
+//                   by merge path (gather.h: LBS count / scan / partition,
+//                   k_lbs_gather, k_lbs_gather_fixup)
+//   expand, LBS     k_lbs_count (apply + post_apply fused as the per-item
+//                   prologue), k_lbs_scan, k_lbs_partition, k_lbs_expand
+//                   (expand_edge + contract + dedup + enqueue per edge)
+//   expand, tiered  k_frontier_prologue (apply + post_apply) when the program
+//                   has them, k_tiered_expand
+// Every kernel reads its frontier size from GasCtl, which holds 0 for the
+// path not taken and for all kernels once the run is over.
+//
+// Device-driven (default): kLoopIters iterations are captured once into a
+// hipGraph (both expansion paths present, one gated off per iteration),
+// cached on the problem with the arguments it was captured with, and
+// replayed until the loop state says done: one host read per replay instead
+// of the reference's host read and synchronisation per iteration
+// (enactor_vertex_centric.cuh:3801-3804).  Host-driven (a KernelTimer is
+// set, or DEBUG output is on): the same kernels one iteration at a time, only
+// the chosen expansion launched, with per-iteration records.
 //
 // post_apply runs right after apply in the same thread: the barrier the
 // reference places between them (kernel boundary) only matters for programs
@@ -25,9 +35,12 @@
 #include <algorithm>
 #include <climits>
 #include <cstdio>
+#include <cstring>
+#include <optional>
 #include <vector>
 
 #include "csr_problem.h"
+#include "gather.h"
 
 namespace GASengine {
 
@@ -44,6 +57,7 @@ struct EnactorConfig {
   int32_t block_degree = 4096;
   int32_t expand_grid = 2048;  // persistent workgroups for k_lbs_expand
   int32_t expand_vt = kExpandVt;  // LBS tile = kExpandBlock * expand_vt positions (2, 4 or 8)
+  bool device_loop = true;     // capture iterations into a hipGraph (off: one host decision per iteration)
 };
 
 // Per-iteration record: frontier size in, edges expanded, frontier size out.
@@ -54,16 +68,76 @@ struct IterationRecord {
   int lbs;
 };
 
+// Iterations per captured replay (even: the ping-pong buffers keep their
+// parity across replays).
+constexpr int kLoopIters = 8;
+
+template <int DUMMY = 0>
+__global__ void k_gas_ctl_init(GasCtl *c, int32_t *sizes, int32_t nf0, int32_t threshold, int32_t iter_num) {
+  c->iter = 0;
+  c->done = 0;
+  c->lbs_size = c->tier_size = c->gather_size = 0;
+  c->threshold = threshold;
+  c->iter_num = iter_num;
+  c->iterations = 0;
+  c->lbs_iters = 0;
+  c->sum_frontier = 0;
+  c->sum_next = 0;
+  sizes[0] = nf0;
+  sizes[1] = 0;
+}
+
+// Start of an iteration whose input frontier is sizes[sel].
+template <int DUMMY = 0>
+__global__ void k_gas_ctl(GasCtl *c, int32_t *sizes, int sel) {
+  if (c->done) return;
+  const int32_t nf = sizes[sel];
+  if (c->iterations > 0) c->sum_next += nf;  // the previous iteration's output
+  if (nf == 0 || c->iterations >= c->iter_num) {
+    c->done = 1;
+    c->lbs_size = c->tier_size = c->gather_size = 0;
+    return;
+  }
+  sizes[sel ^ 1] = 0;
+  c->iter = c->iterations;
+  c->iterations++;
+  c->sum_frontier += nf;
+  const bool lbs = nf > c->threshold;
+  c->lbs_iters += lbs ? 1 : 0;
+  c->lbs_size = lbs ? nf : 0;
+  c->tier_size = lbs ? 0 : nf;
+  c->gather_size = nf;
+}
+
+// MARK_PREDECESSORS: -1 everywhere, then the initial frontier (the sources)
+// as its own predecessors.
+template <int DUMMY = 0>
+__global__ void k_preds_sources(int32_t *preds, const int32_t *frontier, const int32_t *fsize) {
+  const int32_t nf = *fsize;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += (int64_t)gridDim.x * blockDim.x)
+    preds[frontier[i]] = frontier[i];
+}
+
 template <class CsrProblemT, class Program, bool INSTRUMENT = false>
 class EnactorVertexCentric {
  public:
   EnactorConfig config;
   KernelTimer *timer = nullptr;
-  std::vector<IterationRecord> history;
+  std::vector<IterationRecord> history;  // host-driven runs only
   int64_t total_edges = 0;
+  // Totals of the last run (both loop kinds).
+  int32_t iterations_run = 0;
+  int32_t lbs_iterations = 0;
+  int64_t frontier_total = 0, next_total = 0;
+  int32_t replays = 0;  // device-driven runs: graph launches
   int verbose = 0;
+  std::optional<Config> cfg;  // held when constructed from one (see CsrProblem::cfg)
 
   explicit EnactorVertexCentric(bool verbose_ = false) : verbose(verbose_) {}
+  // Reference constructor (enactor_vertex_centric.cuh:273): the driver's
+  // parameters and its DEBUG flag (per-iteration output).
+  EnactorVertexCentric(const Config &cfg_, bool DEBUG) : verbose(DEBUG ? 1 : 0), cfg(cfg_) {}
+  explicit EnactorVertexCentric(const Config &cfg_) : cfg(cfg_) {}
 
   // Reference 7-argument form (as called by Algorithms/SSSP/sssp.cu:397-398).
   hipError_t EnactIterativeSearch(CsrProblemT &problem, const int32_t * /*h_row_offsets*/, int directed,
@@ -74,6 +148,8 @@ class EnactorVertexCentric {
     const int32_t n = g.nodes;
     history.clear();
     total_edges = 0;
+    iterations_run = lbs_iterations = replays = 0;
+    frontier_total = next_total = 0;
 
     GAS_RETURN_IF(problem.Reset());
     GAS_RETURN_IF(ws.AllocateValues());
@@ -85,113 +161,253 @@ class EnactorVertexCentric {
                         (int *)g.column_offsets, (int *)g.row_indices, (typename Program::DataType *)g.edge_values,
                         problem.vertex_list, problem.edge_list, keys, vals);
     GAS_RETURN_IF(hipGetLastError());
-    int32_t nf = Program::srcVertex() == SINGLE ? num_srcs : n;
-    GAS_RETURN_IF(fill_async<int32_t>(ws.sizes, nf, 1, s));
+    const int32_t nf0 = Program::srcVertex() == SINGLE ? num_srcs : n;
+    hipLaunchKernelGGL(k_gas_ctl_init<>, dim3(1), dim3(1), 0, s, ws.ctl, ws.sizes, nf0, threshold,
+                       iter_num < 0 ? 0 : iter_num);
+    if (CsrProblemT::kMarkPredecessors) {
+      GAS_RETURN_IF(ws.AllocatePreds());
+      GAS_RETURN_IF(fill_async<int32_t>(ws.preds, -1, n, s));
+      hipLaunchKernelGGL(k_preds_sources<>, dim3(grid_for(nf0, 256, 4096)), dim3(256), 0, s, ws.preds, ws.frontier[0],
+                         ws.sizes);
+    }
+    GAS_RETURN_IF(hipGetLastError());
+    if (Program::gatherOverEdges() != NO_GATHER_EDGES)
+      GAS_RETURN_IF(ws.EnsureCarry(sizeof(GatherCarry<typename Program::GatherType>) *
+                                   (size_t)gather_carry_entries((int64_t)n + g.edges)));
 
+    GasState<Program> st = state(problem);
+    if (config.device_loop && !timer && !verbose) return device_loop(problem, st, threshold);
+    return host_loop(problem, st, threshold, iter_num);
+  }
+
+ private:
+  GasState<Program> state(CsrProblemT &problem) {
+    Workspace &ws = problem.ws;
     GasState<Program> st;
-    st.g = g;
+    std::memset(static_cast<void *>(&st), 0, sizeof(st));  // padding too: the state's bytes key the captured loop
+    st.g = problem.graph;
     st.vl = problem.vertex_list;
     st.el = problem.edge_list;
     st.gather_tmp = (typename Program::GatherType *)ws.gather_tmp;
     st.changed = ws.changed;
     st.do_apply = Program::applyOverEdges() != NO_APPLY_VERTICES;
     st.do_post = Program::postApplyOverEdges() != NO_POST_APPLY_VERTICES;
+    st.iter_ptr = &ws.ctl->iter;
+    st.preds = CsrProblemT::kMarkPredecessors ? ws.preds : nullptr;
+    return st;
+  }
+
+  // Tiered grid for device-driven loops: enough workgroups for a frontier of
+  // `threshold` items (the largest it handles), the kernel strides beyond.
+  int tier_grid(int threshold, int32_t n) const {
+    const int64_t cap = std::min<int64_t>((int64_t)threshold < 0 ? 0 : threshold, n);
+    return grid_for(std::max<int64_t>(cap, 1), kExpandBlock, 2048);
+  }
+
+  // The kernels of iteration parity `sel`.  mode: 0 both expansions (gated on
+  // the device), 1 LBS only, 2 tiered only.  nf_host sizes grids (n when
+  // unknown).
+  void issue_iteration(CsrProblemT &problem, GasState<Program> st, int sel, int mode, int64_t nf_host, int tgrid,
+                       hipStream_t s) {
+    Workspace &ws = problem.ws;
+    const DeviceGraph &g = problem.graph;
+    const int32_t n = g.nodes;
+    GasCtl *c = ws.ctl;
+    int32_t *f_in = ws.frontier[sel], *f_out = ws.frontier[sel ^ 1];
+    int32_t *sz_out = ws.sizes + (sel ^ 1);
+    st.dedup_next = ws.dedup[sel];
+    st.dedup_prev = ws.dedup[sel ^ 1];
+    hipLaunchKernelGGL(k_gas_ctl<>, dim3(1), dim3(1), 0, s, c, ws.sizes, sel);
+    const int64_t max_path = (int64_t)n + g.edges;
     const ExpandEdges ex = Program::expandOverEdges();
     const GatherEdges ga = Program::gatherOverEdges();
 
+    if (ga != NO_GATHER_EDGES) {
+      tb("gather");
+      using GC = GatherCarry<typename Program::GatherType>;
+      GC *carry = (GC *)ws.gcarry;
+      for (int pass = 0; pass < 2; pass++) {
+        const bool in = pass == 0;
+        if (in && !(ga == GATHER_IN_EDGES || ga == GATHER_ALL_EDGES)) continue;
+        if (!in && !(ga == GATHER_OUT_EDGES || ga == GATHER_ALL_EDGES)) continue;
+        const bool accum = !in && ga == GATHER_ALL_EDGES;
+        GatherRows<Program> gop{st, in ? 1 : 0};
+        hipLaunchKernelGGL((k_lbs_count<GatherRows<Program>, false>), dim3(grid_for(nf_host, kCountTile, kCountGridMax)),
+                           dim3(kCountBlock), 0, s, gop, f_in, &c->gather_size, ws.scan_local, ws.partials);
+        hipLaunchKernelGGL(k_lbs_scan<kScanBlock>, dim3(1), dim3(kScanBlock), 0, s, &c->gather_size, ws.partials,
+                           ws.info, (unsigned long long *)nullptr);
+        hipLaunchKernelGGL(k_lbs_partition<kGatherTile>, dim3(grid_for(max_path / kGatherTile + 2, 256, 4096)),
+                           dim3(256), 0, s, &c->gather_size, ws.scan_local, ws.partials, ws.info, ws.parts);
+        if (accum) {
+          hipLaunchKernelGGL((k_lbs_gather<Program, true>), dim3(config.expand_grid), dim3(kExpandBlock), 0, s, gop,
+                             f_in, &c->gather_size, ws.scan_local, ws.partials, ws.info, ws.parts, carry);
+          hipLaunchKernelGGL((k_lbs_gather_fixup<Program, true>), dim3(grid_for(max_path / kGatherTile + 2, 256, 1024)),
+                             dim3(256), 0, s, gop, f_in, ws.info, carry);
+        } else {
+          hipLaunchKernelGGL((k_lbs_gather<Program, false>), dim3(config.expand_grid), dim3(kExpandBlock), 0, s, gop,
+                             f_in, &c->gather_size, ws.scan_local, ws.partials, ws.info, ws.parts, carry);
+          hipLaunchKernelGGL((k_lbs_gather_fixup<Program, false>), dim3(grid_for(max_path / kGatherTile + 2, 256, 1024)),
+                             dim3(256), 0, s, gop, f_in, ws.info, carry);
+        }
+      }
+      te("gather");
+    }
+
+    int passes[2];
+    int np = 0;
+    if (ex == EXPAND_OUT_EDGES || ex == EXPAND_ALL_EDGES) passes[np++] = 0;
+    if (ex == EXPAND_IN_EDGES || ex == EXPAND_ALL_EDGES) passes[np++] = 1;
+    if (np == 0) {
+      // No expansion: run the vertex phase only; the frontier empties.
+      tb("prologue");
+      hipLaunchKernelGGL(k_frontier_prologue<ProgramOp<Program>>, dim3(grid_for(nf_host, 256, 4096)), dim3(256), 0, s,
+                         ProgramOp<Program>{st}, f_in, &c->gather_size);
+      te("prologue");
+    }
+    for (int p = 0; p < np; p++) {
+      st.in_edges = passes[p];
+      ProgramOp<Program> op{st};
+      const bool first = p == 0;
+      if (mode != 2) {
+        static const char *names[4] = {"lbs_count", "lbs_scan", "lbs_partition", "lbs_expand"};
+        LbsBuffers b{ws.scan_local, ws.partials, ws.info, ws.parts, ws.counters, config.expand_grid,
+                     config.expand_vt};
+        lbs_launch(op, first, f_in, &c->lbs_size, nf_host, max_path, b, f_out, sz_out, s, [&](int phase) {
+          if (phase > 0) te(names[phase - 1]);
+          if (phase < 4) tb(names[phase]);
+        });
+      }
+      if (mode != 1) {
+        const int grid = mode == 2 ? grid_for(nf_host, kExpandBlock) : tgrid;
+        if (first && (st.do_apply || st.do_post)) {
+          tb("prologue");
+          hipLaunchKernelGGL(k_frontier_prologue<ProgramOp<Program>>, dim3(mode == 2 ? grid_for(nf_host, 256) : tgrid),
+                             dim3(256), 0, s, op, f_in, &c->tier_size);
+          te("prologue");
+          tb("tiered_expand");
+          hipLaunchKernelGGL((k_tiered_expand<ProgramOp<Program>, false>), dim3(grid), dim3(kExpandBlock), 0, s, op,
+                             f_in, &c->tier_size, f_out, sz_out, config.wave_degree, config.block_degree,
+                             ws.counters);
+          te("tiered_expand");
+        } else {
+          // No apply phase: the prologue only clears dedup_prev, safe to fuse.
+          tb("tiered_expand");
+          if (first)
+            hipLaunchKernelGGL((k_tiered_expand<ProgramOp<Program>, true>), dim3(grid), dim3(kExpandBlock), 0, s, op,
+                               f_in, &c->tier_size, f_out, sz_out, config.wave_degree, config.block_degree,
+                               ws.counters);
+          else
+            hipLaunchKernelGGL((k_tiered_expand<ProgramOp<Program>, false>), dim3(grid), dim3(kExpandBlock), 0, s, op,
+                               f_in, &c->tier_size, f_out, sz_out, config.wave_degree, config.block_degree,
+                               ws.counters);
+          te("tiered_expand");
+        }
+      }
+    }
+  }
+
+  // Read-back block: the loop state, both frontier sizes and the edge counter.
+  struct Snapshot {
+    GasCtl c;
+    int32_t sizes[2];
+    unsigned long long edges;
+  };
+
+  hipError_t snapshot(Workspace &ws, hipStream_t s, Snapshot *out) {
+    char *h = reinterpret_cast<char *>(ws.h_ctl);
+    GAS_RETURN_IF(hipMemcpyAsync(h, ws.ctl, sizeof(GasCtl), hipMemcpyDeviceToHost, s));
+    GAS_RETURN_IF(hipMemcpyAsync(h + sizeof(GasCtl), ws.sizes, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GAS_RETURN_IF(
+        hipMemcpyAsync(h + sizeof(GasCtl) + 8, ws.counters, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    GAS_RETURN_IF(hipStreamSynchronize(s));
+    std::memcpy(&out->c, h, sizeof(GasCtl));
+    std::memcpy(out->sizes, h + sizeof(GasCtl), 8);
+    std::memcpy(&out->edges, h + sizeof(GasCtl) + 8, 8);
+    return hipSuccess;
+  }
+
+  hipError_t host_loop(CsrProblemT &problem, GasState<Program> st, int threshold, int iter_num) {
+    hipStream_t s = problem.stream;
+    Workspace &ws = problem.ws;
+    Snapshot snap;
+    GAS_RETURN_IF(snapshot(ws, s, &snap));
+    int32_t nf = snap.sizes[0];
     for (int it = 0; it < iter_num && nf > 0; it++) {
       const int sel = it & 1;
-      int32_t *f_in = ws.frontier[sel], *f_out = ws.frontier[sel ^ 1];
-      int32_t *sz_in = ws.sizes + sel, *sz_out = ws.sizes + (sel ^ 1);
-      st.iteration = it;
-      st.dedup_next = ws.dedup[sel];
-      st.dedup_prev = ws.dedup[sel ^ 1];
-      GAS_RETURN_IF(hipMemsetAsync(sz_out, 0, sizeof(int32_t), s));
-
-      if (ga != NO_GATHER_EDGES) {
-        const int grid = grid_for((int64_t)nf * kWave, 256, 16384);
-        if (ga == GATHER_IN_EDGES || ga == GATHER_ALL_EDGES) {
-          tb("gather");
-          hipLaunchKernelGGL((k_gather<Program, false>), dim3(grid), dim3(256), 0, s, st, f_in, sz_in, 1);
-          te("gather");
-        }
-        if (ga == GATHER_OUT_EDGES || ga == GATHER_ALL_EDGES) {
-          tb("gather");
-          if (ga == GATHER_ALL_EDGES)
-            hipLaunchKernelGGL((k_gather<Program, true>), dim3(grid), dim3(256), 0, s, st, f_in, sz_in, 0);
-          else
-            hipLaunchKernelGGL((k_gather<Program, false>), dim3(grid), dim3(256), 0, s, st, f_in, sz_in, 0);
-          te("gather");
-        }
-        GAS_RETURN_IF(hipGetLastError());
-      }
-
-      int passes[2];
-      int np = 0;
-      if (ex == EXPAND_OUT_EDGES || ex == EXPAND_ALL_EDGES) passes[np++] = 0;
-      if (ex == EXPAND_IN_EDGES || ex == EXPAND_ALL_EDGES) passes[np++] = 1;
       const bool lbs = nf > threshold;
-      if (np == 0) {
-        // No expansion: run the vertex phase only; the frontier empties.
-        tb("prologue");
-        hipLaunchKernelGGL(k_frontier_prologue<ProgramOp<Program>>, dim3(grid_for(nf, 256)), dim3(256), 0, s,
-                           ProgramOp<Program>{st}, f_in, sz_in);
-        te("prologue");
-      }
-      for (int p = 0; p < np; p++) {
-        st.in_edges = passes[p];
-        ProgramOp<Program> op{st};
-        const bool first = p == 0;
-        if (lbs) {
-          static const char *names[4] = {"lbs_count", "lbs_scan", "lbs_partition", "lbs_expand"};
-          LbsBuffers b{ws.scan_local, ws.partials, ws.info, ws.parts, ws.counters, config.expand_grid,
-                       config.expand_vt};
-          lbs_launch(op, first, f_in, sz_in, nf, (int64_t)n + g.edges, b, f_out, sz_out, s, [&](int phase) {
-            if (phase > 0) te(names[phase - 1]);
-            if (phase < 4) tb(names[phase]);
-          });
-        } else {
-          if (first && (st.do_apply || st.do_post)) {
-            tb("prologue");
-            hipLaunchKernelGGL(k_frontier_prologue<ProgramOp<Program>>, dim3(grid_for(nf, 256)), dim3(256), 0, s,
-                               op, f_in, sz_in);
-            te("prologue");
-            tb("tiered_expand");
-            hipLaunchKernelGGL((k_tiered_expand<ProgramOp<Program>, false>), dim3(grid_for(nf, kExpandBlock)),
-                               dim3(kExpandBlock), 0, s, op, f_in, sz_in, f_out, sz_out, config.wave_degree,
-                               config.block_degree, ws.counters);
-            te("tiered_expand");
-          } else {
-            // No apply phase: the prologue only clears dedup_prev, safe to fuse.
-            tb("tiered_expand");
-            if (first)
-              hipLaunchKernelGGL((k_tiered_expand<ProgramOp<Program>, true>), dim3(grid_for(nf, kExpandBlock)),
-                                 dim3(kExpandBlock), 0, s, op, f_in, sz_in, f_out, sz_out, config.wave_degree,
-                                 config.block_degree, ws.counters);
-            else
-              hipLaunchKernelGGL((k_tiered_expand<ProgramOp<Program>, false>), dim3(grid_for(nf, kExpandBlock)),
-                                 dim3(kExpandBlock), 0, s, op, f_in, sz_in, f_out, sz_out, config.wave_degree,
-                                 config.block_degree, ws.counters);
-            te("tiered_expand");
-          }
-        }
-        GAS_RETURN_IF(hipGetLastError());
-      }
-      GAS_RETURN_IF(hipMemcpyAsync(ws.h_sizes, sz_out, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-      GAS_RETURN_IF(hipMemcpyAsync(ws.h_sizes + 2, ws.counters, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-      GAS_RETURN_IF(hipStreamSynchronize(s));
-      const int32_t next = ws.h_sizes[0];
-      unsigned long long seen;
-      std::memcpy(&seen, ws.h_sizes + 2, sizeof(seen));
-      history.push_back({nf, (int64_t)seen - total_edges, next, lbs ? 1 : 0});
-      total_edges = (int64_t)seen;
+      issue_iteration(problem, st, sel, lbs ? 1 : 2, nf, 0, s);
+      GAS_RETURN_IF(hipGetLastError());
+      GAS_RETURN_IF(snapshot(ws, s, &snap));
+      const int32_t next = snap.sizes[sel ^ 1];
+      history.push_back({nf, (int64_t)snap.edges - total_edges, next, lbs ? 1 : 0});
+      total_edges = (int64_t)snap.edges;
       if (verbose) std::printf("iteration %d: frontier %d -> %d (%s)\n", it, nf, next, lbs ? "lbs" : "tiered");
       nf = next;
+    }
+    iterations_run = (int32_t)history.size();
+    for (auto &r : history) {
+      frontier_total += r.frontier;
+      next_total += r.next;
+      lbs_iterations += r.lbs;
     }
     return hipSuccess;
   }
 
- private:
+  hipError_t device_loop(CsrProblemT &problem, GasState<Program> st, int threshold) {
+    hipStream_t s = problem.stream;
+    Workspace &ws = problem.ws;
+    const int32_t n = problem.graph.nodes;
+    const int tgrid = tier_grid(threshold, n);
+    std::vector<uint64_t> key;
+    auto add = [&](const void *p, size_t bytes) {
+      const unsigned char *b = static_cast<const unsigned char *>(p);
+      for (size_t i = 0; i < bytes; i += 8) {
+        uint64_t w = 0;
+        std::memcpy(&w, b + i, std::min<size_t>(8, bytes - i));
+        key.push_back(w);
+      }
+    };
+    add(&st, sizeof(st));
+    add(&config, sizeof(config));
+    const void *bufs[] = {ws.frontier[0], ws.frontier[1], ws.dedup[0], ws.dedup[1], ws.sizes, ws.counters,
+                          ws.scan_local, ws.partials, ws.parts, ws.info, ws.gcarry};
+    add(bufs, sizeof(bufs));
+    key.push_back((uint64_t)tgrid);
+    if (!ws.loop_exec || key != ws.loop_key) {
+      ws.ResetLoop();
+      if (!ws.cap_stream) GAS_RETURN_IF(hipStreamCreateWithFlags(&ws.cap_stream, hipStreamNonBlocking));
+      hipStream_t cs = ws.cap_stream;
+      hipGraph_t graph = nullptr;
+      GAS_RETURN_IF(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      for (int j = 0; j < kLoopIters; j++) issue_iteration(problem, st, j & 1, 0, n, tgrid, cs);
+      hipError_t e = hipStreamEndCapture(cs, &graph);
+      if (e != hipSuccess) return e;
+      e = hipGraphInstantiate(&ws.loop_exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        ws.loop_exec = nullptr;
+        return e;
+      }
+      ws.loop_key = key;
+    }
+    Snapshot snap;
+    for (;;) {
+      GAS_RETURN_IF(hipGraphLaunch(ws.loop_exec, s));
+      replays++;
+      GAS_RETURN_IF(snapshot(ws, s, &snap));
+      // Finished: the controller saw the end, or the replay's last iteration
+      // left nothing (or reached iter_num) and no controller has run since.
+      if (snap.c.done || snap.sizes[0] == 0 || snap.c.iterations >= snap.c.iter_num) break;
+    }
+    if (!snap.c.done) snap.c.sum_next += snap.sizes[0];
+    iterations_run = snap.c.iterations;
+    lbs_iterations = snap.c.lbs_iters;
+    frontier_total = snap.c.sum_frontier;
+    next_total = snap.c.sum_next;
+    total_edges = (int64_t)snap.edges;
+    return hipSuccess;
+  }
+
   void tb(const char *name) {
     if (timer) timer->begin(name);
   }

@@ -1,7 +1,7 @@
 // Adapters from a vertex Program (the reference's functor concept,
 // doc/mapgraph-overview.h:246-255; e.g. Algorithms/SSSP/sssp.h:34-393) to the
 // engine's kernels: the expand/contract edge op used by the LBS and tiered
-// expand kernels, and the pull-gather kernel.
+// expand kernels (the gather's item op is in gather.h).
 //
 // Functor arities: the canonical forms are those of sssp.h / cc.h /
 // PageRank.h (apply: 6 args, contract: 6 args).  The fork's BFS contract
@@ -86,6 +86,26 @@ struct has_gather_vertex<P, std::void_t<typename P::gather_vertex>> : std::true_
 
 }  // namespace detail
 
+// Device-side loop state of a GAS run.  k_gas_ctl, the first kernel of every
+// iteration, decides on the device what the host decides in the reference
+// (enactor_vertex_centric.cuh:3637-3667, 3801-3804): stop when the frontier
+// is empty or iter_num iterations have run, else choose the expansion
+// (frontier > threshold: LBS, otherwise the tiered kernel).  The kernels of
+// the iteration read the frontier size from the field meant for them
+// (lbs_size, tier_size, gather_size), which is 0 for the path not taken and
+// for every kernel once the loop is done, so a captured sequence of
+// iterations runs exactly the iterations the run needs.
+struct GasCtl {
+  int32_t iter;        // number of the iteration being run (what the functors see)
+  int32_t done;
+  int32_t lbs_size, tier_size, gather_size;
+  int32_t threshold, iter_num;
+  int32_t iterations;  // iterations run so far
+  int32_t lbs_iters;   // of which expanded by LBS
+  int32_t pad;
+  long long sum_frontier, sum_next;
+};
+
 // Iteration state shared by the kernels of one GAS iteration (by value).
 template <class Program>
 struct GasState {
@@ -100,6 +120,10 @@ struct GasState {
   int in_edges;  // 0: expand over CSR out-edges, 1: over CSC in-edges
   int do_apply;
   int do_post;
+  const int32_t *iter_ptr;  // device-driven loops: the iteration number lives here (GasCtl::iter)
+  int32_t *preds;           // MARK_PREDECESSORS: discovering vertex per vertex, else NULL
+
+  __device__ __forceinline__ int iter() const { return iter_ptr ? *iter_ptr : iteration; }
 };
 
 // Expand + contract fused per edge: expand_edge -> contract -> bitmap dedup
@@ -117,7 +141,7 @@ struct ProgramOp {
     s.dedup_prev[v >> 5] = 0u;  // every set bit there belongs to a current-frontier vertex
     if (s.do_apply) {
       char ch = s.changed[v];
-      detail::call_apply<Program>(v, s.iteration, s.gather_tmp[v], s.vl, s.el, ch);
+      detail::call_apply<Program>(v, s.iter(), s.gather_tmp[v], s.vl, s.el, ch);
       s.changed[v] = ch;
     }
     if (s.do_post) {
@@ -151,76 +175,19 @@ struct ProgramOp {
     int frontier = -1;
     typename Program::MiscType misc = typename Program::MiscType();
     typename Program::expand_edge fe;
-    fe(true, s.iteration, v, nb, eid, s.vl, s.el, frontier, misc);
+    const int it = s.iter();
+    fe(true, it, v, nb, eid, s.vl, s.el, frontier, misc);
     if (frontier == -1) return false;
-    detail::call_contract<Program>(s.iteration, frontier, s.vl, s.el, s.gather_tmp, misc,
+    detail::call_contract<Program>(it, frontier, s.vl, s.el, s.gather_tmp, misc,
                                    reinterpret_cast<unsigned char *>(s.dedup_next));
     if (frontier == -1) return false;
     if (!bit_test_and_set(s.dedup_next, frontier)) return false;
+    if (s.preds) s.preds[frontier] = v;  // the winner of the dedup bit discovered it
     out = frontier;
     return true;
   }
 
   __device__ __forceinline__ void finish() {}
 };
-
-// Pull gather over in-edges (GATHER_IN_EDGES: CSC) or out-edges
-// (GATHER_OUT_EDGES: CSR) of each frontier vertex, reduced with gather_sum
-// starting from INIT_VALUE, stored to gather_tmp[v] (the reference
-// materialises (dst, value) pairs and runs ReduceByKey, enactor_vertex_centric
-// .cuh:2104-2424).  One wave per vertex; lanes stride the edge list.
-// ACCUM: add to gather_tmp[v] instead of overwriting (second pass of
-// GATHER_ALL_EDGES).
-template <class Program, bool ACCUM>
-__global__ void __launch_bounds__(256)
-k_gather(GasState<Program> s, const int32_t *__restrict__ frontier, const int32_t *__restrict__ fsize, int in_edges) {
-  using G = typename Program::GatherType;
-  const int32_t nf = *fsize;
-  const int64_t waves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  typename Program::gather_edge ge;
-  typename Program::gather_sum gs;
-  for (int64_t i = wid; i < nf; i += waves) {
-    const int32_t v = frontier[i];
-    const int32_t *off = in_edges ? s.g.column_offsets : s.g.row_offsets;
-    const int32_t b = off[v], e = off[v + 1];
-    G acc = (G)Program::INIT_VALUE;
-    bool any = false;
-    for (int32_t k = b + lane_id(); k < e; k += kWave) {
-      int32_t nb, eid;
-      if (in_edges) {
-        nb = s.g.row_indices[k];
-        eid = s.g.csc_edge_ids ? s.g.csc_edge_ids[k] : k;
-      } else {
-        nb = s.g.column_indices[k];
-        eid = k;
-      }
-      G val;
-      ge(v, eid, nb, s.vl, s.el, val);
-      acc = any ? gs(acc, val) : val;
-      any = true;
-    }
-    // Lanes without edges hold INIT_VALUE and are excluded from the reduction.
-#pragma unroll
-    for (int off2 = 32; off2 > 0; off2 >>= 1) {
-      G o = __shfl_xor(acc, off2);
-      bool oany = __shfl_xor((int)any, off2);
-      if (any && oany)
-        acc = gs(acc, o);
-      else if (oany)
-        acc = o;
-      any = any || oany;
-    }
-    if (lane_id() == 0) {
-      G r = any ? acc : (G)Program::INIT_VALUE;
-      if (ACCUM) r = gs(s.gather_tmp[v], r);
-      s.gather_tmp[v] = r;
-      if constexpr (detail::has_gather_vertex<Program>::value) {
-        typename Program::gather_vertex gv;
-        gv(v, r, s.vl, s.el);
-      }
-    }
-  }
-}
 
 }  // namespace GASengine

@@ -49,6 +49,14 @@ def lib():
         L.oracle_pagerank_adjoint_f64.argtypes = [C.c_int32, _I, _I, _D, C.c_double, _D, _D, C.c_double, C.c_int32,
                                                   _D, _D, _D]
         L.oracle_pagerank_adjoint_f64.restype = C.c_int32
+        L.oracle_cc_omp.argtypes = [C.c_int32, _I, _I, _I]
+        L.oracle_cc_omp.restype = C.c_int32
+        L.oracle_pagerank_f64_omp.argtypes = [C.c_int32, _I, _I, _I, _D, _D, C.c_double, C.c_double, C.c_int32, _D,
+                                              _D]
+        L.oracle_pagerank_f64_omp.restype = C.c_int32
+        L.oracle_pagerank_fd_f64.argtypes = [C.c_int32, _I, _I, _D, C.c_double, _D, _D, C.c_int32,
+                                             C.POINTER(C.c_int64), _D, C.c_double, C.c_int32, _D]
+        L.oracle_pagerank_fd_f64.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -116,6 +124,14 @@ def cc(ro, col):
     return out, sweeps
 
 
+def cc_omp(ro, col):
+    """Min-label CC on all host cores (same labels as cc)."""
+    ro, col = np.ascontiguousarray(ro, np.int32), np.ascontiguousarray(col, np.int32)
+    out = np.empty(len(ro) - 1, np.int32)
+    sweeps = lib().oracle_cc_omp(len(ro) - 1, _i(ro), _i(col), out.ctypes.data_as(_I))
+    return out, sweeps
+
+
 def csc(ro, col):
     """Stable CSC (rows ascending within a column) of a CSR, in numpy."""
     n = len(ro) - 1
@@ -144,6 +160,19 @@ def pagerank_f64(ro, col, p=None, t=None, d=0.85, tol=1e-12, max_iter=10000):
     return r, it, res.value
 
 
+def pagerank_f64_omp(ro, co, ri, p_csc=None, t=None, d=0.85, tol=1e-12, max_iter=10000):
+    """pagerank_f64 on all host cores over the CSC (co, ri); p_csc = transition
+    weights in CSC slot order (None: 1/outdeg)."""
+    n = len(ro) - 1
+    r = np.empty(n, np.float64)
+    res = C.c_double()
+    p_csc = None if p_csc is None else np.ascontiguousarray(p_csc, np.float64)
+    t = None if t is None else np.ascontiguousarray(t, np.float64)
+    it = lib().oracle_pagerank_f64_omp(n, _i(ro), _i(co), _i(ri), _d(p_csc), _d(t), d, tol, max_iter,
+                                       r.ctypes.data_as(_D), C.byref(res))
+    return r, it, res.value
+
+
 def pagerank_adjoint_f64(ro, col, w, r, c, d=0.85, tol=1e-14, max_iter=10000):
     n = len(ro) - 1
     m = len(col)
@@ -156,3 +185,19 @@ def pagerank_adjoint_f64(ro, col, w, r, c, d=0.85, tol=1e-14, max_iter=10000):
     it = lib().oracle_pagerank_adjoint_f64(n, _i(ro), _i(col), _d(w), d, _d(r), _d(c), tol, max_iter,
                                            lam.ctypes.data_as(_D), gw.ctypes.data_as(_D), gt.ctypes.data_as(_D))
     return lam, gw, gt, it
+
+
+def pagerank_fd_f64(ro, col, w, r, c, edges, h, d=0.85, tol_rel=1e-10, max_iter=10000):
+    """Central differences of L = c . r(w) on the given CSR edges (see
+    oracle_pagerank_fd_f64); r = the FP64 fixed point at w."""
+    n = len(ro) - 1
+    w = np.ascontiguousarray(w, np.float64)
+    r = np.ascontiguousarray(r, np.float64)
+    c = np.ascontiguousarray(c, np.float64)
+    edges = np.ascontiguousarray(edges, np.int64)
+    h = np.ascontiguousarray(h, np.float64)
+    fd = np.empty(len(edges), np.float64)
+    sweeps = lib().oracle_pagerank_fd_f64(n, _i(ro), _i(col), _d(w), d, _d(r), _d(c), len(edges),
+                                          edges.ctypes.data_as(C.POINTER(C.c_int64)), _d(h), tol_rel, max_iter,
+                                          fd.ctypes.data_as(_D))
+    return fd, sweeps

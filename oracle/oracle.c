@@ -216,7 +216,9 @@ int32_t oracle_cc(int32_t n, const int32_t *ro, const int32_t *col,
 
 /* Algorithms/PageRank/PageRank.cu:108-161 -- in-place (Gauss-Seidel) float
  * sweeps over the CSC; 0.15 + 0.85*sum is formed in double then stored to
- * float, as the reference's mixed float/double expression does. */
+ * float, as the reference's mixed float/double expression does, and the
+ * float |new - old| is compared with the double constant 0.01
+ * (PageRank.cu:146). */
 int32_t oracle_pr_reference(int32_t n, const int32_t *ro, const int32_t *co,
                             const int32_t *ri, float *rank) {
   for (int32_t i = 0; i < n; i++) rank[i] = 0.15f;
@@ -231,7 +233,7 @@ int32_t oracle_pr_reference(int32_t n, const int32_t *ro, const int32_t *co,
         sum += rank[nb] / (float)(ro[nb + 1] - ro[nb]);
       }
       sum = (float)(0.15 + 0.85 * (double)sum);
-      if (fabsf(sum - rank[v]) >= 0.01f) changed = 1;
+      if ((double)fabsf(sum - rank[v]) >= 0.01) changed = 1;
       rank[v] = sum;
     }
     iters++;
@@ -519,4 +521,209 @@ void oracle_sssp_omp(int32_t n, const int32_t *ro, const int32_t *col, const int
   }
   free(dirty);
   free(front);
+}
+
+/* ---- finite differences of the FP64 PageRank loss (see oracle.h) ---- */
+#define FD_K 64 /* perturbations per sweep: 32 edges x (+h, -h) */
+
+int32_t oracle_pagerank_fd_f64(int32_t n, const int32_t *ro, const int32_t *col,
+                               const double *w, double d, const double *r,
+                               const double *c, int32_t ne, const int64_t *edges,
+                               const double *h, double tol_rel, int32_t max_iter,
+                               double *fd) {
+  const int64_t m = ro[n];
+  const double inv_n = 1.0 / (double)n;
+  /* In-edges with their transition weights P_yx (CSC by counting sort). */
+  int32_t *co = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+  int32_t *src = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
+  double *pw = (double *)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+  double *S = (double *)malloc(sizeof(double) * (size_t)n);
+  int32_t *fill = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+  unsigned char *dang = (unsigned char *)malloc((size_t)n);
+  for (int32_t u = 0; u < n; u++) {
+    double s = 0.0;
+    for (int32_t e = ro[u]; e < ro[u + 1]; e++) s += w[e];
+    S[u] = s;
+    dang[u] = ro[u + 1] == ro[u];
+  }
+  for (int64_t e = 0; e < m; e++) co[col[e] + 1]++;
+  for (int32_t v = 0; v < n; v++) co[v + 1] += co[v];
+  memcpy(fill, co, sizeof(int32_t) * (size_t)n);
+  for (int32_t u = 0; u < n; u++)
+    for (int32_t e = ro[u]; e < ro[u + 1]; e++) {
+      int32_t k = fill[col[e]]++;
+      src[k] = u;
+      pw[k] = w[e] / S[u];
+    }
+  double *D = (double *)malloc(sizeof(double) * (size_t)n * FD_K);
+  double *Dn = (double *)malloc(sizeof(double) * (size_t)n * FD_K);
+  int32_t max_sweeps = 0;
+  for (int32_t base = 0; base < ne; base += FD_K / 2) {
+    const int32_t kb = (ne - base < FD_K / 2 ? ne - base : FD_K / 2) * 2; /* active perturbations */
+    int32_t pu[FD_K];
+    double sh[FD_K], scale[FD_K]; /* signed h; s h / (S_u + s h) */
+    int64_t pe[FD_K];
+    for (int32_t j = 0; j < FD_K; j++) {
+      const int32_t i = base + j / 2;
+      if (j >= kb) {
+        pu[j] = -1;
+        continue;
+      }
+      const int64_t e = edges[i];
+      int32_t lo = 0, hi = n; /* row of edge e */
+      while (hi - lo > 1) {
+        int32_t mid = (lo + hi) / 2;
+        if (ro[mid] <= e) lo = mid; else hi = mid;
+      }
+      pu[j] = lo;
+      pe[j] = e;
+      sh[j] = (j & 1) ? -h[i] : h[i];
+      scale[j] = sh[j] / (S[lo] + sh[j]);
+    }
+    /* P'_u. - P_u. = scale * (e_v - P_u.) on row u (v = the edge's target). */
+    memset(D, 0, sizeof(double) * (size_t)n * FD_K);
+    double diff[FD_K], norm[FD_K];
+    int32_t sweep = 0;
+    for (;;) {
+      double dsum[FD_K], du[FD_K];
+      for (int32_t j = 0; j < FD_K; j++) {
+        dsum[j] = 0.0;
+        du[j] = pu[j] >= 0 ? D[(size_t)pu[j] * FD_K + j] : 0.0;
+      }
+      for (int32_t y = 0; y < n; y++)
+        if (dang[y])
+          for (int32_t j = 0; j < FD_K; j++) dsum[j] += D[(size_t)y * FD_K + j];
+#pragma omp parallel for schedule(dynamic, 2048)
+      for (int32_t x = 0; x < n; x++) {
+        double acc[FD_K];
+        for (int32_t j = 0; j < FD_K; j++) acc[j] = dsum[j] * inv_n;
+        for (int32_t k = co[x]; k < co[x + 1]; k++) {
+          const double p = pw[k];
+          const double *dy = D + (size_t)src[k] * FD_K;
+#pragma omp simd
+          for (int32_t j = 0; j < FD_K; j++) acc[j] += p * dy[j];
+        }
+        double *o = Dn + (size_t)x * FD_K;
+        for (int32_t j = 0; j < FD_K; j++) o[j] = d * acc[j];
+      }
+      /* The changed row: d (P'_u. - P_u.) (D_u + r_u), applied on row u's targets. */
+      for (int32_t j = 0; j < kb; j++) {
+        const int32_t u = pu[j];
+        const double f = d * scale[j] * (du[j] + r[u]);
+        for (int32_t e = ro[u]; e < ro[u + 1]; e++) Dn[(size_t)col[e] * FD_K + j] -= f * (w[e] / S[u]);
+        Dn[(size_t)col[pe[j]] * FD_K + j] += f;
+      }
+      for (int32_t j = 0; j < FD_K; j++) diff[j] = norm[j] = 0.0;
+#pragma omp parallel
+      {
+        double ld[FD_K], ln[FD_K];
+        for (int32_t j = 0; j < FD_K; j++) ld[j] = ln[j] = 0.0;
+#pragma omp for schedule(static)
+        for (int32_t x = 0; x < n; x++)
+          for (int32_t j = 0; j < FD_K; j++) {
+            const double a = Dn[(size_t)x * FD_K + j];
+            ld[j] += fabs(a - D[(size_t)x * FD_K + j]);
+            ln[j] += fabs(a);
+          }
+#pragma omp critical
+        for (int32_t j = 0; j < FD_K; j++) {
+          diff[j] += ld[j];
+          norm[j] += ln[j];
+        }
+      }
+      double *t = D;
+      D = Dn;
+      Dn = t;
+      sweep++;
+      int conv = 1;
+      for (int32_t j = 0; j < kb; j++)
+        if (diff[j] > tol_rel * norm[j]) conv = 0;
+      if (conv || sweep >= max_iter) break;
+    }
+    if (sweep > max_sweeps) max_sweeps = sweep;
+    for (int32_t j = 0; j < kb; j += 2) {
+      double lp = 0.0, lm = 0.0;
+      for (int32_t x = 0; x < n; x++) {
+        lp += c[x] * D[(size_t)x * FD_K + j];
+        lm += c[x] * D[(size_t)x * FD_K + j + 1];
+      }
+      fd[base + j / 2] = (lp - lm) / (2.0 * sh[j]);
+    }
+  }
+  free(D);
+  free(Dn);
+  free(dang);
+  free(fill);
+  free(S);
+  free(pw);
+  free(src);
+  free(co);
+  return max_sweeps;
+}
+
+int32_t oracle_cc_omp(int32_t n, const int32_t *ro, const int32_t *col,
+                      int32_t *label) {
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < n; i++) label[i] = i;
+  int32_t sweeps = 0;
+  int changed = 1;
+  while (changed) {
+    changed = 0;
+    sweeps++;
+#pragma omp parallel for schedule(static) reduction(| : changed)
+    for (int32_t v = 0; v < n; v++) {
+      int32_t minnb = __atomic_load_n(&label[v], __ATOMIC_RELAXED);
+      const int32_t own = minnb;
+      for (int32_t e = ro[v]; e < ro[v + 1]; e++) {
+        const int32_t l = __atomic_load_n(&label[col[e]], __ATOMIC_RELAXED);
+        if (l < minnb) minnb = l;
+      }
+      if (minnb < own) {
+        __atomic_store_n(&label[v], minnb, __ATOMIC_RELAXED);
+        changed = 1;
+      }
+    }
+  }
+  return sweeps;
+}
+
+int32_t oracle_pagerank_f64_omp(int32_t n, const int32_t *ro, const int32_t *co,
+                                const int32_t *ri, const double *p_csc,
+                                const double *t, double d, double tol,
+                                int32_t max_iter, double *r, double *resid) {
+  double *x = (double *)malloc(sizeof(double) * (size_t)n); /* r_u * P_u. factor per source */
+  double *y = (double *)malloc(sizeof(double) * (size_t)n);
+  const double inv_n = 1.0 / (double)n;
+#pragma omp parallel for schedule(static)
+  for (int32_t v = 0; v < n; v++) r[v] = t ? t[v] : inv_n;
+  int32_t it = 0;
+  double res = INFINITY;
+  while (it < max_iter) {
+    double dangling = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : dangling)
+    for (int32_t u = 0; u < n; u++) {
+      const int32_t deg = ro[u + 1] - ro[u];
+      if (deg == 0) dangling += r[u];
+      x[u] = p_csc ? r[u] : (deg ? r[u] / (double)deg : 0.0);
+    }
+    res = 0.0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : res)
+    for (int32_t v = 0; v < n; v++) {
+      double acc = 0.0;
+      if (p_csc)
+        for (int32_t k = co[v]; k < co[v + 1]; k++) acc += p_csc[k] * x[ri[k]];
+      else
+        for (int32_t k = co[v]; k < co[v + 1]; k++) acc += x[ri[k]];
+      const double tv = t ? t[v] : inv_n;
+      y[v] = d * (acc + dangling * inv_n) + (1.0 - d) * tv;
+      res += fabs(y[v] - r[v]);
+    }
+    memcpy(r, y, sizeof(double) * (size_t)n);
+    it++;
+    if (res <= tol) break;
+  }
+  if (resid) *resid = res;
+  free(x);
+  free(y);
+  return it;
 }

@@ -92,6 +92,22 @@ int32_t oracle_pagerank_adjoint_f64(int32_t n, const int32_t *ro,
                                     double *lambda, double *grad_w,
                                     double *grad_t);
 
+/* Central finite differences of L(w) = sum_v c_v r_v(w) (r as in
+ * oracle_pagerank_f64 with P_uv = w_uv / S_u, any teleport) for `ne` CSR
+ * edges: fd[i] = (L(w + h_i e_i) - L(w - h_i e_i)) / (2 h_i).  `r` is the
+ * FP64 fixed point at w.  Each perturbed fixed point is r + D, where D solves
+ * the difference of the two fixed-point equations (only row u of P changes):
+ *   D = d (P'^T D + (delta . D) / n 1) + d r_u (P'_u. - P_u.)
+ * iterated from 0 until every ||D_new - D||_1 <= tol_rel ||D_new||_1, so
+ * L(w + h) - L(w - h) = c . (D+ - D-) carries no cancellation against L.
+ * Up to 32 edges (64 perturbations) share a sweep (OpenMP).  Returns the
+ * largest sweep count. */
+int32_t oracle_pagerank_fd_f64(int32_t n, const int32_t *ro, const int32_t *col,
+                               const double *w, double d, const double *r,
+                               const double *c, int32_t ne, const int64_t *edges,
+                               const double *h, double tol_rel, int32_t max_iter,
+                               double *fd);
+
 /* Parallel CPU baselines (OpenMP, oracle_threads() threads): outputs equal
  * to oracle_bfs / oracle_sssp_selection.  in_ro / in_col: in-edges (the
  * CSR itself for a symmetric graph); delta: bucket width (e.g. 128). */
@@ -102,6 +118,23 @@ void oracle_bfs_omp(int32_t n, const int32_t *ro, const int32_t *col,
 void oracle_sssp_omp(int32_t n, const int32_t *ro, const int32_t *col,
                      const int32_t *w, int32_t src, int32_t delta,
                      int32_t *dist);
+
+/* Min-label CC on all host cores: in-place sweeps as oracle_cc, each thread
+ * updating its own static chunk of vertices and reading the others' labels
+ * with relaxed atomic loads (labels only decrease, so a stale read only
+ * defers an update to the next sweep); same unique fixed point (the
+ * smallest id of each component).  Returns the sweep count. */
+int32_t oracle_cc_omp(int32_t n, const int32_t *ro, const int32_t *col,
+                      int32_t *label);
+
+/* oracle_pagerank_f64 on all host cores, as a pull over in-edges: co / ri =
+ * CSC, p_csc = transition weight per CSC slot (NULL: 1 / outdeg(u)), outdeg
+ * from ro.  Same iteration (Jacobi), same stopping rule; sums in another
+ * order, so equal to the serial result within rounding. */
+int32_t oracle_pagerank_f64_omp(int32_t n, const int32_t *ro, const int32_t *co,
+                                const int32_t *ri, const double *p_csc,
+                                const double *t, double d, double tol,
+                                int32_t max_iter, double *r, double *resid);
 
 #ifdef __cplusplus
 }

@@ -84,6 +84,47 @@ def test_reference_pagerank_program(gpu_available):
     assert st["iterations"] >= 2
 
 
+@pytest.mark.parametrize("all_edges", [False, True])
+def test_gas_gather_one_iteration(gpu_available, all_edges):
+    """The generic engine's load-balanced frontier gather, pinned by one
+    iteration of the reference PageRank program (iter_num = 1): every vertex
+    starts at 0.15 in the frontier, so after one gather + apply
+    r_v = 0.15 + 0.85 * sum_{u in N(v)} 0.15 / deg(u) exactly (N = in-edges,
+    deg = out-degree; with GATHER_ALL_EDGES N = in- and out-edges, deg = out +
+    in).  R-MAT scale 15 puts hub rows across many gather tiles; thresholds 0
+    and 2^30 run the LBS and tiered expansions after it."""
+    c = gg.gen_rmat(15, 16, undirected=False, seed=81)
+    ro, col = c.row_offsets, c.col_indices
+    n = c.n
+    rows = np.repeat(np.arange(n), c.degrees())
+    outdeg = c.degrees().astype(np.float64)
+    indeg = np.bincount(col, minlength=n).astype(np.float64)
+    deg = outdeg + indeg if all_edges else outdeg
+    share = np.where(deg > 0, 0.15 / np.where(deg > 0, deg, 1.0), 0.0)
+    acc = np.bincount(col, weights=share[rows], minlength=n)  # over in-edges
+    if all_edges:
+        acc += np.bincount(rows, weights=share[col], minlength=n)  # over out-edges
+    exp = 0.15 + 0.85 * acc
+    assert np.max(indeg) > 4096  # rows spanning several 1024-position tiles
+    flags = _abi.GG_PR_ALL_EDGES if all_edges else 0
+    with gg.Graph(c, directed=True) as g:
+        for th in (0, 1 << 30):
+            g.set_options(iter_num=1, threshold=th)
+            r, st = g.pagerank_reference(flags=flags)
+            assert st["iterations"] == 1
+            np.testing.assert_allclose(r, exp, rtol=2e-6, atol=0)
+        # Full runs through the device-driven and the host-driven (instrumented)
+        # loops, each checked as the reference's driver checks the program.
+        g.set_options(iter_num=(1 << 31) - 1, threshold=-1)
+        r_dev, st_dev = g.pagerank_reference(flags=flags)
+        r_host, st_host = g.pagerank_reference(flags=flags | _abi.GG_INSTRUMENT)
+    assert st_dev["iterations"] > 1 and st_host["iterations"] > 1
+    if not all_edges:
+        ref, _ = oracle.pr_reference(ro, col)
+        for r in (r_dev, r_host):
+            assert np.linalg.norm(r - ref) / np.linalg.norm(ref) < 0.02
+
+
 def test_adjoint_vs_fp64_and_finite_differences(gpu_available):
     c = gg.gen_rmat(12, 16, undirected=False, seed=35).with_weights("f32", symmetric=False, seed=36)
     n, m = c.n, c.m

@@ -463,3 +463,34 @@ def test_in_and_all_edge_programs(gpu_available):
             g.cc(flags=_abi.GG_CC_WEAK)
         with pytest.raises(gg.GGError):
             g.pagerank_reference(flags=_abi.GG_PR_ALL_EDGES)
+
+
+@pytest.mark.parametrize("threshold", [-1, 0, 1 << 30])
+def test_gas_path_device_and_host_loops(gpu_available, threshold):
+    """The generic engine's device-driven loop (captured iterations, the
+    LBS / tiered choice made on the device) and its host-driven loop
+    (GG_INSTRUMENT) give the oracle's results for the reference schedules
+    (push-only BFS, GAS SSSP, GAS CC), the same BFS statistics (fixed by the
+    graph), and stop together under an iteration cap."""
+    c = gg.gen_rmat(14, 16, undirected=True, seed=28).with_weights("i32", 1, 64, symmetric=True, seed=29)
+    src = workloads.max_degree_vertex(c)
+    exp_b = oracle.bfs(c.row_offsets, c.col_indices, src)
+    exp_s = oracle.sssp(c.row_offsets, c.col_indices, c.values, src)
+    exp_c = oracle.cc(c.row_offsets, c.col_indices)[0]
+    with gg.Graph(c) as g:
+        g.set_options(threshold=threshold)
+        stats = []
+        for fl in (0, _abi.GG_INSTRUMENT, 0):  # the second device run replays the cached capture
+            d, st = g.bfs(src, push_only=True, flags=fl)
+            np.testing.assert_array_equal(d, exp_b)
+            stats.append((st["iterations"], st["edges_inspected"], st["reached_edges"]))
+            dist, _ = g.sssp(src, flags=fl | _abi.GG_SSSP_GAS)
+            np.testing.assert_array_equal(dist, exp_s)
+            lab, _ = g.cc(gas=True, flags=fl)
+            np.testing.assert_array_equal(lab, exp_c)
+        assert stats[0] == stats[1] == stats[2]
+        g.set_options(iter_num=2)
+        res = [g.bfs(src, push_only=True, flags=fl) for fl in (0, _abi.GG_INSTRUMENT)]
+        np.testing.assert_array_equal(res[0][0], res[1][0])
+        assert res[0][1]["iterations"] == res[1][1]["iterations"] == 2
+        assert res[0][0].max() == 2

@@ -140,6 +140,52 @@ def test_pr_reference_is_near_its_fixed_point():
     assert np.max(np.abs(r - exact)) < 0.1
 
 
+def test_pr_reference_trivial_graph_hand_kat():
+    """CPUPR (PageRank.cu:108-161) on smallRegressionGraphs/trivialGraph.mtx
+    (0->1, 1->2, 2->3, 3->4, 4->3, all out-degree 1), worked by hand.
+    Gauss-Seidel in vertex order from 0.15 everywhere: r0 = 0.15,
+    r1 = 0.15 + 0.85 r0 = 0.2775, r2 = 0.385875 after the first sweep and
+    fixed from then on.  r3_k = 0.15 + 0.85 (r2 + r4_{k-1}) with
+    r4_{k-1} = 0.15 + 0.85 r3_{k-1} gives r3_k = a + q r3_{k-1}, a =
+    0.60549375, q = 0.7225, r3_0 := 0 (r4_0 = 0.15), so r3_k = r3* (1 - q^k),
+    r3* = a / (1 - q).  Sweep k >= 2 moves r3 by a q^(k-1) (r4 by 0.85 of
+    that): >= 0.01 up to k = 13, 0.00885 at k = 14, so CPUPR stops after 14
+    sweeps with r3 = r3* (1 - q^14) and r4 = 0.15 + 0.85 r3.  A Jacobi sweep,
+    another threshold or a float 0.01f that changes the stop would change the
+    count or the values."""
+    ro = np.array([0, 1, 2, 3, 4, 5], np.int32)
+    col = np.array([1, 2, 3, 4, 3], np.int32)
+    a, q = 0.60549375, 0.7225
+    r3 = a / (1 - q) * (1 - q ** 14)
+    exp = np.array([0.15, 0.2775, 0.385875, r3, 0.15 + 0.85 * r3])
+    r, sweeps = oracle.pr_reference(ro, col)
+    assert sweeps == 14
+    np.testing.assert_allclose(r, exp, rtol=0, atol=5e-6)
+
+
+def test_pagerank_fd_matches_brute_force():
+    """oracle.pagerank_fd_f64 (perturbed fixed points from the difference
+    equation, batched) equals plain central differences of two full FP64
+    solves per edge, on a graph small enough for the latter to carry no
+    cancellation error; more than 32 edges so two sweep batches run."""
+    ro, col, _ = random_csr(300, 1500, 13)
+    n, m = len(ro) - 1, len(col)
+    rng = np.random.default_rng(14)
+    w = rng.uniform(0.05, 1.0, m)
+    c = rng.standard_normal(n)
+    deg = np.diff(ro)
+    s = np.add.reduceat(w, ro[:-1])
+    p = w / np.repeat(np.where(deg > 0, s, 1.0), deg)
+    t = np.full(n, 1.0 / n)
+    r, _, _ = oracle.pagerank_f64(ro, col, p=p, tol=1e-16, max_iter=100000)
+    edges = rng.choice(m, 40, replace=False)
+    h = 1e-3 * np.maximum(np.abs(w[edges]), 1e-3)
+    fd, sweeps = oracle.pagerank_fd_f64(ro, col, w, r, c, edges, h, tol_rel=1e-13)
+    brute = fd_grad(ro, col, w, t, c, 0.85, edges)
+    assert np.linalg.norm(fd - brute) / np.linalg.norm(brute) < 1e-7
+    assert sweeps > 5
+
+
 def fd_grad(ro, col, w, t, c, d, edges, h_rel=1e-3):
     deg = np.diff(ro)
 
@@ -308,3 +354,28 @@ def gg_rmat_sym(seed):
 
     c = gg.gen_rmat(13, 16, undirected=True, seed=seed).with_weights("i32", 1, 64, symmetric=True, seed=seed)
     return c.row_offsets, c.col_indices, c.values
+
+
+@pytest.mark.parametrize("seed", [15, 16])
+def test_parallel_cc_and_pagerank_equal_serial(seed):
+    """The all-core CPU baselines compute the serial oracle's results: CC labels
+    equal, PageRank within rounding (other summation order)."""
+    ro, col, _ = random_csr(3000, 6000, seed, symmetric=True)
+    lab, _ = oracle.cc(ro, col)
+    lab2, _ = oracle.cc_omp(ro, col)
+    np.testing.assert_array_equal(lab, lab2)
+    ro, col, _ = random_csr(2000, 12000, seed)
+    n = len(ro) - 1
+    rng = np.random.default_rng(seed)
+    w = rng.uniform(0.05, 1.0, len(col))
+    deg = np.diff(ro)
+    s = np.add.reduceat(w, ro[:-1])
+    p = w / np.repeat(np.where(deg > 0, s, 1.0), deg)
+    t = rng.uniform(0, 1, n)
+    t /= t.sum()
+    co, ri, perm = oracle.csc(ro, col)
+    for pp, tt in ((None, None), (p, t)):
+        r, it, res = oracle.pagerank_f64(ro, col, p=pp, t=tt, tol=1e-13)
+        r2, it2, res2 = oracle.pagerank_f64_omp(ro, co, ri, p_csc=None if pp is None else pp[perm], t=tt, tol=1e-13)
+        assert abs(it - it2) <= 1
+        assert np.abs(r - r2).sum() < 1e-12
