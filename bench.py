@@ -390,6 +390,30 @@ def extra_batch(gg, workloads, torch, np, scale, reorder):
             torch.cuda.synchronize()  # torch's default stream -> the library's stream
             g.pagerank(teleport=tele, tol_l1=1e-6, flags=gg._abi.GG_OUT_DEVICE, out=dev_f)
         plain_ppr = (time.perf_counter() - t0) * 1e3
+        # Batched BFS from the same 128 sources (two passes of 64) against 128
+        # plain gg_bfs launches, results on the device in both.
+        depth_dev = torch.empty((len(src), c.n), dtype=torch.int32, device="cuda")
+        g.batch(bfs_sources=src[:64], flags=gg._abi.GG_OUT_DEVICE, depth_out=depth_dev)  # warm-up
+        bfs_ms = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, _, _, bst = g.batch(bfs_sources=src, flags=gg._abi.GG_OUT_DEVICE, depth_out=depth_dev)
+            bfs_ms.append((time.perf_counter() - t0) * 1e3)
+        depth_rows = depth_dev[sample_s].cpu().numpy()
+        del depth_dev
+        reached = 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s_ in src:
+            _, pst = g.bfs(int(s_), flags=gg._abi.GG_OUT_DEVICE, out=dev_i)
+            reached += pst["reached_edges"]
+        plain_bfs = (time.perf_counter() - t0) * 1e3
+        bb = float(np.median(bfs_ms))
+        out["bfs128"] = {"batched_ms": bb, "plain_ms": plain_bfs, "speedup": plain_bfs / bb,
+                         "batched_gteps": reached / (bb * 1e-3) / 1e9, "plain_gteps": reached / (plain_bfs * 1e-3) / 1e9,
+                         "levels": bst["iterations"], "bottom_up_levels": bst["pull_iterations"],
+                         "edges_inspected": bst["edges_inspected"]}
     out.update({"batched_ms": res, "plain_ms": {"sssp128": plain_sssp, "ppr64": plain_ppr,
                                                 "both": plain_sssp + plain_ppr},
                 "speedup": {"sssp128": plain_sssp / res["sssp128"], "ppr64": plain_ppr / res["ppr64"],
@@ -420,7 +444,9 @@ def extra_batch(gg, workloads, torch, np, scale, reorder):
         tpp += time.perf_counter() - t0
     cpu1 = (t1s * 128 / len(sample_s) + t1p * 64 / len(sample_p)) * 1e3
     cpup = (tps * 128 / len(sample_s) + tpp * 64 / len(sample_p)) * 1e3
-    out["parity"] = {"sssp_rows_bit_exact": ok, "sssp_rows_checked": len(sample_s),
+    bfs_ok = all(np.array_equal(depth_rows[k], oracle.bfs(c.row_offsets, c.col_indices, int(src[i])))
+                 for k, i in enumerate(sample_s))
+    out["parity"] = {"sssp_rows_bit_exact": ok, "sssp_rows_checked": len(sample_s), "bfs_rows_bit_exact": bfs_ok,
                      "ppr_l1_error_vs_fp64_max": max(errs), "ppr_rows_checked": len(sample_p),
                      "ppr_error_bound_at_tol": 0.85 / 0.15 * 1e-6}
     out["cpu"] = {"per_instance_loop_1core_ms": cpu1, "per_instance_loop_all_cores_ms": cpup,

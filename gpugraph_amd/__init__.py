@@ -329,26 +329,35 @@ class Graph:
         self.values_version += 1
 
     def batch(self, sssp_sources=(), ppr_seeds=(), damping: float = 0.85, tol_l1: float = 1e-6,
-              max_iter: int = 1000, flags: int = 0, dist_out=None, rank_out=None):
-        """Batched SSSP / personalised PageRank (gg_batch_run).  Results are
-        instance-major: dist[i][v], rank[j][v].  With GG_OUT_DEVICE in flags,
-        dist_out / rank_out are device buffers (contiguous tensors or raw
-        addresses) of len(sources) x n int32 / len(seeds) x n float32, and are
-        returned in place of host arrays."""
+              max_iter: int = 1000, flags: int = 0, dist_out=None, rank_out=None, bfs_sources=(), depth_out=None):
+        """Batched SSSP / personalised PageRank / BFS (gg_batch_run).  Results
+        are instance-major: dist[i][v], rank[j][v], depth[k][v].  With
+        GG_OUT_DEVICE in flags, dist_out / rank_out / depth_out are device
+        buffers (contiguous tensors or raw addresses) of len(sources) x n int32
+        / len(seeds) x n float32 / len(bfs_sources) x n int32, and are returned
+        in place of host arrays.  Returns (dist, rank, stats), or (dist, rank,
+        depth, stats) when bfs_sources are given."""
         src = np.ascontiguousarray(sssp_sources, dtype=np.int32)
         seeds = np.ascontiguousarray(ppr_seeds, dtype=np.int32)
+        bsrc = np.ascontiguousarray(bfs_sources, dtype=np.int32)
         spec = _abi.BatchSpec()
         if flags & _abi.GG_OUT_DEVICE:
-            if (len(src) and dist_out is None) or (len(seeds) and rank_out is None):
-                raise ValueError("GG_OUT_DEVICE needs dist_out / rank_out device buffers")
-            dist, rank = dist_out, rank_out
+            if (len(src) and dist_out is None) or (len(seeds) and rank_out is None) or (
+                    len(bsrc) and depth_out is None):
+                raise ValueError("GG_OUT_DEVICE needs dist_out / rank_out / depth_out device buffers")
+            dist, rank, depth = dist_out, rank_out, depth_out
             spec.sssp_dist = C.cast(C.c_void_p(_dev_ptr(dist_out) if len(src) else 0), _abi._I32P)
             spec.ppr_rank = C.cast(C.c_void_p(_dev_ptr(rank_out) if len(seeds) else 0), _abi._F32P)
+            spec.bfs_depth = C.cast(C.c_void_p(_dev_ptr(depth_out) if len(bsrc) else 0), _abi._I32P)
         else:
             dist = np.empty((len(src), self.n), np.int32)
             rank = np.empty((len(seeds), self.n), np.float32)
+            depth = np.empty((len(bsrc), self.n), np.int32)
             spec.sssp_dist = dist.ctypes.data_as(_abi._I32P)
             spec.ppr_rank = rank.ctypes.data_as(_abi._F32P)
+            spec.bfs_depth = depth.ctypes.data_as(_abi._I32P)
+        spec.num_bfs = len(bsrc)
+        spec.bfs_sources = bsrc.ctypes.data_as(_abi._I32P)
         spec.num_sssp = len(src)
         spec.sssp_sources = src.ctypes.data_as(_abi._I32P)
         spec.num_ppr = len(seeds)
@@ -358,6 +367,8 @@ class Graph:
         spec.max_iter = max_iter
         st = _abi.Stats()
         _abi.check(self.lib.gg_batch_run(self.h, C.byref(spec), flags, C.byref(st)), "gg_batch_run")
+        if len(bsrc):
+            return dist, rank, depth, st.as_dict()
         return dist, rank, st.as_dict()
 
     def _out(self, out, dtype, flags):

@@ -107,6 +107,9 @@ class BatchSpec(C.Structure):
         ("damping", C.c_float),
         ("tol_l1", C.c_double),
         ("max_iter", C.c_int32),
+        ("num_bfs", C.c_int32),
+        ("bfs_sources", C.POINTER(C.c_int32)),
+        ("bfs_depth", C.POINTER(C.c_int32)),
     ]
 
 

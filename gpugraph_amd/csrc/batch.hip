@@ -760,7 +760,8 @@ extern "C" int gg_batch_run(gg_graph *g, const gg_batch_spec *spec, uint32_t fla
   using namespace gg;
   gg::clear_error();
   GG_CHECK_ARG(g && spec, "graph/spec is NULL");
-  GG_CHECK_ARG(spec->num_sssp >= 0 && spec->num_ppr >= 0, "negative instance count");
+  GG_CHECK_ARG(spec->num_sssp >= 0 && spec->num_ppr >= 0 && spec->num_bfs >= 0, "negative instance count");
+  GG_CHECK_ARG(spec->num_bfs == 0 || (spec->bfs_sources && spec->bfs_depth), "BFS instances need sources and output");
   GG_CHECK_ARG(spec->num_sssp == 0 || (spec->sssp_sources && g->value_dtype == GG_VALUES_I32),
                "SSSP instances need sources and int32 weights");
   GG_CHECK_ARG(spec->num_ppr == 0 || spec->ppr_seeds, "PPR instances need seeds");
@@ -770,6 +771,8 @@ extern "C" int gg_batch_run(gg_graph *g, const gg_batch_spec *spec, uint32_t fla
     GG_CHECK_ARG(spec->sssp_sources[i] >= 0 && spec->sssp_sources[i] < g->n, "source %d out of range", i);
   for (int32_t i = 0; i < spec->num_ppr; i++)
     GG_CHECK_ARG(spec->ppr_seeds[i] >= 0 && spec->ppr_seeds[i] < g->n, "seed %d out of range", i);
+  for (int32_t i = 0; i < spec->num_bfs; i++)
+    GG_CHECK_ARG(spec->bfs_sources[i] >= 0 && spec->bfs_sources[i] < g->n, "BFS source %d out of range", i);
   GG_TRY(activate(g));
   if (spec->num_sssp) GG_TRY(require_nonnegative_weights(g, "batched SSSP"));
   g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
@@ -794,6 +797,7 @@ extern "C" int gg_batch_run(gg_graph *g, const gg_batch_spec *spec, uint32_t fla
     done_s += ns;
     done_p += np;
   }
+  if (spec->num_bfs) GG_TRY(batch_bfs(g, spec->bfs_sources, spec->num_bfs, spec->bfs_depth, flags, &acc));
   finish_profile(g);
   if (stats) *stats = acc;
   return GG_OK;

@@ -225,6 +225,7 @@ struct gg_graph {
   gg::DevBuf pr[8];    // PageRank vectors / CSC-ordered transition weights
   gg::DevBuf batch[8]; // batched-instance state
   gg::DevBuf tree[2];  // gg_bfs_tree: device depth and parent arrays
+  gg::DevBuf msbfs[4]; // batched BFS: seen, frontier words (2 n), next, sizes + sources
   // Degree reordering (GG_GRAPH_REORDER): order[new] = old, new_id[old] =
   // new, emap[new slot] = old slot; results in internal order before mapping;
   // staging for host outputs; host copy of new_id for source ids.
@@ -340,6 +341,10 @@ int weight_range(gg_graph *g, int32_t *min_w, int32_t *max_w);
 // then need not exist (a negative undirected edge is a negative cycle), and
 // runs that go to quiescence would not end.
 int require_nonnegative_weights(gg_graph *g, const char *what);
+// Batched multi-source BFS (batch_bfs.hip): num sources, 64 per pass; depths
+// instance-major (num x n) to host or device memory per flags; statistics
+// added to *acc.
+int batch_bfs(gg_graph *g, const int32_t *sources, int32_t num, int32_t *depth_out, uint32_t flags, gg_stats *acc);
 // Symmetry / value-symmetry check of an undirected graph (sets mirror_ok,
 // values_asymmetric); GG_ERR_INVALID when the CSR is provably not symmetric.
 int check_undirected(gg_graph *g);

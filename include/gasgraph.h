@@ -210,9 +210,12 @@ int gg_pagerank_adjoint(gg_graph *g, float damping, const float *rank,
 /* ---- batched instances -------------------------------------------------- */
 /* num_sssp SSSP sources and num_ppr personalized-PageRank seeds (one-hot
  * teleports) solved together: per-instance frontiers in shared kernels,
- * iteration loop captured as a HIP graph.  Outputs are instance-major:
- * sssp_dist[k*n + v], ppr_rank[k*n + v].  Replaces the drivers' sequential
- * per-source loop (sssp.cu:389-404, bfs.cu:865-881). */
+ * iteration loop captured as a HIP graph; num_bfs BFS sources, 64 per pass,
+ * one bit per instance in a 64-bit frontier word per vertex.  Outputs are
+ * instance-major: sssp_dist[k*n + v], ppr_rank[k*n + v], bfs_depth[k*n + v]
+ * (the BFS program's depths, -1 unreached).  Replaces the drivers'
+ * sequential per-source loops (sssp.cu:389-404, bfs.cu:865-881).  Fields
+ * after max_iter were added later; zero-initialise the struct. */
 typedef struct gg_batch_spec {
   int32_t num_sssp;
   const int32_t *sssp_sources;
@@ -223,6 +226,9 @@ typedef struct gg_batch_spec {
   float damping;
   double tol_l1;
   int32_t max_iter;
+  int32_t num_bfs;
+  const int32_t *bfs_sources;
+  int32_t *bfs_depth;
 } gg_batch_spec;
 
 int gg_batch_run(gg_graph *g, const gg_batch_spec *spec, uint32_t flags,

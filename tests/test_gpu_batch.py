@@ -142,3 +142,45 @@ def test_negative_weights_rejected(gpu_available):
         g.update_values(c.values)
         dist2, _ = g.sssp(0)
         np.testing.assert_array_equal(dist2, dist)
+
+
+@pytest.mark.parametrize("undirected,reorder", [(False, False), (True, True), (False, True)])
+def test_bfs_batch_matches_oracle(gpu_available, undirected, reorder):
+    """Batched multi-source BFS: 70 sources (two passes, 64 + 6, one source
+    repeated, one isolated vertex) bit-exact per instance against the oracle
+    and against plain gg_bfs, host and device outputs, alone and beside a
+    batched SSSP."""
+    import torch
+
+    c = gg.gen_rmat(13, 16, undirected=undirected, seed=55).with_weights("i32", 1, 64, symmetric=undirected,
+                                                                         seed=56)
+    rng = np.random.default_rng(57)
+    src = rng.choice(c.n, 70, replace=False).astype(np.int32)
+    src[3] = src[2]
+    iso = np.flatnonzero(c.degrees() == 0)
+    if len(iso):
+        src[10] = iso[0]
+    with gg.Graph(c, directed=not undirected, reorder=reorder) as g:
+        _, _, depth, st = g.batch(bfs_sources=src)
+        assert depth.shape == (70, c.n)
+        for i in range(70):
+            np.testing.assert_array_equal(depth[i], oracle.bfs(c.row_offsets, c.col_indices, int(src[i])),
+                                          err_msg=f"instance {i}")
+        for i in (0, 63, 64, 69):
+            plain, _ = g.bfs(int(src[i]))
+            np.testing.assert_array_equal(depth[i], plain)
+        dev = torch.empty((70, c.n), dtype=torch.int32, device="cuda")
+        g.batch(bfs_sources=src, flags=_abi_flags().GG_OUT_DEVICE, depth_out=dev)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(dev.cpu().numpy(), depth)
+        dist, _, depth2, _ = g.batch(sssp_sources=src[:5], bfs_sources=src[:5])
+        np.testing.assert_array_equal(depth2, depth[:5])
+        for i in range(5):
+            np.testing.assert_array_equal(dist[i], oracle.sssp(c.row_offsets, c.col_indices, c.values, int(src[i])))
+    assert st["iterations"] >= 2
+
+
+def _abi_flags():
+    from gpugraph_amd import _abi
+
+    return _abi

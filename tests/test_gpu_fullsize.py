@@ -7,7 +7,8 @@ benchmark runs them with (graphs stored renumbered by degree):
   2048 x 2048 percolated lattice: the three CC engines bit-exact.
   R-MAT scale 20 directed, 128 SSSP + 64 personalised PageRank instances in
     one batched run: 8 SSSP rows bit-exact, 4 PPR rows against the FP64
-    oracle (L1 tolerances stated below).
+    oracle (L1 tolerances stated below); 128 batched BFS instances, 8 rows
+    bit-exact.
   R-MAT scale 22 directed, FP32 weights: PageRank L1 <= 1e-6 against the FP64
     oracle; the adjoint's edge-weight gradient within 1e-4 (relative L2) of
     the FP64 CPU adjoint on every edge and of central finite differences on
@@ -106,6 +107,17 @@ def test_config4_batch(gpu_available, rmat20_batch, tol):
         err = l1(rank[i], exp)
         assert err <= bound, (i, err, bound)
     assert st["residual"] <= tol
+
+
+def test_config4_batched_bfs(gpu_available, rmat20_batch):
+    """128 BFS instances (two passes of 64) from the batched workload's SSSP
+    sources on the same renumbered graph: 8 rows bit-exact."""
+    c, src, _ = rmat20_batch
+    with gg.Graph(c, directed=True, reorder=True) as g:
+        _, _, depth, st = g.batch(bfs_sources=src)
+    for i in (0, 1, 31, 63, 64, 65, 100, 127):
+        np.testing.assert_array_equal(depth[i], oracle.bfs(c.row_offsets, c.col_indices, int(src[i])),
+                                      err_msg=f"BFS instance {i}")
 
 
 @pytest.fixture(scope="module")
