@@ -168,6 +168,7 @@ PROTOTYPES = {
     "gg_profile_reset": (C.c_int, [_P]),
     "gg_profile_get": (C.c_int, [_P, C.POINTER(KernelStat), C.c_int32, _I32P]),
     "gg_profile_launches": (C.c_int, [_P, C.POINTER(LaunchStat), C.c_int32, _I32P]),
+    "gg_bfs_noalloc": (C.c_int, [C.c_int32, C.c_int64, _P, _P, C.c_int32, _P, _P, C.POINTER(Stats)]),
     "gg_csr_free": (None, [C.POINTER(Csr)]),
     "gg_read_mtx": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(Csr)]),
     "gg_gen_erdos_renyi": (C.c_int, [C.c_int32, C.c_int64, C.c_uint64, C.POINTER(Csr)]),

@@ -294,6 +294,21 @@ typedef struct gg_launch_stat {
 int gg_profile_launches(gg_graph *g, gg_launch_stat *out, int32_t max_entries,
                         int32_t *count);
 
+/* ---- allocation-free entry point ---------------------------------------- */
+/* BFS over a CSR already on the device, allocating nothing: d_row_offsets
+ * (n+1) and d_col_indices (m) in device memory, depths (the BFS program's:
+ * 0 at src, -1 unreached, same as gg_bfs) written to d_depth (n, device).
+ * Scratch (queues, visited bitmap, scan and partition arrays, counters) is a
+ * fixed device array of the library, about 25 MB per GPU, held until the
+ * process ends; n <= 2^21 and m <= 2^26, larger graphs get GG_ERR_INVALID.
+ * Work runs on `stream` (a hipStream_t; NULL = the null stream) with one
+ * synchronisation per level.  Calls that overlap on one GPU share that
+ * scratch and race.  No counterpart in the reference: for callers that
+ * cannot allocate (serving loops, allocation-free harnesses). */
+int gg_bfs_noalloc(int32_t n, int64_t m, const int32_t *d_row_offsets,
+                   const int32_t *d_col_indices, int32_t src, int32_t *d_depth,
+                   void *stream, gg_stats *stats);
+
 /* ---- graph construction (host) ----------------------------------------- */
 /* Host CSR produced by the builders below; free with gg_csr_free. */
 typedef struct gg_csr {
