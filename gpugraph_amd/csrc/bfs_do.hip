@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -32,6 +33,10 @@ namespace {
 
 using namespace GASengine;
 constexpr int kLoopLevels = 6;  // levels per replay of the captured loop (even)
+// A top-down level over a small frontier (at most kTierItems vertices and
+// kTierEdges out-edges) is one tiered launch instead of the four LBS launches.
+constexpr int32_t kTierItems = 2048;
+constexpr int64_t kTierEdges = 65536;
 
 // counters: [0] next frontier size, [1] out-degree sum of the new frontier
 // (m_f), [2] edges examined, [3] unvisited vertices scanned.
@@ -230,6 +235,97 @@ k_bfs_convert(const int32_t *__restrict__ depth, const int32_t *__restrict__ lev
   }
 }
 
+// Graphs of at most kSmallN vertices: the whole BFS in one workgroup and one
+// launch (level loop inside the kernel, frontiers, their degree prefix and
+// the visited bitmap in LDS), so a small graph costs one launch and one
+// synchronisation instead of several launches per level.  Each level's
+// frontier edges are spread over the 1024 threads by a binary search in the
+// LDS degree prefix (balanced whatever the degrees).  Results: depth (global),
+// counters slot 0 = levels, 1 = reached vertices, 2 = out-degree sum of the
+// reached vertices (= edges examined), every other stripe of those slots 0.
+constexpr int kSmallN = 8192;
+constexpr int kSmallBlock = 1024;
+constexpr int kSmallPer = kSmallN / kSmallBlock;
+
+__global__ void __launch_bounds__(kSmallBlock)
+k_bfs_small(const int32_t *__restrict__ ro, const int32_t *__restrict__ col, int32_t n, int32_t src,
+            int32_t *__restrict__ depth, unsigned long long *__restrict__ ctr) {
+  __shared__ int32_t q[2][kSmallN];
+  __shared__ int32_t off[kSmallN];
+  __shared__ uint32_t vis[kSmallN / 32];
+  __shared__ int32_t qn[2];
+  __shared__ int32_t scan[kSmallBlock / kWave + 1];
+  const int tid = threadIdx.x;
+  for (int v = tid; v < n; v += kSmallBlock) depth[v] = v == src ? 0 : -1;
+  for (int w = tid; w < kSmallN / 32; w += kSmallBlock) vis[w] = w == (src >> 5) ? 1u << (src & 31) : 0u;
+  if (tid == 0) {
+    q[0][0] = src;
+    qn[0] = 1;
+    qn[1] = 0;
+  }
+  __syncthreads();
+  long long reached = 1, edges = 0;
+  int32_t level = 0;
+  for (;;) {
+    const int cur = level & 1;
+    const int32_t nq = qn[cur];
+    if (nq == 0) break;
+    // Degree prefix of the frontier: kSmallPer consecutive items per thread.
+    const int32_t i0 = tid * kSmallPer;
+    int32_t deg[kSmallPer];
+    int32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kSmallPer; j++) {
+      deg[j] = 0;
+      if (i0 + j < nq) {
+        const int32_t u = q[cur][i0 + j];
+        deg[j] = ro[u + 1] - ro[u];
+      }
+      sum += deg[j];
+    }
+    int32_t total;
+    int32_t ex = block_exclusive_scan<kSmallBlock>(sum, scan, &total);
+#pragma unroll
+    for (int j = 0; j < kSmallPer; j++) {
+      if (i0 + j < nq) off[i0 + j] = ex;
+      ex += deg[j];
+    }
+    if (tid == 0) qn[cur ^ 1] = 0;
+    __syncthreads();
+    edges += total;
+    for (int32_t k = tid; k < total; k += kSmallBlock) {
+      int32_t lo = 0, hi = nq - 1;  // last item with off <= k
+      while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= k)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const int32_t u = q[cur][lo];
+      const int32_t v = col[ro[u] + (k - off[lo])];
+      const uint32_t bit = 1u << (v & 31);
+      if (!(vis[v >> 5] & bit)) {
+        const uint32_t old = atomicOr(&vis[v >> 5], bit);
+        if (!(old & bit)) {
+          depth[v] = level + 1;
+          q[cur ^ 1][atomicAdd(&qn[cur ^ 1], 1)] = v;
+        }
+      }
+    }
+    __syncthreads();
+    reached += qn[cur ^ 1];
+    level++;
+  }
+  if (tid < kStripes * 3) {
+    const int q0 = tid / 3, slot = tid % 3;
+    unsigned long long v = 0;
+    if (q0 == 0) v = slot == 0 ? (unsigned long long)level : slot == 1 ? (unsigned long long)reached
+                                                                      : (unsigned long long)edges;
+    ctr[q0 * kCounterSlots + slot] = v;
+  }
+}
+
 // State of the device-driven loop.  One controller launch per level closes
 // the previous level (folds its counters, the host loop's bookkeeping),
 // clears the counters for the new one and takes its direction decision; a
@@ -239,7 +335,8 @@ struct BfsCtl {
   int32_t bottom_up;  // direction of the level in flight
   int32_t q2b, b2q;   // conversions before this level
   int32_t bu_skip;    // 1 unless this level runs bottom-up
-  int32_t push_size;  // frontier items of a top-down level, else 0
+  int32_t push_size;  // frontier items of a top-down level by LBS, else 0
+  int32_t tier_size;  // frontier items of a small top-down level (tiered launch), else 0
   int32_t done, pull_levels, pending;
   long long nf, prev_nf, mf, mu, reached, reached_edges, examined;
   double bytes;
@@ -269,6 +366,7 @@ __global__ void k_bfs_start(int32_t *depth, int32_t n, uint32_t *visited, int32_
       c->q2b = c->b2q = 0;
       c->bu_skip = 1;
       c->push_size = 0;
+      c->tier_size = 0;
       c->done = 0;
       c->pull_levels = 0;
       c->pending = 0;
@@ -317,6 +415,7 @@ k_bfs_ctrl(BfsCtl *c, unsigned long long *__restrict__ ctr, int32_t *__restrict_
     c->q2b = c->b2q = 0;
     c->bu_skip = 1;
     c->push_size = 0;
+    c->tier_size = 0;
     s_go = !close_only && !c->done;
   }
   __syncthreads();
@@ -338,6 +437,8 @@ k_bfs_ctrl(BfsCtl *c, unsigned long long *__restrict__ ctr, int32_t *__restrict_
     }
     if (c->bottom_up)
       c->bu_skip = 0;
+    else if (c->nf <= kTierItems && c->mf <= kTierEdges)
+      c->tier_size = (int32_t)c->nf;
     else
       c->push_size = (int32_t)c->nf;
     c->pending = 1;
@@ -353,6 +454,44 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
   hipStream_t s = g->stream;
   LbsBuffers lb;
   GG_TRY(lbs_buffers(g, &lb));
+  if (n <= kSmallN && g->opt.iter_num == INT_MAX) {
+    int32_t *depth = depth_out;
+    if (!(flags & GG_OUT_DEVICE)) {
+      GG_TRY(g->vtx[0].ensure(4 * (size_t)n));
+      depth = g->vtx[0].as<int32_t>();
+    } else {
+      GG_CHECK_ARG(depth_out, "depth is NULL");
+    }
+    g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
+    unsigned long long *ctr = g->counters.as<unsigned long long>();
+    hipEvent_t e0, e1;
+    GG_HIP(g->bfs_ev.get(&e0, &e1));
+    GG_HIP(hipEventRecord(e0, s));
+    {
+      TimedRegion t(g->prof, "bfs.small");
+      hipLaunchKernelGGL(k_bfs_small, dim3(1), dim3(kSmallBlock), 0, s, g->dg.row_offsets, g->dg.column_indices, n,
+                         src, depth, ctr);
+      GG_HIP(hipGetLastError());
+    }
+    GG_HIP(hipEventRecord(e1, s));
+    GG_TRY(read_counters(g, 3));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const int64_t E = g->h64()[2], N = g->h64()[1];
+    g->prof.add_bytes("bfs.small", 12.0 * N + 8.0 * E + 8.0 * N + 4.0 * n);
+    finish_profile(g);
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      stats->iterations = (int32_t)g->h64()[0];
+      stats->reached_vertices = N;
+      stats->reached_edges = E;
+      stats->edges_inspected = E;
+      stats->alg_bytes = 12.0 * N + 8.0 * E + 8.0 * N + 4.0 * n;
+      stats->gpu_ms = ms;
+    }
+    if (flags & GG_OUT_DEVICE) return GG_OK;
+    return copy_out(g, depth_out, depth, 4 * (size_t)n, flags);
+  }
   GG_TRY(g->bits[0].ensure(4 * (size_t)words));  // visited
   GG_TRY(g->bits[1].ensure(4 * (size_t)words));  // frontier bitmaps (ping-pong)
   GG_TRY(g->bits[2].ensure(4 * (size_t)words));
@@ -410,7 +549,8 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
         (uint64_t)depth, (uint64_t)visited, (uint64_t)fbits[0], (uint64_t)fbits[1], (uint64_t)queue[0],
         (uint64_t)queue[1], (uint64_t)ctr, (uint64_t)ctl, (uint64_t)lb.scan_local, (uint64_t)lb.partials,
         (uint64_t)lb.info, (uint64_t)lb.parts, (uint64_t)lb.edge_counter, (uint64_t)lb.expand_grid, (uint64_t)lb.vt,
-        (uint64_t)alpha, (uint64_t)beta, (uint64_t)g->opt.wave_degree, (uint64_t)g->opt.iter_num};
+        (uint64_t)alpha, (uint64_t)beta, (uint64_t)g->opt.wave_degree, (uint64_t)g->opt.iter_num,
+        (uint64_t)g->opt.block_degree};
     if (!g->bfs_loop.exec || g->bfs_loop.key != key) {
       g->bfs_loop.reset();
       hipGraph_t graph = nullptr;
@@ -428,6 +568,9 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
                            (const int32_t *)&ctl->level);
         BfsPushOp op{g->dg.row_offsets, g->dg.column_indices, depth, visited, ctr, 0, &ctl->level, 0ull};
         lbs_launch(op, false, queue[c], &ctl->push_size, n, (int64_t)n + g->m, lb, queue[x], qsz[x], s, [](int) {});
+        hipLaunchKernelGGL((k_tiered_expand<BfsPushOp, false>), dim3(grid_for(kTierItems, kExpandBlock)),
+                           dim3(kExpandBlock), 0, s, op, queue[c], (const int32_t *)&ctl->tier_size, queue[x], qsz[x],
+                           g->opt.wave_degree, g->opt.block_degree, lb.edge_counter);
       }
       // close the last level of the replay (kLoopLevels is even: its output queue is qsz[0])
       hipLaunchKernelGGL(k_bfs_ctrl, dim3(1), dim3(64), 0, s, ctl, ctr, qsz[0], qsz[1], n, alpha, beta,
@@ -510,15 +653,27 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
       pull_levels++;
     } else {
       BfsPushOp op{g->dg.row_offsets, g->dg.column_indices, depth, visited, ctr, levels + 1, nullptr, 0ull};
-      lbs_launch(op, false, queue[cur], qsz[cur], nf, (int64_t)n + g->m, lb, queue[nxt], qsz[nxt], s,
-                 lbs_timer);
+      const bool small = nf <= kTierItems && mf <= kTierEdges;
+      if (small) {  // as k_bfs_ctrl: a small level is one tiered launch
+        TimedRegion t(g->prof, "bfs.top_down_small");
+        hipLaunchKernelGGL((k_tiered_expand<BfsPushOp, false>), dim3(grid_for(kTierItems, kExpandBlock)),
+                           dim3(kExpandBlock), 0, s, op, queue[cur], (const int32_t *)qsz[cur], queue[nxt], qsz[nxt],
+                           g->opt.wave_degree, g->opt.block_degree, lb.edge_counter);
+      } else {
+        lbs_launch(op, false, queue[cur], qsz[cur], nf, (int64_t)n + g->m, lb, queue[nxt], qsz[nxt], s,
+                   lbs_timer);
+      }
       GG_HIP(hipGetLastError());
       GG_TRY(read_counters(g, 10));
       const int64_t E = g->h64()[2];
       const int64_t N = g->h_pinned[(8 + nxt) * 2];
       lvl_bytes = 12.0 * nf + 8.0 * E + 8.0 * N;
-      g->prof.add_launch_bytes("bfs.lbs_count", 12.0 * nf);
-      g->prof.add_launch_bytes("bfs.top_down", 8.0 * E + 8.0 * N);
+      if (small) {
+        g->prof.add_launch_bytes("bfs.top_down_small", lvl_bytes);
+      } else {
+        g->prof.add_launch_bytes("bfs.lbs_count", 12.0 * nf);
+        g->prof.add_launch_bytes("bfs.top_down", 8.0 * E + 8.0 * N);
+      }
       examined_total += E;
       prev_nf = nf;
       nf = N;

@@ -12,6 +12,8 @@
 //                   (expand_edge + contract + dedup + enqueue per edge)
 //   expand, tiered  k_frontier_prologue (apply + post_apply) when the program
 //                   has them, k_tiered_expand
+//   k_gas_order     a next frontier of at least n/64 vertices rewritten in
+//                   vertex order from its dedup bitmap
 // Every kernel reads its frontier size from GasCtl, which holds 0 for the
 // path not taken and for all kernels once the run is over.
 //
@@ -99,6 +101,7 @@ __global__ void k_gas_ctl(GasCtl *c, int32_t *sizes, int sel) {
     return;
   }
   sizes[sel ^ 1] = 0;
+  c->order_count = 0;
   c->iter = c->iterations;
   c->iterations++;
   c->sum_frontier += nf;
@@ -107,6 +110,34 @@ __global__ void k_gas_ctl(GasCtl *c, int32_t *sizes, int sel) {
   c->lbs_size = lbs ? nf : 0;
   c->tier_size = lbs ? 0 : nf;
   c->gather_size = nf;
+}
+
+// A large next frontier is rewritten in vertex order from the dedup bitmap
+// that built it (the expand appends it tile by tile, in whatever order the
+// tiles finish): the next iteration then reads its vertex state and rows in
+// runs of consecutive vertices.  Same set, same size; one wave per 64 bitmap
+// words, one atomic per wave.
+template <int DUMMY = 0>
+__global__ void __launch_bounds__(256)
+k_gas_order(const GasCtl *__restrict__ c, const uint32_t *__restrict__ bits, int32_t words,
+            const int32_t *__restrict__ next_size, int32_t min_size, int32_t *__restrict__ queue,
+            int32_t *__restrict__ counter) {
+  if (c->done || *next_size < min_size) return;  // order_count is reset only for a running iteration
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < words; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t w = base + threadIdx.x;
+    uint32_t word = w < words ? bits[w] : 0u;
+    const int32_t cnt = __popc(word);
+    const int32_t incl = wave_inclusive_scan(cnt);
+    const int32_t total = __shfl(incl, kWave - 1);
+    int32_t start = 0;
+    if (lane_id() == 0 && total) start = atomicAdd(counter, total);
+    start = __shfl(start, 0) + incl - cnt;
+    while (word) {
+      const int b = __builtin_ctz(word);
+      word &= word - 1;
+      queue[start++] = (int32_t)(w * 32 + b);
+    }
+  }
 }
 
 // MARK_PREDECESSORS: -1 everywhere, then the initial frontier (the sources)
@@ -303,6 +334,14 @@ class EnactorVertexCentric {
           te("tiered_expand");
         }
       }
+    }
+    if (np > 0) {
+      const int32_t words = (n + 31) / 32;
+      tb("order");
+      hipLaunchKernelGGL(k_gas_order<>, dim3(grid_for(words, 256, 1024)), dim3(256), 0, s, (const GasCtl *)c,
+                         st.dedup_next, words, (const int32_t *)sz_out, std::max<int32_t>(n / 64, 4096), f_out,
+                         &c->order_count);
+      te("order");
     }
   }
 

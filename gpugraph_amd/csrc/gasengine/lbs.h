@@ -515,9 +515,10 @@ k_pull_reduce(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
 // does not depend on a gather is issued at once (stage 1; the tile's edges
 // are the contiguous slots [j0, j1), known from the partition alone): the
 // row offsets, each thread's VT edges (Edge loads) and the current state of
-// the tile's vertices.  The merge-path slot map is then built in LDS, the VT
-// gathers go out together (stage 2), and the results are combined and
-// handed to item_done with the prefetched state.
+// the tile's vertices.  When a tile starts, its VT gathers go out together
+// (stage 2: they need only the edges, already in registers), the merge-path
+// slot map is built in LDS while they are in flight, and the results are
+// combined and handed to item_done with the prefetched state.
 //
 // The loop is software-pipelined: a workgroup issues the stage-1 loads of
 // its next tile (and the partition bounds of the one after) before the
@@ -592,6 +593,13 @@ k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
     const int32_t i0 = a0, i1 = a1;
     const int32_t j0 = (int32_t)(d0 - i0), j1 = (int32_t)(d1 - i1);
     const int32_t ni = i1 - i0, ne = j1 - j0;
+    // Stage 2 first: the gathers depend only on the edges already in
+    // registers, so they are in flight while the slot map is built.
+    T val[VT];
+    if (ne > 0) {
+#pragma unroll
+      for (int r = 0; r < VT; r++) val[r] = op.gather(ed[r]);
+    }
 #pragma unroll
     for (int q = 0; q < Q; q++) {
       const int32_t k = tid + q * kExpandBlock;
@@ -627,11 +635,8 @@ k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
       }
     }
     __syncthreads();
-    // Stage 2: the gathers, VT in flight per thread.
+    // Combine the gathered values (issued at the top of the tile).
     if (ne > 0) {
-      T val[VT];
-#pragma unroll
-      for (int r = 0; r < VT; r++) val[r] = op.gather(ed[r]);
 #pragma unroll
       for (int r = 0; r < VT; r++) {
         const int32_t k = tid + r * kExpandBlock;

@@ -102,7 +102,7 @@ struct GasCtl {
   int32_t threshold, iter_num;
   int32_t iterations;  // iterations run so far
   int32_t lbs_iters;   // of which expanded by LBS
-  int32_t pad;
+  int32_t order_count; // slots written by k_gas_order in this iteration
   long long sum_frontier, sum_next;
 };
 

@@ -10,7 +10,8 @@
 //      merge, the row of each of its VT path positions (s_slot, as in
 //      k_pull_reduce);
 //   2. the edge contributions are gathered with consecutive lanes on
-//      consecutive edges (VT loads in flight per thread) into LDS;
+//      consecutive edges (VT loads in flight per thread, issued when the tile
+//      starts, before step 1's LDS work) into LDS;
 //   3. thread t sums edges [VT t, VT t + VT) of the tile in order, one
 //      Neumaier-compensated FP32 partial per run of equal rows;
 //   4. a segmented inclusive scan of the threads' last-run partials (wave
@@ -129,6 +130,13 @@ k_seg_rows(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const 
     const int32_t i0 = a0, i1 = a1;
     const int32_t j0 = (int32_t)(d0 - i0), j1 = (int32_t)(d1 - i1);
     const int32_t ni = i1 - i0, ne = j1 - j0;
+    // Second stage first: the gathers need only the edges, already in
+    // registers, so they are in flight while the slot map is built.
+    float val[VT];
+    if (ne > 0) {
+#pragma unroll
+      for (int r = 0; r < VT; r++) val[r] = op.gather(ed[r]);
+    }
 #pragma unroll
     for (int q = 0; q < Q; q++) {
       const int32_t k = tid + q * kRowBlock;
@@ -166,10 +174,7 @@ k_seg_rows(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const 
         }
       }
     }
-    if (ne > 0) {  // second stage: the gathers, VT in flight per thread (ne <= TILE)
-      float val[VT];
-#pragma unroll
-      for (int r = 0; r < VT; r++) val[r] = op.gather(ed[r]);
+    if (ne > 0) {  // the gathered values (ne <= TILE)
 #pragma unroll
       for (int r = 0; r < VT; r++)
         if (tid + r * kRowBlock < ne) s_val[tid + r * kRowBlock] = val[r];

@@ -14,7 +14,9 @@
 //            dirty cleared); counts the frontier's out-edges and
 //            the smallest distance still dirty.  Empty frontier with dirty
 //            vertices left: bound jumps to the bucket of that distance.
-//   push     frontier edges <= m / pull_alpha: LBS expand over the queue,
+//   push     frontier edges <= m / pull_alpha: LBS expand over the queue
+//            (one tiered launch instead when the frontier is small: at most
+//            kTierItems vertices and kTierEdges edges),
 //            nd = dist[v] + w, plain pre-check then atomicMin(dist[nb]); an
 //            improvement stores dirty[nb] = 1 (no atomic, no enqueue).  On an
 //            undirected graph whose in-edges are packed (below) the push
@@ -54,6 +56,9 @@ static_assert(kLoopIters <= kSsspReplayMax, "replay lengths are cached up to kSs
 // queue sizes (stripe 0 only).  Reached vertices
 // and their degree sum (the run's statistics) come from one pass over the
 // final distances (reach_stats), not from the relaxations.
+// Small pushes run as one tiered launch instead of the four LBS launches.
+constexpr int32_t kTierItems = 2048;
+constexpr int64_t kTierEdges = 65536;
 struct Tally {
   unsigned long long improved = 0;
   __device__ __forceinline__ void add() { improved++; }
@@ -268,9 +273,11 @@ __device__ __forceinline__ void sssp_reset_collect(unsigned long long *ctr) {
 // after copying what it needs out of the counters and clearing them.
 struct SsspCtl {
   int32_t bound;      // frontier bound of the current bucket
-  int32_t push_size;  // frontier items the LBS push sees: the queue size when pushing, else 0
+  int32_t push_size;  // frontier items the LBS push sees: the queue size when pushing by LBS, else 0
+  int32_t tier_size;  // frontier items the tiered push sees: the queue size when the push is small, else 0
   int32_t pull_skip;  // 0 when this iteration pulls
   int32_t done;
+  int32_t pad;
   int32_t iters, pulls;
   int32_t steps;      // controller calls that found the run unfinished (the last one ends it)
   long long edges;    // edges inspected
@@ -281,6 +288,7 @@ static_assert(sizeof(SsspCtl) % 8 == 0, "read back in 8-byte words");
 __global__ void k_sssp_ctrl(SsspCtl *c, unsigned long long *__restrict__ ctr, int32_t n, int64_t m, int64_t delta,
                             int64_t pull_alpha, int32_t max_iter, double pull_bytes) {
   c->push_size = 0;
+  c->tier_size = 0;
   c->pull_skip = 1;
   if (c->done) return;
   c->steps++;
@@ -308,7 +316,10 @@ __global__ void k_sssp_ctrl(SsspCtl *c, unsigned long long *__restrict__ ctr, in
     c->edges += m;
     c->pulls++;
   } else {
-    c->push_size = (int32_t)nf;
+    if (nf <= kTierItems && (int64_t)degsum <= kTierEdges)
+      c->tier_size = (int32_t)nf;
+    else
+      c->push_size = (int32_t)nf;
     c->bytes += 16.0 * nf + 12.0 * (double)degsum;
     c->edges += (long long)degsum;
   }
@@ -467,6 +478,7 @@ __global__ void k_sssp_start(int32_t *dist, int32_t n, uint4 *dirty16, int64_t n
   if (ctl && t0 == 0) {
     ctl->bound = bound;
     ctl->push_size = 0;
+    ctl->tier_size = 0;
     ctl->pull_skip = 1;
     ctl->done = 0;
     ctl->iters = 0;
@@ -568,6 +580,18 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       lbs_launch(push_op, false, queue, fsize, nf_host, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(), qsize + 1,
                  s, timer);
   };
+  // The small-frontier push: one tiered launch (at most kTierItems items).
+  const int tier_grid = grid_for(kTierItems, kExpandBlock);
+  auto launch_tier = [&](const int32_t *fsize) {
+    if (push_packed)
+      hipLaunchKernelGGL((k_tiered_expand<PackedPushOp, false>), dim3(tier_grid), dim3(kExpandBlock), 0, s, push_pk,
+                         queue, fsize, g->frontier[1].as<int32_t>(), qsize + 1, g->opt.wave_degree,
+                         g->opt.block_degree, lb.edge_counter);
+    else
+      hipLaunchKernelGGL((k_tiered_expand<PushOp, false>), dim3(tier_grid), dim3(kExpandBlock), 0, s, push_op, queue,
+                         fsize, g->frontier[1].as<int32_t>(), qsize + 1, g->opt.wave_degree, g->opt.block_degree,
+                         lb.edge_counter);
+  };
   bool device_tail = false;  // the device-driven loop read the counters and reach statistics itself
   if (!g->prof.enabled) {
     // Device-driven: a replay of a captured launch
@@ -580,7 +604,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
         (uint64_t)pparts, (uint64_t)lb.scan_local, (uint64_t)lb.partials, (uint64_t)lb.info, (uint64_t)lb.parts,
         (uint64_t)lb.edge_counter, (uint64_t)lb.expand_grid, (uint64_t)lb.vt, (uint64_t)delta,
         (uint64_t)pull_alpha, (uint64_t)g->opt.iter_num, (uint64_t)g->frontier[1].p, (uint64_t)pk,
-        (uint64_t)pk_bits};
+        (uint64_t)pk_bits, (uint64_t)g->opt.wave_degree, (uint64_t)g->opt.block_degree};
     // The replay of `iters` iterations, captured on first use for this graph
     // and these buffers / options.
     auto loop_exec = [&](int iters, hipGraphExec_t *out) -> int {
@@ -595,6 +619,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
           hipLaunchKernelGGL(k_sssp_ctrl, dim3(1), dim3(1), 0, s, ctl, ctr, n, m, delta, pull_alpha, g->opt.iter_num,
                              pull_bytes);
           launch_push(&ctl->push_size, n, [](int) {});
+          launch_tier(&ctl->tier_size);
           if (pull_alpha > 0) launch_pull((const int32_t *)&ctl->pull_skip);
         }
         const hipError_t le = hipGetLastError();
@@ -676,6 +701,17 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       bytes += pb;
       edges += m;
       pulls++;
+    } else if (nf <= kTierItems && degsum <= kTierEdges) {
+      // As k_sssp_ctrl: a small frontier is one tiered launch.
+      {
+        TimedRegion t(g->prof, "sssp.push_small");
+        launch_tier(qsize);
+        GG_HIP(hipGetLastError());
+      }
+      const double pb = 16.0 * nf + 12.0 * degsum;
+      g->prof.add_launch_bytes("sssp.push_small", pb);
+      bytes += pb;
+      edges += degsum;
     } else {
       // PushOp never enqueues: the output queue stays empty.
       launch_push(qsize, nf, lbs_timer);
