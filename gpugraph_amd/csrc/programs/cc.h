@@ -105,10 +105,13 @@ struct cc {
     }
   };
 
+  // atomicMin as the reference's contract (cc.h:339), after a plain read:
+  // gather_tmp only decreases during an iteration, so a read that is not
+  // larger than misc_value means the atomic could not change it.
   struct contract {
     __device__ void operator()(const int, int &vertex_id, VertexType &, EdgeType &, GatherType *gather_tmp,
                                int &misc_value) {
-      atomicMin(&gather_tmp[vertex_id], misc_value);
+      if (misc_value < gather_tmp[vertex_id]) atomicMin(&gather_tmp[vertex_id], misc_value);
     }
   };
 
