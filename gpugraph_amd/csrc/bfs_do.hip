@@ -6,13 +6,12 @@
 //   top-down  (queue -> queue): LBS expand over the frontier's out-edges
 //             (GASengine::lbs_launch with BfsPushOp); first touch wins an
 //             atomic test-and-set in the visited bitmap and stamps the depth.
-//   bottom-up (bitmap -> bitmap): one wave per 64 consecutive vertices;
-//             each unvisited vertex looks for a parent in the frontier
-//             bitmap (n/8 bytes, L2-resident) among its in-edges, stopping
-//             at the first hit.  Degree <= wave_degree: one lane, 4 loads in
-//             flight per step; larger: the whole wave, 64 edges per step.
-//             The wave owns its two bitmap words and writes them with plain
-//             stores.
+//   bottom-up (bitmap -> bitmap): a lane per unvisited vertex looks for a
+//             parent in the frontier bitmap among its in-edges, stopping at
+//             the first hit; the first four in-neighbours of every vertex are
+//             read from a dense 16-byte-per-vertex copy, the hubs' frontier
+//             bits from LDS (k_bfs_bottom_up).  A wave owns its 64 vertices'
+//             bitmap words and writes them with plain stores.
 //   switch    top-down -> bottom-up when m_f > m_u / alpha and the frontier
 //             grows; back when it shrinks below n / beta (Beamer et al.).
 //
@@ -66,62 +65,110 @@ struct BfsPushOp {
   __device__ __forceinline__ void finish() { block_counter_add<kExpandBlock>(&counters[1], mf); }
 };
 
-__device__ __forceinline__ uint32_t fbit(const uint32_t *__restrict__ b, int32_t u) {
-  return (b[u >> 5] >> (u & 31)) & 1u;
-}
-
+// Bottom-up level: each unvisited vertex looks for a parent (a frontier
+// vertex among its in-neighbours), stopping at the first hit.  One lane per
+// vertex, a wave per group of 64 consecutive vertices.  The first four
+// in-neighbours of every vertex also sit in a dense array of 16 bytes per
+// vertex (k_first_in_edges, built once per graph): a wave reads those of its
+// 64 vertices with one coalesced load instead of touching up to 64 rows, and
+// with the in-rows sorted by source id and the vertices renumbered by
+// degree they are the likeliest parents, so most vertices need nothing else.
+// The rest continue through their rows, 4 in-edges per step, or with the
+// whole wave when the row is longer than kBuSerialMax.  The frontier bits of
+// the kBuHotBits lowest-numbered vertices (the hubs) are copied to LDS at
+// the start, so most probes are LDS reads.  Software-pipelined over a wave's
+// groups: while group p is probed, the first in-neighbours of group p + 1 and
+// the visited word and in-row offsets of group p + 2 are in flight (only this
+// wave writes those groups' visited words).  The wave owns its groups'
+// bitmap words and writes them with plain stores; vertices without in-edges
+// were marked visited at the start of the run.
+//
 // ro: CSR offsets for the out-degree sum of the new frontier; nullptr on an
 // undirected (symmetric) graph, where the in-row length is the out-degree.
-// Each wave owns the two bitmap words of its 64 vertices per step, so the
-// next step's visited word and in-row offsets are loaded a step ahead.
-__global__ void __launch_bounds__(256)
-k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, const int32_t *__restrict__ ro,
-                int32_t *__restrict__ depth, uint32_t *__restrict__ visited, const uint32_t *__restrict__ front,
-                uint32_t *__restrict__ next, int32_t n, int32_t next_depth, int32_t serial_max,
+constexpr int kBuBlock = 256;
+constexpr int kBuHotWords = 4096;  // 16 KB of LDS
+constexpr int32_t kBuHotBits = kBuHotWords * 32;
+constexpr int kBuGridMax = 2048;
+constexpr int32_t kBuSerialMax = 512;
+static_assert(kBuHotWords % kBuBlock == 0, "the LDS copy takes the same number of words per thread");
+
+__global__ void __launch_bounds__(kBuBlock)
+k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, const int4 *__restrict__ first4,
+                const int32_t *__restrict__ ro, int32_t *__restrict__ depth, uint32_t *__restrict__ visited,
+                const uint32_t *__restrict__ front, uint32_t *__restrict__ next, int32_t n, int32_t next_depth,
                 unsigned long long *__restrict__ counters, const int32_t *__restrict__ skip,
                 const int32_t *__restrict__ level) {
   if (skip && *skip) return;
   if (level) next_depth = *level + 1;
   const int32_t words = (n + 31) >> 5;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  __shared__ uint32_t s_hot[kBuHotWords];
+  {
+    // All of a thread's loads in flight before its LDS stores.
+    constexpr int kPer = kBuHotWords / kBuBlock;
+    uint32_t t[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int i = (int)threadIdx.x + k * kBuBlock;
+      t[k] = i < words ? front[i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) s_hot[threadIdx.x + k * kBuBlock] = t[k];
+  }
+  __syncthreads();
+  auto fbit = [&](int32_t u) -> uint32_t {
+    return ((u < kBuHotBits ? s_hot[u >> 5] : front[u >> 5]) >> (u & 31)) & 1u;
+  };
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
   const int lane = lane_id();
   unsigned long long found_cnt = 0, mf = 0, examined = 0, scanned = 0;
   int64_t p = (((int64_t)blockIdx.x * blockDim.x) + threadIdx.x) >> 6;
-  uint32_t vw = 0;  // visited word of this lane's vertex, and its in-row
-  int32_t b = 0, e = 0;
-  {
-    const int64_t v = p * 64 + lane;
+  // In-row offsets of this lane's vertex of group q; b = e = 0 when it is
+  // visited or beyond n (nothing to scan).
+  auto rows = [&](int64_t q, int32_t &rb, int32_t &re) {
+    rb = re = 0;
+    const int64_t v = q * 64 + lane;
     if (v < n) {
-      vw = visited[2 * p + (lane >> 5)];
-      b = co[v];
-      e = co[v + 1];
-    }
-  }
-  for (; p * 64 < n; p += nwaves) {
-    const int32_t v = (int32_t)(p * 64 + lane);
-    uint32_t vw_next = 0;
-    int32_t b_next = 0, e_next = 0;
-    {
-      const int64_t pn = p + nwaves, vn = pn * 64 + lane;
-      if (vn < n) {
-        vw_next = visited[2 * pn + (lane >> 5)];
-        b_next = co[vn];
-        e_next = co[vn + 1];
+      const uint32_t w = visited[2 * q + (lane >> 5)];
+      if (!((w >> (lane & 31)) & 1u)) {
+        rb = co[v];
+        re = co[v + 1];
       }
     }
-    bool todo = v < n && !((vw >> (lane & 31)) & 1u) && e > b;
+  };
+  auto firsts = [&](int64_t q, int32_t rb, int32_t re) {
+    int4 f = make_int4(-1, -1, -1, -1);
+    if (re > rb) f = first4[q * 64 + lane];
+    return f;
+  };
+  int32_t b0, e0, b1, e1;
+  rows(p, b0, e0);
+  int4 f0 = firsts(p, b0, e0);
+  rows(p + nw, b1, e1);
+  for (; p * 64 < n; p += nw) {
+    const int4 f1 = firsts(p + nw, b1, e1);
+    int32_t b2, e2;
+    rows(p + 2 * nw, b2, e2);
+    const int32_t v = (int32_t)(p * 64 + lane);
+    const bool todo = e0 > b0;
     if (todo) scanned++;
-    const bool coop = todo && (e - b) > serial_max;
     bool found = false;
-    if (todo && !coop) {
-      int32_t k = b;
-      for (; k + 4 <= e && !found; k += 4) {
+    if (todo) {
+      // f0.x is an in-neighbour (row not empty); the others are -1 past the row's end
+      found = (fbit(f0.x) | (f0.y >= 0 ? fbit(f0.y) : 0u) | (f0.z >= 0 ? fbit(f0.z) : 0u) |
+               (f0.w >= 0 ? fbit(f0.w) : 0u)) != 0;
+      examined += (unsigned long long)min(4, e0 - b0);
+    }
+    const bool more = todo && !found && e0 - b0 > 4;
+    const bool coop = more && (e0 - b0) > kBuSerialMax;
+    if (more && !coop) {
+      int32_t k = b0 + 4;
+      for (; k + 4 <= e0 && !found; k += 4) {
         const int32_t u0 = ri[k], u1 = ri[k + 1], u2 = ri[k + 2], u3 = ri[k + 3];
-        found = (fbit(front, u0) | fbit(front, u1) | fbit(front, u2) | fbit(front, u3)) != 0;
+        found = (fbit(u0) | fbit(u1) | fbit(u2) | fbit(u3)) != 0;
         examined += 4;
       }
-      for (; k < e && !found; k++) {
-        found = fbit(front, ri[k]) != 0;
+      for (; k < e0 && !found; k++) {
+        found = fbit(ri[k]) != 0;
         examined++;
       }
     }
@@ -129,11 +176,11 @@ k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, 
     while (cm) {
       const int l = __builtin_ctzll(cm);
       cm &= cm - 1;
-      const int32_t wb = __shfl(b, l), we = __shfl(e, l);
+      const int32_t wb = __shfl(b0, l) + 4, we = __shfl(e0, l);
       bool hit = false;
       for (int32_t k0 = wb; k0 < we; k0 += kWave) {
         const int32_t k = k0 + lane;
-        const bool h = k < we && fbit(front, ri[k]);
+        const bool h = k < we && fbit(ri[k]);
         if (lane == 0) examined += (unsigned long long)min(kWave, we - k0);
         if (ballot(h)) {
           hit = true;
@@ -145,7 +192,7 @@ k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, 
     const uint64_t mask = ballot(found);
     if (found) {
       depth[v] = next_depth;
-      mf += (unsigned long long)(ro ? ro[v + 1] - ro[v] : e - b);
+      mf += (unsigned long long)(ro ? ro[v + 1] - ro[v] : e0 - b0);
     }
     if (lane < 2) {
       const int32_t ww = (int32_t)(2 * p + lane);
@@ -156,14 +203,43 @@ k_bfs_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, 
       }
     }
     if (lane == 0) found_cnt += (unsigned long long)__popcll(mask);
-    vw = vw_next;
-    b = b_next;
-    e = e_next;
+    b0 = b1;
+    e0 = e1;
+    f0 = f1;
+    b1 = b2;
+    e1 = e2;
   }
-  block_counter_add<256>(&counters[0], found_cnt);
-  block_counter_add<256>(&counters[1], mf);
-  block_counter_add<256>(&counters[2], examined);
-  block_counter_add<256>(&counters[3], scanned);
+  block_counter_add<kBuBlock>(&counters[0], found_cnt);
+  block_counter_add<kBuBlock>(&counters[1], mf);
+  block_counter_add<kBuBlock>(&counters[2], examined);
+  block_counter_add<kBuBlock>(&counters[3], scanned);
+}
+
+inline int bottom_up_grid(int32_t n) { return grid_for(((int64_t)n + 63) / 64, kBuBlock / kWave, kBuGridMax); }
+
+// first4[v] = the first four in-neighbours of v (CSC row order), -1 past the
+// end of a shorter row.
+__global__ void __launch_bounds__(256)
+k_first_in_edges(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, int32_t n, int4 *__restrict__ first4) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t b = co[v], d = co[v + 1] - b;
+    first4[v] = make_int4(d > 0 ? ri[b] : -1, d > 1 ? ri[b + 1] : -1, d > 2 ? ri[b + 2] : -1, d > 3 ? ri[b + 3] : -1);
+  }
+}
+
+// Bit v of the result: vertex v has no in-edges (CSC row empty).  Bits at
+// and beyond n stay clear.
+__global__ void __launch_bounds__(256)
+k_no_in_edges(const int32_t *__restrict__ co, int32_t n, uint32_t *__restrict__ bits) {
+  const int32_t words = (n + 31) >> 5;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int lane = lane_id();
+  for (int64_t p = (((int64_t)blockIdx.x * blockDim.x) + threadIdx.x) >> 6; p * 64 < n; p += nwaves) {
+    const int64_t v = p * 64 + lane;
+    const uint64_t mask = ballot(v < n && co[v + 1] == co[v]);
+    const int64_t w = 2 * p + lane;
+    if (lane < 2 && w < words) bits[w] = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
+  }
 }
 
 __global__ void k_queue_to_bits(const int32_t *__restrict__ queue, const int32_t *__restrict__ qsize,
@@ -345,17 +421,19 @@ static_assert(sizeof(BfsCtl) % 8 == 0, "read back in 8-byte words");
 
 // The start of a run in one launch, every element at its final initial value
 // (no fill then patch, so no ordering between threads is needed): depth = -1
-// except depth[src] = 0; visited clear except src's bit; the counters zero
+// except depth[src] = 0; visited = the vertices without in-edges (no level
+// can reach them) plus src; the counters zero
 // except [1] = out-degree of the source and the first queue size (low half
 // of slot 8) = 1; queue[0] = src; and, for the device-driven loop, its state
 // block.
-__global__ void k_bfs_start(int32_t *depth, int32_t n, uint32_t *visited, int32_t words, unsigned long long *ctr,
-                            int32_t *queue, const int32_t *ro, int32_t src, BfsCtl *c, int64_t m, double bytes0) {
+__global__ void k_bfs_start(int32_t *depth, int32_t n, uint32_t *visited, const uint32_t *noin, int32_t words,
+                            unsigned long long *ctr, int32_t *queue, const int32_t *ro, int32_t src, BfsCtl *c,
+                            int64_t m, double bytes0) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const long long mf = (long long)(ro[src + 1] - ro[src]);
   for (int64_t i = t0; i < n; i += stride) depth[i] = i == src ? 0 : -1;
-  for (int64_t i = t0; i < words; i += stride) visited[i] = i == (src >> 5) ? 1u << (src & 31) : 0u;
+  for (int64_t i = t0; i < words; i += stride) visited[i] = noin[i] | (i == (src >> 5) ? 1u << (src & 31) : 0u);
   for (int64_t i = t0; i < kCounterWords; i += stride)
     ctr[i] = i == 1 ? (unsigned long long)mf : i == 8 ? 1ull : 0ull;  // slot 8: int32 queue sizes {1, 0}
   if (t0 == 0) {
@@ -521,17 +599,36 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
     GG_TRY(g->loop_ctl.ensure(sizeof(BfsCtl)));
     ctl = g->loop_ctl.as<BfsCtl>();
   }
+  // Vertices without in-edges, computed once per graph: every run starts
+  // with them visited.
+  if (!g->noin_ready) {
+    GG_TRY(g->noin.ensure(4 * (size_t)words));
+    hipLaunchKernelGGL(k_no_in_edges, dim3(grid_for((int64_t)n, 256, 2048)), dim3(256), 0, s, g->dg.column_offsets, n,
+                       g->noin.as<uint32_t>());
+    GG_HIP(hipGetLastError());
+    g->noin_ready = true;
+  }
+  const uint32_t *noin = g->noin.as<uint32_t>();
+  // The first four in-neighbours of every vertex, dense (bottom-up levels).
+  if (!g->first4_ready) {
+    GG_TRY(g->first4.ensure(16 * (size_t)n));
+    hipLaunchKernelGGL(k_first_in_edges, dim3(grid_for((int64_t)n, 256, 8192)), dim3(256), 0, s,
+                       g->dg.column_offsets, g->dg.row_indices, n, g->first4.as<int4>());
+    GG_HIP(hipGetLastError());
+    g->first4_ready = true;
+  }
+  const int4 *first4 = g->first4.as<int4>();
   {
     TimedRegion t(g->prof, "bfs.init");
     hipLaunchKernelGGL(k_bfs_start, dim3(grid_for(std::max<int64_t>(n, kCounterWords), 256, 8192)), dim3(256), 0, s,
-                       depth, n, visited, words, ctr, queue[0], g->dg.row_offsets, src, ctl, g->m,
-                       4.0 * n + words * 4.0);
+                       depth, n, visited, noin, words, ctr, queue[0], g->dg.row_offsets, src, ctl, g->m,
+                       4.0 * n + words * 8.0);
     GG_HIP(hipGetLastError());
   }
-  g->prof.add_bytes("bfs.init", 4.0 * n + words * 4.0);
+  g->prof.add_bytes("bfs.init", 4.0 * n + words * 8.0);
   int64_t mf = 0, mu = 0, nf = 1, prev_nf = 0;
   int64_t reached = 1, reached_edges = 0, examined_total = 0;
-  double bytes = 4.0 * n + words * 4.0;
+  double bytes = 4.0 * n + words * 8.0;
   const int alpha = g->opt.bfs_alpha, beta = g->opt.bfs_beta;
   bool bottom_up = false;
   int cur = 0;  // index of the current queue / frontier bitmap
@@ -562,10 +659,9 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
                            g->opt.iter_num, 0);
         hipLaunchKernelGGL(k_bfs_convert, dim3(conv_grid), dim3(256), 0, s, depth, (const int32_t *)&ctl->level,
                            (const int32_t *)&ctl->q2b, (const int32_t *)&ctl->b2q, n, fbits[c], queue[c], qsz[c]);
-        hipLaunchKernelGGL(k_bfs_bottom_up, dim3(grid_for((int64_t)words * 32, 256, 2048)), dim3(256), 0, s,
-                           g->dg.column_offsets, g->dg.row_indices, g->directed ? g->dg.row_offsets : nullptr, depth,
-                           visited, fbits[c], fbits[x], n, 0, g->opt.wave_degree, ctr, (const int32_t *)&ctl->bu_skip,
-                           (const int32_t *)&ctl->level);
+        hipLaunchKernelGGL(k_bfs_bottom_up, dim3(bottom_up_grid(n)), dim3(kBuBlock), 0, s, g->dg.column_offsets,
+                           g->dg.row_indices, first4, g->directed ? g->dg.row_offsets : nullptr, depth, visited,
+                           fbits[c], fbits[x], n, 0, ctr, (const int32_t *)&ctl->bu_skip, (const int32_t *)&ctl->level);
         BfsPushOp op{g->dg.row_offsets, g->dg.column_indices, depth, visited, ctr, 0, &ctl->level, 0ull};
         lbs_launch(op, false, queue[c], &ctl->push_size, n, (int64_t)n + g->m, lb, queue[x], qsz[x], s, [](int) {});
         hipLaunchKernelGGL((k_tiered_expand<BfsPushOp, false>), dim3(grid_for(kTierItems, kExpandBlock)),
@@ -636,10 +732,9 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
     if (bottom_up) {
       {
         TimedRegion t(g->prof, "bfs.bottom_up");
-        hipLaunchKernelGGL(k_bfs_bottom_up, dim3(grid_for((int64_t)words * 32, 256, 2048)), dim3(256), 0, s,
-                           g->dg.column_offsets, g->dg.row_indices, g->directed ? g->dg.row_offsets : nullptr,
-                           depth, visited, fbits[cur],
-                           fbits[nxt], n, levels + 1, g->opt.wave_degree, ctr, (const int32_t *)nullptr,
+        hipLaunchKernelGGL(k_bfs_bottom_up, dim3(bottom_up_grid(n)), dim3(kBuBlock), 0, s, g->dg.column_offsets,
+                           g->dg.row_indices, first4, g->directed ? g->dg.row_offsets : nullptr, depth, visited,
+                           fbits[cur], fbits[nxt], n, levels + 1, ctr, (const int32_t *)nullptr,
                            (const int32_t *)nullptr);
         GG_HIP(hipGetLastError());
       }

@@ -132,6 +132,40 @@ __device__ __forceinline__ void block_counter_add(unsigned long long *counter, u
   if (threadIdx.x == 0 && t) atomicAdd(stripe_of<STRIDE>(counter), t);
 }
 
+// ---- single-pass scan across workgroups ------------------------------
+//
+// For grids of at most BLOCK workgroups: workgroup b publishes its aggregate
+// in status[b] (bit 63 set over a 63-bit value), then its threads read the
+// words of workgroups 0..b-1 at once, one each, waiting on any not yet
+// published, and sum them.  No word is written twice and no address takes a
+// second write, so nothing serialises the way one shared atomicAdd per
+// workgroup does, and there is no chain of prefixes to wait along.  Values
+// are sums of packed fields that must not carry into each other (the caller's
+// packing keeps every field's total in range).  The words start at 0 for
+// every scan.  Workgroups wait only on lower-numbered ones, which are
+// dispatched before them.  All threads of the workgroup must call it; a
+// workgroup that does not need its prefix (need false, the same in all its
+// threads) only publishes and gets 0.
+constexpr uint64_t kScanPublished = 1ull << 63;
+constexpr uint64_t kScanValueMask = kScanPublished - 1;
+
+template <int BLOCK>
+__device__ __forceinline__ uint64_t block_exclusive_across(uint64_t *status, int b, uint64_t aggregate, bool need) {
+  __shared__ unsigned long long red[BLOCK / kWave];
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&status[b], kScanPublished | aggregate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!need) return 0;
+  unsigned long long v = 0;
+  if ((int)threadIdx.x < b) {
+    uint64_t w;
+    while (!((w = __hip_atomic_load(&status[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) &
+             kScanPublished))
+      __builtin_amdgcn_s_sleep(1);
+    v = w & kScanValueMask;
+  }
+  return block_sum_u64<BLOCK>(v, red);
+}
+
 // Workgroup-aggregated queue append: one global atomic per call.  All
 // threads of the workgroup must call it.
 template <int BLOCK>

@@ -254,6 +254,11 @@ struct gg_graph {
   // (directed graphs; undirected graphs read the CSR values directly).
   gg::DevBuf pull_parts, w_csc;
   bool pull_parts_ready = false;
+  // Bitmap of the vertices without in-edges (BFS starts with them visited),
+  // and the first four in-neighbours of every vertex (16 bytes each, -1 past
+  // a shorter row) for bottom-up BFS levels.
+  gg::DevBuf noin, first4;
+  bool noin_ready = false, first4_ready = false;
   // The same partition over (vertices, CSR out-edges), and the per-tile
   // carries of the segmented row sums.
   gg::DevBuf csr_parts, seg_carry;
@@ -282,6 +287,9 @@ struct gg_graph {
   gg::CapturedLoop sssp_loops[gg::kSsspReplayMax + 1];  // by iterations per replay
   int sssp_steps_hint = 0;          // loop iterations the last device-driven SSSP took
   gg::DevBuf loop_ctl;
+  // Aggregate words of single-pass scans across workgroups (the SSSP
+  // collect: two sets, by collect parity).
+  gg::DevBuf scan_words;
 
   gg::EventProfiler prof;
 

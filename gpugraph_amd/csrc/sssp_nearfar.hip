@@ -11,17 +11,22 @@
 // State: dist[n], dirty[n] (byte: distance dropped since the vertex was last
 // expanded), frontier queue.  Per iteration:
 //   collect  every dirty vertex with dist < bound joins the frontier (queue,
-//            dirty cleared); counts the frontier's out-edges and
-//            the smallest distance still dirty.  Empty frontier with dirty
-//            vertices left: bound jumps to the bucket of that distance.
-//   push     frontier edges <= m / pull_alpha: LBS expand over the queue
+//            dirty cleared), in vertex order, with its out-degree offset:
+//            one launch that is also the LBS count and scan (a single-pass
+//            scan across workgroups) and the controller (its
+//            last workgroup decides push, tiered push, pull or end).  Empty
+//            frontier with dirty vertices left: the next collect moves the
+//            bound to the bucket of the smallest distance still dirty.
+//   push     frontier edges <= m / pull_alpha: LBS partition + expand over
+//            the queue
 //            (one tiered launch instead when the frontier is small: at most
 //            kTierItems vertices and kTierEdges edges),
 //            nd = dist[v] + w, plain pre-check then atomicMin(dist[nb]); an
 //            improvement stores dirty[nb] = 1 (no atomic, no enqueue).  On an
 //            undirected graph whose in-edges are packed (below) the push
 //            reads the same packed words: they are the CSR's own slots.
-//   pull     otherwise: merge-path pass over every vertex's in-edges, min
+//   pull     otherwise, unless the previous iteration pulled: merge-path
+//            pass over every vertex's in-edges, min
 //            of dist[u] + w over all in-neighbours u reduced in LDS (each
 //            thread's edges loaded back to back), written with a plain store
 //            when the tile holds all of the vertex's in-edges (atomicMin at
@@ -30,7 +35,8 @@
 //            pull relaxed every in-edge, so bucket order saves no more work
 //            and the tail needs fewer iterations).
 //
-// Byte model: collect n + 4(dirty) + 4F; push 16F + 12E + 8N;
+// Byte model: collect n + 16F (dirty bytes; dist and offsets read, queue
+// slot and offset written per frontier vertex); push 16F + 12E + 8N;
 // pull 4n + 12m (offsets; source, weight and distance per in-edge: the same
 // 12 bytes per inspected edge as the push).  Both count 12 bytes per edge
 // whichever layout is read; the packed words move 8 of them.
@@ -51,12 +57,13 @@ constexpr int kLoopIters = 12;  // iterations per replay after the first
 static_assert(kLoopIters <= kSsspReplayMax, "replay lengths are cached up to kSsspReplayMax");
 
 // counters (striped, GASengine::block_counter_add): [2] LBS edge counter,
-// [4] improvements, [16] dirty left, [17] min dist left (minimum over the
-// stripes), [18] out-degree sum of the frontier; slots 8, 9 hold the int32
-// queue sizes (stripe 0 only).  Reached vertices
+// [4] improvements, [16..19] the collect's dirty-left counts and minimum
+// distances (two pairs, by collect parity); slot 8 holds the int32 queue
+// sizes the expand kernels append to (stripe 0 only; the push never
+// appends).  Reached vertices
 // and their degree sum (the run's statistics) come from one pass over the
 // final distances (reach_stats), not from the relaxations.
-// Small pushes run as one tiered launch instead of the four LBS launches.
+// Small pushes run as one tiered launch instead of the LBS partition and expand.
 constexpr int32_t kTierItems = 2048;
 constexpr int64_t kTierEdges = 65536;
 struct Tally {
@@ -257,87 +264,53 @@ int packed_in_edges(gg_graph *g, const int32_t *wc, const uint32_t **pk, int32_t
   return GG_OK;
 }
 
-// Clears the collect's counters (queue sizes, dirty left, min dist left in
-// every stripe, frontier degree sum) for the next collect.
-__device__ __forceinline__ void sssp_reset_collect(unsigned long long *ctr) {
-  ctr[8] = 0ull;  // both int32 queue sizes
-  for (int q = 0; q < kStripes; q++) {
-    ctr[q * kCounterSlots + 16] = 0ull;
-    ctr[q * kCounterSlots + 17] = (unsigned long long)INT_MAX;
-    ctr[q * kCounterSlots + 18] = 0ull;
-  }
-}
 
-// State of the device-driven loop: k_sssp_ctrl takes, after each collect,
-// the decisions the host loop takes (bucket advance, push or pull, end),
-// after copying what it needs out of the counters and clearing them.
+// Counter slots of the collect (striped): dirty vertices left above the
+// bound and their smallest distance, one pair per collect parity (slots
+// kLeftSlot + 2p and kLeftSlot + 2p + 1).  Collect e adds into pair e & 1 and
+// reads pair (e - 1) & 1 when it has to move the bound; its last workgroup
+// clears the pair it read.
+constexpr int kLeftSlot = 16;
+
+// State of a run, on the device.  Every collect reads it at the start of
+// each workgroup; only the collect's last workgroup writes it, after its
+// scan has read every other workgroup's published aggregate (so every
+// workgroup has read the state by then), and the push / pull launches after
+// the collect read its decisions.
 struct SsspCtl {
   int32_t bound;      // frontier bound of the current bucket
   int32_t push_size;  // frontier items the LBS push sees: the queue size when pushing by LBS, else 0
   int32_t tier_size;  // frontier items the tiered push sees: the queue size when the push is small, else 0
   int32_t pull_skip;  // 0 when this iteration pulls
   int32_t done;
-  int32_t pad;
+  int32_t advance;    // the last collect selected nothing: the next one moves the bound (or ends the run)
+  int32_t epoch;      // collects run; its parity picks the scan words and counter slots
   int32_t iters, pulls;
-  int32_t steps;      // controller calls that found the run unfinished (the last one ends it)
+  int32_t steps;      // collects that ran (the last one ends the run)
+  int32_t last_nf;    // frontier size of the last collect
+  int32_t prev_pull;  // the last iteration pulled
+  long long last_deg; // its out-degree sum
   long long edges;    // edges inspected
   double bytes;       // algorithmic bytes (model in the header comment)
 };
 static_assert(sizeof(SsspCtl) % 8 == 0, "read back in 8-byte words");
 
-__global__ void k_sssp_ctrl(SsspCtl *c, unsigned long long *__restrict__ ctr, int32_t n, int64_t m, int64_t delta,
-                            int64_t pull_alpha, int32_t max_iter, double pull_bytes) {
-  c->push_size = 0;
-  c->tier_size = 0;
-  c->pull_skip = 1;
-  if (c->done) return;
-  c->steps++;
-  if (c->iters >= max_iter) {
-    c->done = 1;
-    return;
-  }
-  const int64_t nf = reinterpret_cast<const int32_t *>(ctr + 8)[0];
-  const unsigned long long remain = stripe_sum(ctr + 16), degsum = stripe_sum(ctr + 18);
-  unsigned long long dmin = ctr[17];
-  for (int q = 1; q < kStripes; q++) dmin = min(dmin, ctr[17 + q * kCounterSlots]);
-  sssp_reset_collect(ctr);
-  c->bytes += n + 4.0 * (double)(nf + (int64_t)remain) + 4.0 * nf;
-  if (nf == 0) {
-    if (remain == 0)
-      c->done = 1;
-    else
-      c->bound = (int32_t)min<int64_t>(((int64_t)dmin / delta + 1) * delta, kInf);
-    return;
-  }
-  if (pull_alpha > 0 && (int64_t)degsum > m / pull_alpha) {
-    c->pull_skip = 0;
-    c->bound = kInf;  // after a pull, buckets end: every dirty vertex is collected
-    c->bytes += pull_bytes;
-    c->edges += m;
-    c->pulls++;
-  } else {
-    if (nf <= kTierItems && (int64_t)degsum <= kTierEdges)
-      c->tier_size = (int32_t)nf;
-    else
-      c->push_size = (int32_t)nf;
-    c->bytes += 16.0 * nf + 12.0 * (double)degsum;
-    c->edges += (long long)degsum;
-  }
-  c->iters++;
-}
-
-// A workgroup takes 4096 consecutive vertices per step, 16 per thread: the
-// thread's 16 dirty bytes arrive in one 16-byte load (the dirty array is
-// padded with zeros to a multiple of 16), and a step where no byte is set
-// ends there, after one barrier.  For the set bytes the thread reads the 16
-// distances and 17 row offsets as 16-byte vectors (VEC: dist and the offsets
-// 16-byte aligned and the 16 vertices inside n; otherwise one load per set
-// byte).  Selected
-// vertices go to the queue in vertex order within a step, one global atomic
-// per step; the counters take one atomic per workgroup.
-constexpr int kCollectBlock = 256;
+// A workgroup takes kCollectStep consecutive vertices per tile, 16 per
+// thread: the thread's 16 dirty bytes arrive in one 16-byte load (the dirty
+// array is padded with zeros to a multiple of 16), and a tile where no byte is
+// set ends there, after one barrier.  For the set bytes the thread reads the
+// 16 distances and 17 row offsets as 16-byte vectors (VEC: dist and the
+// offsets 16-byte aligned and the 16 vertices inside n; otherwise one load
+// per set byte).
+constexpr int kCollectBlock = 512;
 constexpr int kCollectPer = 16;
 constexpr int kCollectStep = kCollectPer * kCollectBlock;
+// At most this many workgroups (each then takes a run of consecutive tiles),
+// so that a workgroup's threads read all its predecessors' aggregates at once.
+constexpr int kCollectGridMax = kCollectBlock;
+// Frontier size and out-degree sum share one scan word: 31 bits each.
+constexpr int kPackShift = 31;
+constexpr uint64_t kPackMask = (1ull << kPackShift) - 1;
 
 // Bits j of the result: byte j of the 16 bytes is nonzero.
 __device__ __forceinline__ uint32_t nonzero_bytes(uint4 d) {
@@ -357,100 +330,251 @@ __device__ __forceinline__ uint4 clear_bytes(uint4 d, uint32_t clear) {
   return make_uint4(w[0], w[1], w[2], w[3]);
 }
 
-template <bool VEC>
-__global__ void __launch_bounds__(kCollectBlock)
-k_sssp_collect(uint8_t *__restrict__ dirty, const int32_t *__restrict__ dist, const int32_t *__restrict__ ro,
-               int32_t n, int32_t bound, int32_t *__restrict__ queue, int32_t *__restrict__ qsize,
-               unsigned long long *__restrict__ counters, const SsspCtl *__restrict__ ctl) {
-  if (ctl) {  // device-driven loop: the bound is on the device, and a finished run skips
-    if (ctl->done) return;
-    bound = ctl->bound;
-  }
-  __shared__ int32_t s_scan[kCollectBlock / kWave + 1];
-  __shared__ int32_t s_base;
-  __shared__ int32_t s_min[kCollectBlock / kWave];
-  const int lane = lane_id(), wid = threadIdx.x / kWave;
-  unsigned long long remain = 0, degsum = 0;
-  int32_t dmin = INT_MAX;
-  for (int64_t base = (int64_t)blockIdx.x * kCollectStep; base < n; base += (int64_t)gridDim.x * kCollectStep) {
-    const int64_t v0 = base + (int64_t)threadIdx.x * kCollectPer;
-    uint4 d4 = make_uint4(0u, 0u, 0u, 0u);
-    if (v0 < n) d4 = *reinterpret_cast<const uint4 *>(dirty + v0);
-    const uint32_t nz = nonzero_bytes(d4);
-    if (!__syncthreads_or(nz != 0)) continue;
-    uint32_t sel = 0;  // bit j: vertex v0 + j selected
-    if (nz) {
-      if (VEC && v0 + kCollectPer <= n) {
-        int32_t dv[16], rv[17];
-        const int4 *dp = reinterpret_cast<const int4 *>(dist + v0);
+struct CollectArgs {
+  uint8_t *dirty;
+  const int32_t *dist;
+  const int32_t *ro;
+  int32_t n;
+  int32_t tiles;            // kCollectStep-vertex tiles over n
+  int32_t tiles_per_block;  // consecutive tiles per workgroup
+  int32_t max_iter;
+  int32_t *queue;
+  int32_t *scan_local;      // LBS offsets: global exclusive out-degree prefix per queue slot
+  int32_t *partials;        // LBS partials: 0 (scan_local holds the global prefix)
+  LbsInfo *info;
+  unsigned long long *ctr;
+  unsigned long long *edge_counter;
+  uint64_t *status;         // scan words, 2 x gridDim.x (one set per collect parity)
+  SsspCtl *ctl;
+  int64_t m, delta, pull_alpha;
+  double pull_bytes;
+};
+
+// The thread's 16 vertices from v0: bit j of the result selects v0 + j
+// (dirty and dist < bound), deg[j] its out-degree.  ACC: count the dirty
+// vertices left (dist >= bound) and their smallest distance.
+template <bool VEC, bool ACC>
+__device__ __forceinline__ uint32_t collect_select(const CollectArgs &a, int64_t v0, int32_t bound, uint4 &d4,
+                                                   int32_t (&deg)[kCollectPer], unsigned long long &left,
+                                                   int32_t &dmin) {
+  d4 = make_uint4(0u, 0u, 0u, 0u);
+  if (v0 < a.n) d4 = *reinterpret_cast<const uint4 *>(a.dirty + v0);
+  const uint32_t nz = nonzero_bytes(d4);
+  uint32_t sel = 0;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const int4 x = dp[q];
-          dv[4 * q] = x.x;
-          dv[4 * q + 1] = x.y;
-          dv[4 * q + 2] = x.z;
-          dv[4 * q + 3] = x.w;
-        }
-        const int4 *rp = reinterpret_cast<const int4 *>(ro + v0);
+  for (int j = 0; j < kCollectPer; j++) deg[j] = 0;
+  if (!nz) return 0;
+  if (VEC && v0 + kCollectPer <= a.n) {
+    int32_t dv[16], rv[17];
+    const int4 *dp = reinterpret_cast<const int4 *>(a.dist + v0);
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const int4 x = rp[q];
-          rv[4 * q] = x.x;
-          rv[4 * q + 1] = x.y;
-          rv[4 * q + 2] = x.z;
-          rv[4 * q + 3] = x.w;
-        }
-        rv[16] = ro[v0 + 16];  // v0 + 16 <= n: offsets has n + 1 entries
+    for (int q = 0; q < 4; q++) {
+      const int4 x = dp[q];
+      dv[4 * q] = x.x;
+      dv[4 * q + 1] = x.y;
+      dv[4 * q + 2] = x.z;
+      dv[4 * q + 3] = x.w;
+    }
+    const int4 *rp = reinterpret_cast<const int4 *>(a.ro + v0);
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-          if (!(nz & (1u << j))) continue;
-          if (dv[j] < bound) {
-            sel |= 1u << j;
-            degsum += (unsigned long long)(rv[j + 1] - rv[j]);
-          } else {
-            remain++;
-            dmin = min(dmin, dv[j]);
-          }
-        }
-      } else {
+    for (int q = 0; q < 4; q++) {
+      const int4 x = rp[q];
+      rv[4 * q] = x.x;
+      rv[4 * q + 1] = x.y;
+      rv[4 * q + 2] = x.z;
+      rv[4 * q + 3] = x.w;
+    }
+    rv[16] = a.ro[v0 + 16];  // v0 + 16 <= n: offsets has n + 1 entries
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-          if (!(nz & (1u << j))) continue;
-          const int64_t v = v0 + j;  // a set byte lies below n
-          const int32_t d = dist[v];
-          if (d < bound) {
-            sel |= 1u << j;
-            degsum += (unsigned long long)(ro[v + 1] - ro[v]);
-          } else {
-            remain++;
-            dmin = min(dmin, d);
-          }
-        }
+    for (int j = 0; j < 16; j++) {
+      if (!(nz & (1u << j))) continue;
+      if (dv[j] < bound) {
+        sel |= 1u << j;
+        deg[j] = rv[j + 1] - rv[j];
+      } else if (ACC) {
+        left++;
+        dmin = min(dmin, dv[j]);
       }
-      if (sel) *reinterpret_cast<uint4 *>(dirty + v0) = clear_bytes(d4, sel);
     }
-    int32_t total;
-    const int32_t off = block_exclusive_scan<kCollectBlock>(__popc(sel), s_scan, &total);
-    if (total) {
-      if (threadIdx.x == 0) s_base = atomicAdd(qsize, total);
-      __syncthreads();
-      int32_t at = s_base + off;
+  } else {
 #pragma unroll
-      for (int j = 0; j < 16; j++)
-        if (sel & (1u << j)) queue[at++] = (int32_t)(v0 + j);
-      __syncthreads();
+    for (int j = 0; j < 16; j++) {
+      if (!(nz & (1u << j))) continue;
+      const int64_t v = v0 + j;  // a set byte lies below n
+      const int32_t d = a.dist[v];
+      if (d < bound) {
+        sel |= 1u << j;
+        deg[j] = a.ro[v + 1] - a.ro[v];
+      } else if (ACC) {
+        left++;
+        dmin = min(dmin, d);
+      }
     }
   }
-  block_counter_add<kCollectBlock>(&counters[16], remain);
-  block_counter_add<kCollectBlock>(&counters[18], degsum);
+  return sel;
+}
+
+// Writes the thread's selected vertices from queue slot *pos on (global
+// out-degree offset *eoff on) and clears their dirty bytes.
+__device__ __forceinline__ void collect_write(const CollectArgs &a, int64_t v0, uint4 d4, uint32_t sel,
+                                              const int32_t (&deg)[kCollectPer], int32_t &pos, int32_t &eoff) {
+  if (!sel) return;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) dmin = min(dmin, __shfl_xor(dmin, off));
-  if (lane == 0) s_min[wid] = dmin;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int32_t mn = INT_MAX;
-    for (int w = 0; w < kCollectBlock / kWave; w++) mn = min(mn, s_min[w]);
-    if (mn != INT_MAX) atomicMin(stripe_of(&counters[17]), (unsigned long long)mn);
+  for (int j = 0; j < kCollectPer; j++) {
+    if (!(sel & (1u << j))) continue;
+    a.queue[pos] = (int32_t)(v0 + j);
+    a.scan_local[pos] = eoff;
+    if (pos % kCountTile == 0) a.partials[pos / kCountTile] = 0;
+    pos++;
+    eoff += deg[j];
+  }
+  *reinterpret_cast<uint4 *>(a.dirty + v0) = clear_bytes(d4, sel);
+}
+
+// The decisions of one iteration, taken by the collect's last workgroup
+// once the frontier size nf and its out-degree sum are known: bucket
+// advance, push (LBS or tiered) or pull, end.
+__device__ void sssp_decide(const CollectArgs &a, int32_t bound, int32_t nf, int64_t degsum) {
+  SsspCtl *c = a.ctl;
+  c->epoch++;
+  c->steps++;
+  c->bound = bound;
+  c->advance = 0;
+  c->push_size = 0;
+  c->tier_size = 0;
+  c->pull_skip = 1;
+  c->last_nf = nf;
+  c->last_deg = degsum;
+  a.info->edges = 0;
+  a.info->path = 0;
+  c->bytes += a.n + 16.0 * nf;
+  if (c->iters >= a.max_iter) {
+    c->done = 1;
+    return;
+  }
+  if (nf == 0) {
+    c->advance = 1;
+    return;
+  }
+  // Pull when the frontier's edges exceed m / pull_alpha, but never twice in
+  // a row: after a pull, a push costs about as much per edge as a pull and
+  // touches only the frontier's edges (degsum <= m).
+  const bool pull = a.pull_alpha > 0 && !c->prev_pull && degsum > a.m / a.pull_alpha;
+  c->prev_pull = pull;
+  if (pull) {
+    c->pull_skip = 0;
+    c->bound = kInf;  // after a pull, buckets end: every dirty vertex is collected
+    c->bytes += a.pull_bytes;
+    c->edges += a.m;
+    c->pulls++;
+  } else {
+    if (nf <= kTierItems && degsum <= kTierEdges) {
+      c->tier_size = nf;
+    } else {
+      c->push_size = nf;
+      a.info->edges = (int32_t)degsum;
+      a.info->path = (int64_t)nf + degsum;
+      atomicAdd(a.edge_counter, (unsigned long long)degsum);
+    }
+    c->bytes += 16.0 * nf + 12.0 * (double)degsum;
+    c->edges += degsum;
+  }
+  c->iters++;
+}
+
+// The collect, fused with the LBS count and scan and with the controller:
+// every dirty vertex with dist < bound joins the frontier (dirty cleared),
+// at a queue slot and out-degree offset from a single-pass scan across the
+// workgroups, so the queue comes out in vertex order with the LBS
+// offsets the partition and expand read (partials all 0).  The last
+// workgroup then takes the iteration's decisions.  When the previous collect
+// selected nothing, every workgroup first moves the bound to the bucket of
+// the smallest distance still dirty (the same value everywhere), or finds the
+// run finished.
+template <bool VEC>
+__global__ void __launch_bounds__(kCollectBlock) k_sssp_collect(CollectArgs a) {
+  SsspCtl *c = a.ctl;
+  if (c->done) return;
+  const int par = c->epoch & 1;
+  int32_t bound = c->bound;
+  const int64_t G = gridDim.x, b = blockIdx.x;
+  const bool last = b == G - 1;
+  if (c->advance) {
+    const unsigned long long *prev = a.ctr + kLeftSlot + 2 * (par ^ 1);
+    const unsigned long long left = stripe_sum(prev);
+    if (left == 0) {  // nothing dirty anywhere: the run is over
+      if (last && threadIdx.x == 0) {
+        c->steps++;
+        c->done = 1;
+      }
+      return;
+    }
+    unsigned long long dm = prev[1];
+    for (int q = 1; q < kStripes; q++) dm = min(dm, prev[1 + q * kCounterSlots]);
+    bound = (int32_t)min<int64_t>(((int64_t)dm / a.delta + 1) * a.delta, kInf);
+  }
+  // This workgroup's word of the next collect's scan (the previous collect,
+  // which used it, has finished).
+  if (threadIdx.x == 0) a.status[(par ^ 1) * G + b] = 0ull;
+  __shared__ int32_t s_scan[2][kCollectBlock / kWave + 1];
+  const int64_t t0 = b * a.tiles_per_block;
+  const int64_t t1 = min<int64_t>(t0 + a.tiles_per_block, a.tiles);
+  unsigned long long left = 0;
+  int32_t dmin = INT_MAX;
+  int32_t cnt = 0, dsum = 0;
+  uint4 d4 = make_uint4(0u, 0u, 0u, 0u);
+  uint32_t sel = 0;
+  int32_t deg[kCollectPer];
+  for (int64_t t = t0; t < t1; t++) {
+    const int64_t v0 = t * kCollectStep + (int64_t)threadIdx.x * kCollectPer;
+    sel = collect_select<VEC, true>(a, v0, bound, d4, deg, left, dmin);
+    cnt += __popc(sel);
+#pragma unroll
+    for (int j = 0; j < kCollectPer; j++) dsum += deg[j];
+  }
+  int32_t tot_cnt, tot_dsum;
+  int32_t pos = block_exclusive_scan<kCollectBlock>(cnt, s_scan[0], &tot_cnt);
+  int32_t eoff = block_exclusive_scan<kCollectBlock>(dsum, s_scan[1], &tot_dsum);
+  const uint64_t agg = ((uint64_t)tot_cnt << kPackShift) | (uint64_t)tot_dsum;
+  // A workgroup with nothing selected needs no prefix, except the last one,
+  // whose prefix plus its own aggregate is the frontier's size and degree sum.
+  const uint64_t excl = block_exclusive_across<kCollectBlock>(a.status + par * G, (int)b, agg, tot_cnt > 0 || last);
+  pos += (int32_t)(excl >> kPackShift);
+  eoff += (int32_t)(excl & kPackMask);
+  if (t1 - t0 == 1) {
+    collect_write(a, t0 * kCollectStep + (int64_t)threadIdx.x * kCollectPer, d4, sel, deg, pos, eoff);
+  } else {
+    for (int64_t t = t0; t < t1; t++) {
+      const int64_t v0 = t * kCollectStep + (int64_t)threadIdx.x * kCollectPer;
+      sel = collect_select<VEC, false>(a, v0, bound, d4, deg, left, dmin);
+      collect_write(a, v0, d4, sel, deg, pos, eoff);
+    }
+  }
+  // What is left above the bound, for a later bucket move.
+  block_counter_add<kCollectBlock>(&a.ctr[kLeftSlot + 2 * par], left);
+  {
+    __shared__ int32_t s_min[kCollectBlock / kWave];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) dmin = min(dmin, __shfl_xor(dmin, off));
+    if (lane_id() == 0) s_min[threadIdx.x / kWave] = dmin;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int32_t mn = INT_MAX;
+      for (int w = 0; w < kCollectBlock / kWave; w++) mn = min(mn, s_min[w]);
+      if (mn != INT_MAX) atomicMin(stripe_of(&a.ctr[kLeftSlot + 2 * par + 1]), (unsigned long long)mn);
+    }
+  }
+  if (last) {
+    // Every workgroup has read the state and the previous pair of counters
+    // (it published its aggregate after): clear that pair, then decide.
+    if (threadIdx.x < kStripes * 2) {
+      const int q = threadIdx.x >> 1, k = threadIdx.x & 1;
+      a.ctr[q * kCounterSlots + kLeftSlot + 2 * (par ^ 1) + k] = k ? (unsigned long long)INT_MAX : 0ull;
+    }
+    if (threadIdx.x == 0) {
+      const uint64_t incl = excl + agg;
+      sssp_decide(a, bound, (int32_t)(incl >> kPackShift), (int64_t)(incl & kPackMask));
+    }
   }
 }
 
@@ -458,10 +582,11 @@ k_sssp_collect(uint8_t *__restrict__ dirty, const int32_t *__restrict__ dist, co
 // value (no fill then patch, so no ordering between threads is needed):
 // dist = kInf except dist[src] = 0; the dirty bytes (16 per thread, the array
 // padded to n16 x 16 bytes) clear except src's; the counters zero except the
-// collect's minimum slot, at its identity in every stripe; and, for the
-// device-driven loop, its state block.
+// collect's minimum slots, at their identity in every stripe; both sets of
+// scan words clear; and the run state.
 __global__ void k_sssp_start(int32_t *dist, int32_t n, uint4 *dirty16, int64_t n16, unsigned long long *ctr,
-                             int32_t src, SsspCtl *ctl, int32_t bound, double bytes0) {
+                             uint64_t *status, int64_t status_words, int32_t src, SsspCtl *ctl, int32_t bound,
+                             double bytes0) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (int64_t i = t0; i < n; i += stride) dist[i] = i == src ? 0 : kInf;
@@ -473,17 +598,25 @@ __global__ void k_sssp_start(int32_t *dist, int32_t n, uint4 *dirty16, int64_t n
     dirty16[i] = make_uint4(here && (sb >> 2) == 0 ? bit : 0u, here && (sb >> 2) == 1 ? bit : 0u,
                             here && (sb >> 2) == 2 ? bit : 0u, here && (sb >> 2) == 3 ? bit : 0u);
   }
-  for (int64_t i = t0; i < kCounterWords; i += stride)
-    ctr[i] = i % kCounterSlots == 17 ? (unsigned long long)INT_MAX : 0ull;
-  if (ctl && t0 == 0) {
+  for (int64_t i = t0; i < kCounterWords; i += stride) {
+    const int slot = (int)(i % kCounterSlots);
+    ctr[i] = slot == kLeftSlot + 1 || slot == kLeftSlot + 3 ? (unsigned long long)INT_MAX : 0ull;
+  }
+  for (int64_t i = t0; i < status_words; i += stride) status[i] = 0ull;
+  if (t0 == 0) {
     ctl->bound = bound;
     ctl->push_size = 0;
     ctl->tier_size = 0;
     ctl->pull_skip = 1;
     ctl->done = 0;
+    ctl->advance = 0;
+    ctl->epoch = 0;
     ctl->iters = 0;
     ctl->pulls = 0;
     ctl->steps = 0;
+    ctl->last_nf = 0;
+    ctl->prev_pull = 0;
+    ctl->last_deg = 0;
     ctl->edges = 0;
     ctl->bytes = bytes0;
   }
@@ -496,6 +629,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   const int32_t n = g->n;
   const int64_t m = g->m;
   hipStream_t s = g->stream;
+  GG_CHECK_ARG(m < ((int64_t)1 << kPackShift), "SSSP needs fewer than 2^31 edges (m = %lld)", (long long)m);
   LbsBuffers lb;
   GG_TRY(lbs_buffers(g, &lb));
   const int64_t n_dirty = ((int64_t)n + kCollectPer - 1) / kCollectPer * kCollectPer;  // zero padding for the collect
@@ -528,33 +662,38 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   int32_t *qsize = reinterpret_cast<int32_t *>(ctr + 8);
   const int64_t delta = g->opt.sssp_delta > 0 ? g->opt.sssp_delta : 128;
   g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
-  LbsPhaseTimer lbs_timer(g->prof, "sssp", "push");
+
+  // The collect's shape: tiles of kCollectStep vertices, at most
+  // kCollectGridMax workgroups, each over a run of consecutive tiles.
+  const int32_t tiles = (int32_t)std::max<int64_t>(1, ((int64_t)n + kCollectStep - 1) / kCollectStep);
+  const int32_t per_block = (tiles + kCollectGridMax - 1) / kCollectGridMax;
+  const int32_t collect_grid = (tiles + per_block - 1) / per_block;
+  GG_TRY(g->scan_words.ensure(2 * sizeof(uint64_t) * (size_t)collect_grid));
+  GG_TRY(g->loop_ctl.ensure(sizeof(SsspCtl)));
+  SsspCtl *ctl = g->loop_ctl.as<SsspCtl>();
 
   hipEvent_t e0, e1;
   GG_HIP(g->sssp_ev.get(&e0, &e1));
   GG_HIP(hipEventRecord(e0, s));
-  SsspCtl *ctl = nullptr;  // state block of the device-driven loop
-  if (!g->prof.enabled) {
-    GG_TRY(g->loop_ctl.ensure(sizeof(SsspCtl)));
-    ctl = g->loop_ctl.as<SsspCtl>();
-  }
   {
     TimedRegion t(g->prof, "sssp.init");
     const int64_t n16 = n_dirty / kCollectPer;
     hipLaunchKernelGGL(k_sssp_start, dim3(grid_for(std::max<int64_t>(n, kCounterWords), 256, 8192)), dim3(256), 0, s,
-                       dist, n, reinterpret_cast<uint4 *>(dirty), n16, ctr, src, ctl,
-                       (int32_t)std::min<int64_t>(delta, kInf), 5.0 * n);
+                       dist, n, reinterpret_cast<uint4 *>(dirty), n16, ctr, g->scan_words.as<uint64_t>(),
+                       2 * (int64_t)collect_grid, src, ctl, (int32_t)std::min<int64_t>(delta, kInf), 5.0 * n);
     GG_HIP(hipGetLastError());
   }
   g->prof.add_bytes("sssp.init", 5.0 * n);
-  int64_t bound = delta;
-  int iterations = 0, pulls = 0;
-  int64_t edges = 0, prev_improved = 0;
-  double bytes = 5.0 * n;
-  const int collect_grid = grid_for(n, kCollectStep, 4096);
   // The collect reads dist and the offsets as 16-byte vectors when both arrays allow.
   const bool collect_vec = ((reinterpret_cast<uintptr_t>(dist) | reinterpret_cast<uintptr_t>(g->dg.row_offsets)) & 15) == 0;
   const auto collect = collect_vec ? k_sssp_collect<true> : k_sssp_collect<false>;
+  CollectArgs ca{dirty,       dist,        g->dg.row_offsets, n,        tiles,       per_block,
+                 g->opt.iter_num, queue,   lb.scan_local,     lb.partials, lb.info, ctr,
+                 lb.edge_counter, g->scan_words.as<uint64_t>(), ctl, m,      delta,       pull_alpha,
+                 pull_bytes};
+  auto launch_collect = [&]() {
+    hipLaunchKernelGGL(collect, dim3(collect_grid), dim3(kCollectBlock), 0, s, ca);
+  };
   PullOp pull_op{g->dg.row_indices, static_cast<const int32_t *>(wcv), dist, dirty, ctr, {}};
   PackedPullOp pk_op{pk, pk_bits, pk_bits ? (uint32_t)((1ull << pk_bits) - 1) : 0u, dist, dirty, ctr, {}};
   // One pull launch (packed in-edges when the graph has them).
@@ -572,17 +711,30 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   const bool push_packed = pk && g->dg.row_indices == g->dg.column_indices && wcv == g->dg.edge_values;
   PackedPushOp push_pk{g->dg.row_offsets, g->dg.column_indices, w_out, pk, pk_bits,
                        pk_bits ? (uint32_t)((1ull << pk_bits) - 1) : 0u, dist, dirty, ctr, {}};
-  auto launch_push = [&](const int32_t *fsize, int64_t nf_host, auto &&timer) {
+  // The LBS push over the collect's queue and offsets: partition + expand
+  // (no count or scan launches: the collect wrote the offsets).
+  auto launch_push = [&](auto &&timer) {
+    const int32_t *fsize = &ctl->push_size;
+    auto go = [&](const auto &op) {
+      if (lb.vt == 2)
+        lbs_partition_expand<2>(op, queue, fsize, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(), qsize + 1, s,
+                                timer);
+      else if (lb.vt == 8)
+        lbs_partition_expand<8>(op, queue, fsize, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(), qsize + 1, s,
+                                timer);
+      else
+        lbs_partition_expand<4>(op, queue, fsize, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(), qsize + 1, s,
+                                timer);
+    };
     if (push_packed)
-      lbs_launch(push_pk, false, queue, fsize, nf_host, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(), qsize + 1,
-                 s, timer);
+      go(push_pk);
     else
-      lbs_launch(push_op, false, queue, fsize, nf_host, (int64_t)n + m, lb, g->frontier[1].as<int32_t>(), qsize + 1,
-                 s, timer);
+      go(push_op);
   };
   // The small-frontier push: one tiered launch (at most kTierItems items).
   const int tier_grid = grid_for(kTierItems, kExpandBlock);
-  auto launch_tier = [&](const int32_t *fsize) {
+  auto launch_tier = [&]() {
+    const int32_t *fsize = &ctl->tier_size;
     if (push_packed)
       hipLaunchKernelGGL((k_tiered_expand<PackedPushOp, false>), dim3(tier_grid), dim3(kExpandBlock), 0, s, push_pk,
                          queue, fsize, g->frontier[1].as<int32_t>(), qsize + 1, g->opt.wave_degree,
@@ -592,19 +744,21 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
                          fsize, g->frontier[1].as<int32_t>(), qsize + 1, g->opt.wave_degree, g->opt.block_degree,
                          lb.edge_counter);
   };
-  bool device_tail = false;  // the device-driven loop read the counters and reach statistics itself
+  SsspCtl h{};
   if (!g->prof.enabled) {
-    // Device-driven: a replay of a captured launch
-    // sequence (collect, k_sssp_ctrl (which also resets the collect's
-    // counters), LBS push over ctl->push_size, pull unless ctl->pull_skip);
-    // the host reads the loop state once per replay.  Per-kernel events
-    // (GG_INSTRUMENT) need the host loop below.
+    // Device-driven: a replay of a captured launch sequence (per iteration:
+    // the collect, which also takes the iteration's decisions; the LBS
+    // partition and expand over ctl->push_size; the tiered push over
+    // ctl->tier_size; the pull unless ctl->pull_skip); the host reads the
+    // run state once per replay.  Per-kernel events (GG_INSTRUMENT) need the
+    // host loop below.
     const std::vector<uint64_t> key = {
         (uint64_t)dist, (uint64_t)dirty, (uint64_t)queue, (uint64_t)ctr, (uint64_t)ctl, (uint64_t)wcv,
         (uint64_t)pparts, (uint64_t)lb.scan_local, (uint64_t)lb.partials, (uint64_t)lb.info, (uint64_t)lb.parts,
         (uint64_t)lb.edge_counter, (uint64_t)lb.expand_grid, (uint64_t)lb.vt, (uint64_t)delta,
         (uint64_t)pull_alpha, (uint64_t)g->opt.iter_num, (uint64_t)g->frontier[1].p, (uint64_t)pk,
-        (uint64_t)pk_bits, (uint64_t)g->opt.wave_degree, (uint64_t)g->opt.block_degree};
+        (uint64_t)pk_bits, (uint64_t)g->opt.wave_degree, (uint64_t)g->opt.block_degree,
+        (uint64_t)g->scan_words.p, (uint64_t)collect_grid};
     // The replay of `iters` iterations, captured on first use for this graph
     // and these buffers / options.
     auto loop_exec = [&](int iters, hipGraphExec_t *out) -> int {
@@ -614,12 +768,9 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
         hipGraph_t graph = nullptr;
         GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (int it = 0; it < iters; it++) {
-          hipLaunchKernelGGL(collect, dim3(collect_grid), dim3(kCollectBlock), 0, s, dirty, dist, g->dg.row_offsets, n,
-                             0, queue, qsize, ctr, (const SsspCtl *)ctl);
-          hipLaunchKernelGGL(k_sssp_ctrl, dim3(1), dim3(1), 0, s, ctl, ctr, n, m, delta, pull_alpha, g->opt.iter_num,
-                             pull_bytes);
-          launch_push(&ctl->push_size, n, [](int) {});
-          launch_tier(&ctl->tier_size);
+          launch_collect();
+          launch_push([](int) {});
+          launch_tier();
           if (pull_alpha > 0) launch_pull((const int32_t *)&ctl->pull_skip);
         }
         const hipError_t le = hipGetLastError();
@@ -641,9 +792,9 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     // graph took (a repeated query then needs one replay and no idle
     // iterations); later replays run kLoopIters.
     int iters = g->sssp_steps_hint > 0 ? std::min(g->sssp_steps_hint, kSsspReplayMax) : kLoopIters;
-    // After each replay, one synchronisation reads the loop state, the
-    // counters and (when asked for) the reach statistics, enqueued on the
-    // chance that the replay ends the run (otherwise they are dropped).
+    // After each replay, one synchronisation reads the run state and (when
+    // asked for) the reach statistics, enqueued on the chance that the replay
+    // ends the run (otherwise they are dropped).
     for (;;) {
       hipGraphExec_t exec = nullptr;
       GG_TRY(loop_exec(iters, &exec));
@@ -652,98 +803,74 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       if (stats) GG_TRY(reach_launch(g, dist, kInf));
       GG_TRY(counters_to_host_async(g, 34, 1ull << 17, ctl, sizeof(SsspCtl)));
       GG_HIP(hipStreamSynchronize(s));
-      SsspCtl h;
       std::memcpy(&h, g->h64() + kCounterSlots, sizeof(h));
       if (h.done) {
-        iterations = h.iters;
-        pulls = h.pulls;
-        edges = h.edges;
-        bytes = h.bytes;
         g->sssp_steps_hint = h.steps;
-        device_tail = true;
         break;
       }
       iters = kLoopIters;
     }
-  }
-  while (g->prof.enabled && iterations < g->opt.iter_num) {
-    GG_HIP(slot_resets_async(ctr, {{8, 0ull}, {9, 0ull}, {16, 0ull}, {17, (unsigned long long)INT_MAX}, {18, 0ull}}, s));
-    {
-      TimedRegion t(g->prof, "sssp.collect");
-      hipLaunchKernelGGL(collect, dim3(collect_grid), dim3(kCollectBlock), 0, s, dirty, dist, g->dg.row_offsets, n,
-                         (int32_t)std::min<int64_t>(bound, kInf), queue, qsize, ctr, (const SsspCtl *)nullptr);
-      GG_HIP(hipGetLastError());
-    }
-    GG_TRY(read_counters(g, 20, 1ull << 17));
-    const int64_t nf = reinterpret_cast<int32_t *>(g->h_pinned + 16)[0];
-    const int64_t remain = g->h64()[16], dmin = g->h64()[17], degsum = g->h64()[18];
-    const int64_t improved_so_far = g->h64()[4];
-    const double cb = n + 4.0 * (nf + remain) + 4.0 * nf;
-    g->prof.add_launch_bytes("sssp.collect", cb);
-    bytes += cb;
-    if (nf == 0) {
-      if (remain == 0) break;
-      bound = std::min<int64_t>((dmin / delta + 1) * delta, kInf);
-      continue;
-    }
-    const int64_t N = improved_so_far - prev_improved;  // improvements of the previous step
-    (void)N;
-    prev_improved = improved_so_far;
-    if (pull_alpha > 0 && degsum > m / pull_alpha) {
-      bound = kInf;  // as k_sssp_ctrl
+  } else {
+    // Host-driven, one kernel event each: the collect (with the same
+    // on-device decisions), a read-back of the run state, then the launch it
+    // asks for.
+    auto timer = [&](int) {
+      g->prof.end("sssp.lbs_partition");
+      g->prof.begin("sssp.push");
+    };
+    for (;;) {
       {
-        TimedRegion t(g->prof, "sssp.pull");
-        launch_pull((const int32_t *)nullptr);
+        TimedRegion t(g->prof, "sssp.collect");
+        launch_collect();
         GG_HIP(hipGetLastError());
       }
-      const double pb = pull_bytes;
-      g->prof.add_launch_bytes("sssp.pull", pb);
-      bytes += pb;
-      edges += m;
-      pulls++;
-    } else if (nf <= kTierItems && degsum <= kTierEdges) {
-      // As k_sssp_ctrl: a small frontier is one tiered launch.
-      {
-        TimedRegion t(g->prof, "sssp.push_small");
-        launch_tier(qsize);
+      GG_TRY(counters_to_host_async(g, 0, 0, ctl, sizeof(SsspCtl)));
+      GG_HIP(hipStreamSynchronize(s));
+      std::memcpy(&h, g->h64() + kCounterSlots, sizeof(h));
+      if (h.done) break;
+      g->prof.add_launch_bytes("sssp.collect", n + 16.0 * h.last_nf);
+      const double pb = 16.0 * h.last_nf + 12.0 * (double)h.last_deg;
+      if (!h.pull_skip) {
+        {
+          TimedRegion t(g->prof, "sssp.pull");
+          launch_pull((const int32_t *)nullptr);
+          GG_HIP(hipGetLastError());
+        }
+        g->prof.add_launch_bytes("sssp.pull", pull_bytes);
+      } else if (h.tier_size > 0) {
+        {
+          TimedRegion t(g->prof, "sssp.push_small");
+          launch_tier();
+          GG_HIP(hipGetLastError());
+        }
+        g->prof.add_launch_bytes("sssp.push_small", pb);
+      } else if (h.push_size > 0) {
+        g->prof.begin("sssp.lbs_partition");
+        launch_push(timer);
+        g->prof.end("sssp.push");
         GG_HIP(hipGetLastError());
+        g->prof.add_launch_bytes("sssp.push", pb);
       }
-      const double pb = 16.0 * nf + 12.0 * degsum;
-      g->prof.add_launch_bytes("sssp.push_small", pb);
-      bytes += pb;
-      edges += degsum;
-    } else {
-      // PushOp never enqueues: the output queue stays empty.
-      launch_push(qsize, nf, lbs_timer);
-      GG_HIP(hipGetLastError());
-      const double pb = 16.0 * nf + 12.0 * degsum;
-      g->prof.add_launch_bytes("sssp.lbs_count", 16.0 * nf);
-      g->prof.add_launch_bytes("sssp.push", 12.0 * degsum);
-      bytes += pb;
-      edges += degsum;
     }
-    iterations++;
-  }
-  if (!device_tail) {
     GG_HIP(hipEventRecord(e1, s));
     GG_TRY(read_counters(g, 8));
   }
   const int64_t improved = g->h64()[4];
-  bytes += 8.0 * improved;  // dist + dirty writes of every improvement
+  const double bytes = h.bytes + 8.0 * improved;  // dist + dirty writes of every improvement
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
   finish_profile(g);
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
-    if (device_tail) {
+    if (!g->prof.enabled) {
       stats->reached_vertices = g->h64()[32];
       stats->reached_edges = g->h64()[33];
     } else {
       GG_TRY(reach_stats(g, dist, kInf, stats));
     }
-    stats->iterations = iterations;
-    stats->pull_iterations = pulls;
-    stats->edges_inspected = edges;
+    stats->iterations = h.iters;
+    stats->pull_iterations = h.pulls;
+    stats->edges_inspected = h.edges;
     stats->alg_bytes = bytes;
     stats->gpu_ms = ms;
   }

@@ -69,6 +69,7 @@ def test_sssp_near_far_bucket_widths(gpu_available, delta):
     exp = oracle.sssp(c.row_offsets, c.col_indices, c.values, src)
     with gg.Graph(c, directed=True) as g:
         # pull_alpha 0: push only; 8: default switch; 2^20: pull whenever the frontier has edges
+        # (and the previous iteration did not pull)
         for pull_alpha in (0, 8, 1 << 20):
             g.set_options(sssp_delta=delta, pull_alpha=pull_alpha)
             dist, st = g.sssp(src)
@@ -133,6 +134,29 @@ def test_direction_switch_parameters(gpu_available):
         g.set_options(bfs_alpha=1, bfs_beta=1000000)
         d, st = g.bfs(src)
         np.testing.assert_array_equal(d, exp)
+
+
+@pytest.mark.parametrize("undirected", [True, False])
+def test_bottom_up_every_level(gpu_available, undirected):
+    """Bottom-up from the second level on (alpha 1, beta huge), bit-exact:
+    an Erdos-Renyi graph whose vertex count is no multiple of 64 (partial
+    last chunk and word), and an R-MAT of 2^20 vertices with unreachable ones,
+    whose in-edges reach past the frontier bits kept in LDS (the first 2^19
+    vertices).  From the hub and from vertex 1; device-driven and
+    host-driven loops."""
+    er = gg.gen_erdos_renyi(70001, 1200000, 77)
+    rm = gg.gen_rmat(20, 8, undirected=undirected, seed=78)
+    for c in (er, rm):
+        hub = workloads.max_degree_vertex(c)
+        with gg.Graph(c, directed=not undirected) as g:
+            g.set_options(bfs_alpha=1, bfs_beta=1000000)
+            for src in (hub, 1):
+                exp = oracle.bfs(c.row_offsets, c.col_indices, src)
+                for flags in (0, _abi.GG_INSTRUMENT):
+                    d, st = g.bfs(src, flags=flags)
+                    np.testing.assert_array_equal(d, exp, err_msg=str((c.n, src, flags)))
+                    if src == hub:
+                        assert st["pull_iterations"] >= 1, st
 
 
 def test_device_outputs_and_instrumentation(gpu_available):
