@@ -61,6 +61,7 @@ struct Workspace {
   GasCtl *h_ctl = nullptr;                    // pinned read-back of ctl, sizes and the edge counter
   void *gcarry = nullptr;                     // gather tile carries
   size_t gcarry_bytes = 0;
+  uint64_t *order_words = nullptr;            // k_gas_order's scan words (two sets)
   // Device-driven loops: a captured sequence of iterations, the arguments it
   // was captured with, and the stream it is captured on (launched on the
   // problem's stream, which may be the null stream).
@@ -90,6 +91,7 @@ struct Workspace {
     GAS_RETURN_IF(hipMalloc(&counters, 8 * sizeof(unsigned long long)));
     GAS_RETURN_IF(hipHostMalloc(&h_sizes, 8 * sizeof(int32_t)));
     GAS_RETURN_IF(hipMalloc(&ctl, sizeof(GasCtl)));
+    GAS_RETURN_IF(hipMalloc(&order_words, 2 * kOrderGridMax * sizeof(uint64_t)));
     GAS_RETURN_IF(hipHostMalloc(&h_ctl, 2 * sizeof(GasCtl)));
     capacity = n;
     gather_bytes = gather_elem_bytes * nn;
@@ -138,13 +140,15 @@ struct Workspace {
     ResetLoop();
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     cap_stream = nullptr;
-    void *ps[] = {changed, gather_tmp, scan_local, partials, parts, info, sizes, counters, preds, ctl, gcarry};
+    void *ps[] = {changed, gather_tmp, scan_local, partials, parts, info, sizes, counters, preds, ctl, gcarry,
+                  order_words};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     if (h_sizes) (void)hipHostFree(h_sizes);
     if (h_ctl) (void)hipHostFree(h_ctl);
     ctl = nullptr;
     h_ctl = nullptr;
+    order_words = nullptr;
     gcarry = nullptr;
     gcarry_bytes = 0;
     changed = nullptr;

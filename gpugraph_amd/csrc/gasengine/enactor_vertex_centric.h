@@ -83,6 +83,7 @@ __global__ void k_gas_ctl_init(GasCtl *c, int32_t *sizes, int32_t nf0, int32_t t
   c->iter_num = iter_num;
   c->iterations = 0;
   c->lbs_iters = 0;
+  c->order_epoch = 0;
   c->sum_frontier = 0;
   c->sum_next = 0;
   sizes[0] = nf0;
@@ -101,7 +102,6 @@ __global__ void k_gas_ctl(GasCtl *c, int32_t *sizes, int sel) {
     return;
   }
   sizes[sel ^ 1] = 0;
-  c->order_count = 0;
   c->iter = c->iterations;
   c->iterations++;
   c->sum_frontier += nf;
@@ -112,31 +112,48 @@ __global__ void k_gas_ctl(GasCtl *c, int32_t *sizes, int sel) {
   c->gather_size = nf;
 }
 
-// A large next frontier is rewritten in vertex order from the dedup bitmap
-// that built it (the expand appends it tile by tile, in whatever order the
-// tiles finish): the next iteration then reads its vertex state and rows in
-// runs of consecutive vertices.  Same set, same size; one wave per 64 bitmap
-// words, one atomic per wave.
+// The next frontier listed in vertex order from the dedup bitmap that built
+// it: always after an LBS expansion (whose edges only mark the bitmap, so no
+// workgroup serialises on the queue's size counter), and after a tiered one
+// (which appends, in whatever order its waves finish) when the frontier has
+// at least min_size vertices; the next iteration then reads its vertex
+// state and rows in runs of consecutive vertices.  Each thread counts a run
+// of per_thread consecutive words; queue slots come from a single-pass scan
+// across the workgroups (block_exclusive_across), and the last workgroup
+// writes the size.  Same gating argument as every collect of this engine:
+// the state is read at each workgroup's start and written only by the last
+// workgroup, after every other one has published its count.
 template <int DUMMY = 0>
-__global__ void __launch_bounds__(256)
-k_gas_order(const GasCtl *__restrict__ c, const uint32_t *__restrict__ bits, int32_t words,
-            const int32_t *__restrict__ next_size, int32_t min_size, int32_t *__restrict__ queue,
-            int32_t *__restrict__ counter) {
-  if (c->done || *next_size < min_size) return;  // order_count is reset only for a running iteration
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < words; base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t w = base + threadIdx.x;
-    uint32_t word = w < words ? bits[w] : 0u;
-    const int32_t cnt = __popc(word);
-    const int32_t incl = wave_inclusive_scan(cnt);
-    const int32_t total = __shfl(incl, kWave - 1);
-    int32_t start = 0;
-    if (lane_id() == 0 && total) start = atomicAdd(counter, total);
-    start = __shfl(start, 0) + incl - cnt;
+__global__ void __launch_bounds__(kOrderBlock)
+k_gas_order(GasCtl *__restrict__ c, const uint32_t *__restrict__ bits, int32_t words, int32_t per_thread,
+            int32_t *__restrict__ next_size, int32_t min_size, int32_t *__restrict__ queue,
+            uint64_t *__restrict__ status) {
+  if (c->done || (c->lbs_size == 0 && *next_size < min_size)) return;
+  const int par = c->order_epoch & 1;
+  const int G = gridDim.x, b = blockIdx.x;
+  if (threadIdx.x == 0) status[(par ^ 1) * G + b] = 0ull;  // this workgroup's word of the next run
+  __shared__ int32_t s_scan[kOrderBlock / kWave + 1];
+  const int64_t w0 = ((int64_t)b * kOrderBlock + threadIdx.x) * per_thread;
+  int32_t cnt = 0;
+  for (int k = 0; k < per_thread; k++)
+    if (w0 + k < words) cnt += __popc(bits[w0 + k]);
+  int32_t total;
+  int32_t pos = block_exclusive_scan<kOrderBlock>(cnt, s_scan, &total);
+  const bool last = b == G - 1;
+  const uint64_t excl = block_exclusive_across<kOrderBlock>(status + par * G, b, (uint64_t)total, total > 0 || last);
+  pos += (int32_t)excl;
+  for (int k = 0; k < per_thread; k++) {
+    if (w0 + k >= words) break;
+    uint32_t word = bits[w0 + k];
     while (word) {
-      const int b = __builtin_ctz(word);
+      const int bb = __builtin_ctz(word);
       word &= word - 1;
-      queue[start++] = (int32_t)(w * 32 + b);
+      queue[pos++] = (int32_t)((w0 + k) * 32 + bb);
     }
+  }
+  if (last && threadIdx.x == 0) {
+    *next_size = (int32_t)excl + total;
+    c->order_epoch++;
   }
 }
 
@@ -195,6 +212,7 @@ class EnactorVertexCentric {
     const int32_t nf0 = Program::srcVertex() == SINGLE ? num_srcs : n;
     hipLaunchKernelGGL(k_gas_ctl_init<>, dim3(1), dim3(1), 0, s, ws.ctl, ws.sizes, nf0, threshold,
                        iter_num < 0 ? 0 : iter_num);
+    GAS_RETURN_IF(fill_async<uint64_t>(ws.order_words, 0ull, 2 * kOrderGridMax, s));
     if (CsrProblemT::kMarkPredecessors) {
       GAS_RETURN_IF(ws.AllocatePreds());
       GAS_RETURN_IF(fill_async<int32_t>(ws.preds, -1, n, s));
@@ -297,13 +315,19 @@ class EnactorVertexCentric {
     }
     for (int p = 0; p < np; p++) {
       st.in_edges = passes[p];
+      // The LBS expansion only marks the next frontier in dedup_next
+      // (k_gas_order lists it); the tiered one appends it.
+      GasState<Program> st_lbs = st;
+      st_lbs.append = 0;
+      st.append = 1;
+      ProgramOp<Program> op_lbs{st_lbs};
       ProgramOp<Program> op{st};
       const bool first = p == 0;
       if (mode != 2) {
         static const char *names[4] = {"lbs_count", "lbs_scan", "lbs_partition", "lbs_expand"};
         LbsBuffers b{ws.scan_local, ws.partials, ws.info, ws.parts, ws.counters, config.expand_grid,
                      config.expand_vt};
-        lbs_launch(op, first, f_in, &c->lbs_size, nf_host, max_path, b, f_out, sz_out, s, [&](int phase) {
+        lbs_launch(op_lbs, first, f_in, &c->lbs_size, nf_host, max_path, b, f_out, sz_out, s, [&](int phase) {
           if (phase > 0) te(names[phase - 1]);
           if (phase < 4) tb(names[phase]);
         });
@@ -337,10 +361,12 @@ class EnactorVertexCentric {
     }
     if (np > 0) {
       const int32_t words = (n + 31) / 32;
+      const int grid = grid_for(words, kOrderBlock, kOrderGridMax);
+      const int32_t per_thread = (int32_t)(((int64_t)words + (int64_t)grid * kOrderBlock - 1) / ((int64_t)grid * kOrderBlock));
       tb("order");
-      hipLaunchKernelGGL(k_gas_order<>, dim3(grid_for(words, 256, 1024)), dim3(256), 0, s, (const GasCtl *)c,
-                         st.dedup_next, words, (const int32_t *)sz_out, std::max<int32_t>(n / 64, 4096), f_out,
-                         &c->order_count);
+      hipLaunchKernelGGL(k_gas_order<>, dim3(grid), dim3(kOrderBlock), 0, s, c, st.dedup_next, words,
+                         per_thread < 1 ? 1 : per_thread, sz_out, std::max<int32_t>(n / 64, 4096), f_out,
+                         ws.order_words);
       te("order");
     }
   }

@@ -86,6 +86,12 @@ struct has_gather_vertex<P, std::void_t<typename P::gather_vertex>> : std::true_
 
 }  // namespace detail
 
+// k_gas_order: workgroups of kOrderBlock threads, at most kOrderGridMax of
+// them (each thread a run of consecutive bitmap words), so that a
+// workgroup's threads read all its predecessors' counts at once.
+constexpr int kOrderBlock = 512;
+constexpr int kOrderGridMax = kOrderBlock;
+
 // Device-side loop state of a GAS run.  k_gas_ctl, the first kernel of every
 // iteration, decides on the device what the host decides in the reference
 // (enactor_vertex_centric.cuh:3637-3667, 3801-3804): stop when the frontier
@@ -102,7 +108,7 @@ struct GasCtl {
   int32_t threshold, iter_num;
   int32_t iterations;  // iterations run so far
   int32_t lbs_iters;   // of which expanded by LBS
-  int32_t order_count; // slots written by k_gas_order in this iteration
+  int32_t order_epoch; // k_gas_order runs so far (its parity picks the scan words)
   long long sum_frontier, sum_next;
 };
 
@@ -122,6 +128,7 @@ struct GasState {
   int do_post;
   const int32_t *iter_ptr;  // device-driven loops: the iteration number lives here (GasCtl::iter)
   int32_t *preds;           // MARK_PREDECESSORS: discovering vertex per vertex, else NULL
+  int append;               // 0: the next frontier is only marked in dedup_next (k_gas_order lists it)
 
   __device__ __forceinline__ int iter() const { return iter_ptr ? *iter_ptr : iteration; }
 };
@@ -183,6 +190,7 @@ struct ProgramOp {
     if (frontier == -1) return false;
     if (!bit_test_and_set(s.dedup_next, frontier)) return false;
     if (s.preds) s.preds[frontier] = v;  // the winner of the dedup bit discovered it
+    if (!s.append) return false;
     out = frontier;
     return true;
   }
