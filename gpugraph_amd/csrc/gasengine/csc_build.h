@@ -3,16 +3,16 @@
 // Reference: the CSC and its CSC->CSR edge-id permutation are built with a
 // thrust stable sort by column of the (row-sorted) edge list
 // (util.cuh:55-126, called csr_problem.cuh:754-765).  Here: one stable LSD
-// radix sort of (column, edge id) pairs (hipCUB), so within a column the
+// radix sort of (column, edge id) pairs (sort.h), so within a column the
 // source rows stay ascending exactly as the reference's stable sort leaves
 // them; then row ids are gathered through the permutation and the column
-// offsets found by a lower-bound search per vertex.  Device memory: five
-// arrays of m int32 plus the sort's temporary storage, freed on return.
+// offsets found by a lower-bound search per vertex.  Device memory: four
+// arrays of m int32 plus the sort's temporaries, freed on return.
 #pragma once
 
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
+#include "sort.h"
 #include "util.h"
 
 namespace GASengine {
@@ -89,26 +89,18 @@ inline hipError_t device_csr_to_csc(int32_t n, int64_t m, const int32_t *ro, con
                        (int64_t)n + 1);
     return hipGetLastError();
   }
-  int32_t *keys_in = nullptr, *keys_out = nullptr, *ids = nullptr, *rows = nullptr;
-  void *tmp = nullptr;
-  size_t tmp_bytes = 0;
+  int32_t *keys_out = nullptr, *ids = nullptr, *rows = nullptr;
   int end_bit = 1;
   while (end_bit < 31 && ((int64_t)1 << end_bit) < n) end_bit++;
-  hipError_t e = hipMalloc((void **)&keys_in, 4 * (size_t)m);
-  if (e == hipSuccess) e = hipMalloc((void **)&keys_out, 4 * (size_t)m);
+  hipError_t e = hipMalloc((void **)&keys_out, 4 * (size_t)m);
   if (e == hipSuccess) e = hipMalloc((void **)&ids, 4 * (size_t)m);
   if (e == hipSuccess) e = hipMalloc((void **)&rows, 4 * (size_t)m);
-  if (e == hipSuccess) e = hipMemcpyAsync(keys_in, col, 4 * (size_t)m, hipMemcpyDeviceToDevice, s);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_iota<0>, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, ids, m);
     hipLaunchKernelGGL(k_edge_rows<0>, dim3(grid_for((int64_t)n * kWave, 256, 16384)), dim3(256), 0, s, ro, n, rows);
     e = hipGetLastError();
   }
-  if (e == hipSuccess)
-    e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys_in, keys_out, ids, eid, (int)m, 0, end_bit, s);
-  if (e == hipSuccess) e = hipMalloc(&tmp, tmp_bytes);
-  if (e == hipSuccess)
-    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, keys_out, ids, eid, (int)m, 0, end_bit, s);
+  if (e == hipSuccess) e = radix_sort_pairs<int32_t, int32_t>(col, keys_out, ids, eid, m, 0, end_bit, s);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_gather_rows<0>, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, rows, eid, m, ri);
     hipLaunchKernelGGL(k_column_offsets<0>, dim3(grid_for((int64_t)n + 1, 256, 16384)), dim3(256), 0, s, keys_out, m,
@@ -117,7 +109,7 @@ inline hipError_t device_csr_to_csc(int32_t n, int64_t m, const int32_t *ro, con
   }
   const hipError_t sy = hipStreamSynchronize(s);
   if (e == hipSuccess) e = sy;
-  for (void *p : {(void *)keys_in, (void *)keys_out, (void *)ids, (void *)rows, tmp})
+  for (void *p : {(void *)keys_out, (void *)ids, (void *)rows})
     if (p) (void)hipFree(p);
   return e;
 }

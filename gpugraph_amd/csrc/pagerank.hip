@@ -36,7 +36,6 @@
 #include <cstddef>
 #include <cstring>
 
-#include <hipcub/hipcub.hpp>
 
 #include "gasengine/rowreduce.h"
 #include "gasengine/segreduce.h"
@@ -444,12 +443,8 @@ int ensure_slices(gg_graph *g, int dir, SliceLists *L) {
     hipLaunchKernelGGL(k_slice_count<0>, dim3(grid_for((int64_t)n * kWave, 256, 16384)), dim3(256), 0, s, off, nbr, n,
                        cnt.as<int32_t>());
     GG_HIP(hipGetLastError());
-    size_t tmp_bytes = 0;
-    GG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt.as<int32_t>(), g->sl_off[dir].as<int32_t>(),
-                                            (int)(cells + 1), s));
-    GG_TRY(tmp.ensure(tmp_bytes));
-    GG_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, cnt.as<int32_t>(), g->sl_off[dir].as<int32_t>(),
-                                            (int)(cells + 1), s));
+    GG_TRY(tmp.ensure(4 * (size_t)scan_totals_entries(cells + 1)));
+    GG_HIP(exclusive_scan_i32(cnt.as<int32_t>(), g->sl_off[dir].as<int32_t>(), cells + 1, tmp.as<int32_t>(), s));
     hipLaunchKernelGGL(k_slice_fill<0>, dim3(grid_for((int64_t)n * kWave, 256, 16384)), dim3(256), 0, s, off, nbr, n,
                        g->sl_off[dir].as<int32_t>(), g->sl_idx[dir].as<int32_t>(), g->sl_perm[dir].as<int32_t>());
     GG_HIP(hipGetLastError());

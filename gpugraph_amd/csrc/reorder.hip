@@ -20,7 +20,6 @@
 //   emap[k]   old CSR slot of new slot k; rows keep their edges sorted by new
 //             column id
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <climits>
 #include <cstring>
@@ -135,19 +134,16 @@ int build_reorder(gg_graph *g, const int32_t *ro0, const int32_t *col0, const vo
   GG_TRY(g->re_emap.ensure(4 * (size_t)std::max<int64_t>(m, 1)));
   // 1. order: vertices by out-degree descending (stable).
   {
-    DevBuf key_in, key_out, id_in, tmp;
+    DevBuf key_in, key_out, id_in;
     GG_TRY(key_in.ensure(4 * (size_t)n));
     GG_TRY(key_out.ensure(4 * (size_t)n));
     GG_TRY(id_in.ensure(4 * (size_t)n));
     hipLaunchKernelGGL(k_degree_keys, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, ro0, n, key_in.as<uint32_t>(),
                        id_in.as<int32_t>());
     GG_HIP(hipGetLastError());
-    size_t tb = 0;
-    GG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key_in.as<uint32_t>(), key_out.as<uint32_t>(),
-                                              id_in.as<int32_t>(), g->re_order.as<int32_t>(), n, 0, 32, s));
-    GG_TRY(tmp.ensure(tb));
-    GG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, key_in.as<uint32_t>(), key_out.as<uint32_t>(),
-                                              id_in.as<int32_t>(), g->re_order.as<int32_t>(), n, 0, 32, s));
+    GG_HIP((GASengine::radix_sort_pairs<uint32_t, int32_t>(key_in.as<uint32_t>(), key_out.as<uint32_t>(),
+                                                            id_in.as<int32_t>(), g->re_order.as<int32_t>(), n, 0, 32,
+                                                            s)));
     hipLaunchKernelGGL(k_invert, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, g->re_order.as<int32_t>(), n,
                        g->re_new.as<int32_t>());
     GG_HIP(hipGetLastError());
@@ -155,7 +151,7 @@ int build_reorder(gg_graph *g, const int32_t *ro0, const int32_t *col0, const vo
   }
   // 2. renumbered CSR: sort the slots by (new row, new column).
   if (m > 0) {
-    DevBuf keys_in, keys_out, slot_in, tmp;
+    DevBuf keys_in, keys_out, slot_in;
     GG_TRY(keys_in.ensure(8 * (size_t)m));
     GG_TRY(keys_out.ensure(8 * (size_t)m));
     GG_TRY(slot_in.ensure(4 * (size_t)m));
@@ -164,14 +160,9 @@ int build_reorder(gg_graph *g, const int32_t *ro0, const int32_t *col0, const vo
     GG_HIP(hipGetLastError());
     int bits = 1;
     while (bits < 31 && ((int64_t)1 << bits) < n) bits++;
-    size_t tb = 0;
-    GG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys_in.as<uint64_t>(), keys_out.as<uint64_t>(),
-                                              slot_in.as<int32_t>(), g->re_emap.as<int32_t>(), (int)m, 0, 32 + bits,
-                                              s));
-    GG_TRY(tmp.ensure(tb));
-    GG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, keys_in.as<uint64_t>(), keys_out.as<uint64_t>(),
-                                              slot_in.as<int32_t>(), g->re_emap.as<int32_t>(), (int)m, 0, 32 + bits,
-                                              s));
+    GG_HIP((GASengine::radix_sort_pairs<uint64_t, int32_t>(keys_in.as<uint64_t>(), keys_out.as<uint64_t>(),
+                                                            slot_in.as<int32_t>(), g->re_emap.as<int32_t>(), m, 0,
+                                                            32 + bits, s)));
     hipLaunchKernelGGL(k_unpack_cols, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, keys_out.as<uint64_t>(), m,
                        g->col.as<int32_t>());
     hipLaunchKernelGGL(k_rows_from_keys, dim3(grid_for((int64_t)n + 1, 256, 16384)), dim3(256), 0, s,
