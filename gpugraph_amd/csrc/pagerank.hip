@@ -127,7 +127,11 @@ struct PrForwardOp {
   struct State {
     float r_old, io;
   };
-  __device__ __forceinline__ Edge load_edge(int32_t k) { return {src[k], pcsc ? pcsc[k] : 0.0f}; }
+  // The edge streams are read once per sweep: non-temporal loads keep them from
+  // displacing the gathered contributions in L2.
+  __device__ __forceinline__ Edge load_edge(int32_t k) {
+    return {__builtin_nontemporal_load(&src[k]), pcsc ? __builtin_nontemporal_load(&pcsc[k]) : 0.0f};
+  }
   __device__ __forceinline__ float gather(const Edge &e) { return pcsc ? rc[e.u] * e.p : xc[e.u]; }
   __device__ __forceinline__ State vertex_state(int32_t v) { return {rc[v], inv_out[v]}; }
   __device__ __forceinline__ void item_done(int32_t v, float sum, const State &st) {
@@ -183,7 +187,9 @@ struct PrAdjointOp {
   struct State {
     float l_old, io, cu;
   };
-  __device__ __forceinline__ Edge load_edge(int32_t e) { return {dst[e], pcsr ? pcsr[e] : 0.0f}; }
+  __device__ __forceinline__ Edge load_edge(int32_t e) {
+    return {__builtin_nontemporal_load(&dst[e]), pcsr ? __builtin_nontemporal_load(&pcsr[e]) : 0.0f};
+  }
   __device__ __forceinline__ float gather(const Edge &e) {
     const float l = lc[e.v];
     return pcsr ? e.p * l : l;

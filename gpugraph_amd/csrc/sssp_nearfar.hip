@@ -148,7 +148,9 @@ struct PullOp {
   struct Edge {
     int32_t u, w;
   };
-  __device__ __forceinline__ Edge load_edge(int32_t slot) { return {ri[slot], wc[slot]}; }
+  __device__ __forceinline__ Edge load_edge(int32_t slot) {
+    return {__builtin_nontemporal_load(&ri[slot]), __builtin_nontemporal_load(&wc[slot])};
+  }
   __device__ __forceinline__ int32_t gather(const Edge &e) {
     const int64_t c = (int64_t)dist[e.u] + e.w;
     return c < kInf ? (int32_t)c : INT_MAX;
@@ -191,8 +193,10 @@ struct PackedPullOp {
   struct Edge {
     int32_t u, w;
   };
+  // The edge stream is read once per pull: a non-temporal load keeps it from
+  // displacing the gathered distances in L2.
   __device__ __forceinline__ Edge load_edge(int32_t slot) {
-    const uint32_t e = pk[slot];
+    const uint32_t e = __builtin_nontemporal_load(&pk[slot]);
     return {(int32_t)(e >> wbits), (int32_t)(e & wmask)};
   }
   __device__ __forceinline__ int32_t gather(const Edge &e) {
