@@ -657,7 +657,7 @@ static int batch_pass(gg_graph *g, const int32_t *h_src, int32_t ns, int32_t *di
   GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int it = 0; it < kIters; it++) {
     if (ns) {
-      hipLaunchKernelGGL(k_bs_count, dim3(grid_for(n, kCountTile, 2048)), dim3(kCountBlock), 0, s, st,
+      hipLaunchKernelGGL(k_bs_count, dim3(grid_for(n, kCountTile, kCountGridMax)), dim3(kCountBlock), 0, s, st,
                          g->dg.row_offsets, lb.scan_local, lb.partials);
       hipLaunchKernelGGL(k_bs_scan, dim3(1), dim3(kScanBlock), 0, s, st, lb.partials, lb.info);
       hipLaunchKernelGGL(k_bs_partition, dim3(grid_for(((int64_t)n + m) / kLbsTile + 2, 256, 4096)), dim3(256), 0, s,

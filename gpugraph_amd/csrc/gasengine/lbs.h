@@ -41,7 +41,13 @@
 //   using Loads = ...                    what one edge's loads produce
 //   Loads load(int32_t v, int32_t val, int32_t slot)   loads only, no effects
 //   bool  apply(int32_t v, Loads l, int32_t &out)      as edge(), given l
+//   void  emit(bool keep, int32_t out, int32_t *s_out, int32_t *s_nout)
+//         optional: what becomes of apply()'s result (whole-wave call;
+//         default: wave_append to the tile's output list)
 #pragma once
+
+#include <type_traits>
+#include <utility>
 
 #include "util.h"
 
@@ -55,6 +61,14 @@ template <class Op>
 struct StagedEdge<Op, decltype((void)sizeof(typename Op::Loads))> {
   static constexpr bool value = true;
 };
+// An op may route apply()'s results itself: void emit(bool keep, int32_t
+// out, int32_t *s_out, int32_t *s_nout), called by every lane of the wave
+// (the default appends kept values to the tile's output list).
+template <class Op, class = void>
+struct EdgeEmit : std::false_type {};
+template <class Op>
+struct EdgeEmit<Op, std::void_t<decltype(std::declval<Op &>().emit(false, 0, (int32_t *)nullptr, (int32_t *)nullptr))>>
+    : std::true_type {};
 template <class Op, bool = StagedEdge<Op>::value>
 struct StagedEdgeLoads {
   using type = int32_t;
@@ -64,13 +78,16 @@ struct StagedEdgeLoads<Op, true> {
   using type = typename Op::Loads;
 };
 
-constexpr int kCountBlock = 256;
-constexpr int kCountIpt = 4;
+// One item per thread, so a wave's loads and stores of per-vertex arrays (an
+// op's prologue touches several) cover 64 consecutive items, not 64 items
+// four apart.
+constexpr int kCountBlock = 1024;
+constexpr int kCountIpt = 1;
 constexpr int kCountTile = kCountBlock * kCountIpt;  // items per scan partial
 // k_lbs_count strides over its tiles; captured loops launch it at its largest
-// size every iteration, and a gated launch of 2048 workgroups costs less than
-// one of 4096.
-constexpr int kCountGridMax = 2048;
+// size every iteration, and a gated launch costs by the waves it starts:
+// 512 workgroups of 1024 threads fill the GPU's wave slots once.
+constexpr int kCountGridMax = 512;
 constexpr int kScanBlock = 1024;
 constexpr int kExpandBlock = 256;
 constexpr int kExpandVt = 4;  // default 1024-position tiles: ~21 KB LDS, 7 workgroups per CU
@@ -263,7 +280,10 @@ k_lbs_expand(Op op, const int32_t *__restrict__ frontier, const int32_t *__restr
           bool keep = false;
           int32_t out = 0;
           if (k < ne) keep = op.apply(s_vert[sl[r]], pre[r], out);
-          wave_append(keep, out, s_out, &s_nout);
+          if constexpr (EdgeEmit<Op>::value)
+            op.emit(keep, out, s_out, &s_nout);
+          else
+            wave_append(keep, out, s_out, &s_nout);
         }
       }
     } else {

@@ -170,29 +170,64 @@ struct ProgramOp {
 
   __device__ __forceinline__ int32_t item_value(int32_t) { return 0; }
 
-  __device__ __forceinline__ bool edge(int32_t v, int32_t, int32_t slot, int32_t &out) {
+  // Staged form for k_lbs_expand: the neighbour ids of a thread's edges load
+  // together, then each edge runs the functors (expand_edge, contract) once.
+  struct Loads {
     int32_t nb, eid;
-    if (s.in_edges) {
-      nb = s.g.row_indices[slot];
-      eid = s.g.csc_edge_ids ? s.g.csc_edge_ids[slot] : slot;
-    } else {
-      nb = s.g.column_indices[slot];
-      eid = slot;
-    }
+  };
+  __device__ __forceinline__ Loads load(int32_t, int32_t, int32_t slot) {
+    if (s.in_edges) return {s.g.row_indices[slot], s.g.csc_edge_ids ? s.g.csc_edge_ids[slot] : slot};
+    return {s.g.column_indices[slot], slot};
+  }
+
+  __device__ __forceinline__ bool apply(int32_t v, const Loads &l, int32_t &out) {
     int frontier = -1;
     typename Program::MiscType misc = typename Program::MiscType();
     typename Program::expand_edge fe;
     const int it = s.iter();
-    fe(true, it, v, nb, eid, s.vl, s.el, frontier, misc);
+    fe(true, it, v, l.nb, l.eid, s.vl, s.el, frontier, misc);
     if (frontier == -1) return false;
+    // The dedup word loads together with the contract's own loads.  Bits are
+    // only set while the kernel runs, so an early read can only miss a set
+    // bit, and then the atomic decides.
+    const int32_t f0 = frontier;
+    uint32_t word = s.dedup_next[f0 >> 5];
     detail::call_contract<Program>(it, frontier, s.vl, s.el, s.gather_tmp, misc,
                                    reinterpret_cast<unsigned char *>(s.dedup_next));
     if (frontier == -1) return false;
-    if (!bit_test_and_set(s.dedup_next, frontier)) return false;
+    if (frontier != f0) word = s.dedup_next[frontier >> 5];
+    const uint32_t m = 1u << (frontier & 31);
+    if (word & m) return false;
+    if (!s.append && !s.preds) {
+      // Only marked (k_gas_order lists the next frontier from the bitmap):
+      // the caller sets the bit (emit), a whole wave's marks together.
+      out = frontier;
+      return true;
+    }
+    if (atomicOr(&s.dedup_next[frontier >> 5], m) & m) return false;
     if (s.preds) s.preds[frontier] = v;  // the winner of the dedup bit discovered it
     if (!s.append) return false;
     out = frontier;
     return true;
+  }
+
+  // What k_lbs_expand does with apply()'s result, on every lane of the wave.
+  __device__ __forceinline__ void emit(bool keep, int32_t out, int32_t *s_out, int32_t *s_nout) {
+    if (!s.append && !s.preds)
+      wave_mark(keep, out, s.dedup_next);
+    else
+      wave_append(keep, out, s_out, s_nout);
+  }
+
+  __device__ __forceinline__ bool edge(int32_t v, int32_t val, int32_t slot, int32_t &out) {
+    int32_t o = 0;
+    const bool keep = apply(v, load(v, val, slot), o);
+    if (keep && !s.append && !s.preds) {
+      atomicOr(&s.dedup_next[o >> 5], 1u << (o & 31));
+      return false;
+    }
+    out = o;
+    return keep;
   }
 
   __device__ __forceinline__ void finish() {}

@@ -37,6 +37,31 @@ __device__ __forceinline__ void wave_append(bool keep, int32_t value, int32_t *q
   if (keep) queue[base + rank_below(mask)] = value;
 }
 
+// Set bit v of bits for every lane with `want`.  Lanes whose bits share a
+// word combine into one atomicOr by the lowest of them (a word per round, at
+// most kMarkRounds rounds; lanes left after that set their own bits).  The
+// atomics' results are unused, so the wave does not wait for them.  Must be
+// reached by all lanes of the wave.
+constexpr int kMarkRounds = 8;
+__device__ __forceinline__ void wave_mark(bool want, int32_t v, uint32_t *bits) {
+  uint64_t todo = ballot(want);
+  if (todo == 0) return;
+  const int lane = lane_id();
+  const int32_t w = v >> 5;
+  const uint32_t m = 1u << (v & 31);
+  for (int r = 0; r < kMarkRounds && todo; r++) {
+    const int leader = __builtin_ctzll(todo);
+    const int32_t lw = __builtin_amdgcn_readlane(w, leader);
+    const bool peer = ((todo >> lane) & 1ull) && w == lw;
+    uint32_t acc = peer ? m : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc |= __shfl_xor(acc, off);
+    if (lane == leader) atomicOr(&bits[lw], acc);
+    todo &= ~ballot(peer);
+  }
+  if ((todo >> lane) & 1ull) atomicOr(&bits[w], m);
+}
+
 // Wave-wide reductions (all lanes participate).
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
