@@ -122,7 +122,10 @@ __global__ void k_gas_ctl(GasCtl *c, int32_t *sizes, int sel) {
 // across the workgroups (block_exclusive_across), and the last workgroup
 // writes the size.  Same gating argument as every collect of this engine:
 // the state is read at each workgroup's start and written only by the last
-// workgroup, after every other one has published its count.
+// workgroup, after every other one has published its count.  A workgroup's
+// ids are staged in LDS (kOrderStage per pass) and stored from there, so
+// consecutive lanes store consecutive queue slots.
+constexpr int kOrderStage = 8192;
 template <int DUMMY = 0>
 __global__ void __launch_bounds__(kOrderBlock)
 k_gas_order(GasCtl *__restrict__ c, const uint32_t *__restrict__ bits, int32_t words, int32_t per_thread,
@@ -141,15 +144,25 @@ k_gas_order(GasCtl *__restrict__ c, const uint32_t *__restrict__ bits, int32_t w
   int32_t pos = block_exclusive_scan<kOrderBlock>(cnt, s_scan, &total);
   const bool last = b == G - 1;
   const uint64_t excl = block_exclusive_across<kOrderBlock>(status + par * G, b, (uint64_t)total, total > 0 || last);
-  pos += (int32_t)excl;
-  for (int k = 0; k < per_thread; k++) {
-    if (w0 + k >= words) break;
-    uint32_t word = bits[w0 + k];
-    while (word) {
-      const int bb = __builtin_ctz(word);
-      word &= word - 1;
-      queue[pos++] = (int32_t)((w0 + k) * 32 + bb);
+  __shared__ int32_t s_ids[kOrderStage];
+  for (int32_t base = 0; base < total; base += kOrderStage) {  // total is the workgroup's own count
+    const int32_t lim = min(total - base, kOrderStage);
+    if (cnt > 0 && pos < base + lim && pos + cnt > base) {
+      int32_t p = pos;
+      for (int k = 0; k < per_thread && p < base + lim; k++) {
+        if (w0 + k >= words) break;
+        uint32_t word = bits[w0 + k];
+        while (word && p < base + lim) {
+          const int bb = __builtin_ctz(word);
+          word &= word - 1;
+          if (p >= base) s_ids[p - base] = (int32_t)((w0 + k) * 32 + bb);
+          p++;
+        }
+      }
     }
+    __syncthreads();
+    for (int32_t k = threadIdx.x; k < lim; k += kOrderBlock) queue[(int64_t)excl + base + k] = s_ids[k];
+    __syncthreads();
   }
   if (last && threadIdx.x == 0) {
     *next_size = (int32_t)excl + total;
