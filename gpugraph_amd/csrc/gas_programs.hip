@@ -132,6 +132,10 @@ int run_program(gg_graph *g, int num_srcs, int *srcs, typename P::DataType *out,
   enactor.config.block_degree = g->opt.block_degree;
   enactor.config.expand_grid = g->opt.expand_grid;
   enactor.config.expand_vt = g->opt.lbs_tile / GASengine::kExpandBlock;
+  // Under the per-algorithm default threshold, a small frontier holding a
+  // row longer than 4 * block_degree (a hub that one tiered workgroup would
+  // walk alone) is expanded by LBS; an explicit threshold decides alone.
+  enactor.config.tier_max_degree = g->opt.threshold < 0 ? 4 * g->opt.block_degree : 0;
   g->prof.enabled = (flags & GG_INSTRUMENT) != 0;
   PrefixTimer timer(g->prof, Slot<P>::name);
   if (g->prof.enabled) enactor.timer = &timer;

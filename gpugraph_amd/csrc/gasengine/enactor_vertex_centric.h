@@ -59,6 +59,11 @@ struct EnactorConfig {
   int32_t block_degree = 4096;
   int32_t expand_grid = 2048;  // persistent workgroups for k_lbs_expand
   int32_t expand_vt = kExpandVt;  // LBS tile = kExpandBlock * expand_vt positions (2, 4 or 8)
+  // 0: LBS or tiered by frontier size against the threshold, as the
+  // reference chooses.  > 0: a frontier within the threshold that holds a
+  // row longer than this is expanded by LBS too (the tiered kernel gives
+  // such a row to one workgroup).
+  int32_t tier_max_degree = 0;
   bool device_loop = true;     // capture iterations into a hipGraph (off: one host decision per iteration)
 };
 
@@ -75,7 +80,8 @@ struct IterationRecord {
 constexpr int kLoopIters = 8;
 
 template <int DUMMY = 0>
-__global__ void k_gas_ctl_init(GasCtl *c, int32_t *sizes, int32_t nf0, int32_t threshold, int32_t iter_num) {
+__global__ void k_gas_ctl_init(GasCtl *c, int32_t *sizes, int32_t nf0, int32_t threshold, int32_t iter_num,
+                               int32_t tier_max_degree) {
   c->iter = 0;
   c->done = 0;
   c->lbs_size = c->tier_size = c->gather_size = 0;
@@ -84,17 +90,54 @@ __global__ void k_gas_ctl_init(GasCtl *c, int32_t *sizes, int32_t nf0, int32_t t
   c->iterations = 0;
   c->lbs_iters = 0;
   c->order_epoch = 0;
+  c->tier_max_degree = tier_max_degree;
   c->sum_frontier = 0;
   c->sum_next = 0;
   sizes[0] = nf0;
   sizes[1] = 0;
 }
 
-// Start of an iteration whose input frontier is sizes[sel].
+// Start of an iteration whose input frontier is sizes[sel] (listed in
+// frontier).  With tier_max_degree set, a frontier of at most
+// kCtlBlock * kCtlItems vertices within the threshold has its rows' lengths
+// read (offs_a, and offs_b when the program expands both ways; all loads of
+// a thread at once) and goes to LBS if one is longer.  Thread 0 reads the
+// state and writes it; the other threads only read the frontier.
+constexpr int kCtlBlock = 1024;
+constexpr int kCtlItems = 16;
 template <int DUMMY = 0>
-__global__ void k_gas_ctl(GasCtl *c, int32_t *sizes, int sel) {
-  if (c->done) return;
-  const int32_t nf = sizes[sel];
+__global__ void __launch_bounds__(kCtlBlock)
+k_gas_ctl(GasCtl *c, int32_t *sizes, int sel, const int32_t *__restrict__ frontier,
+          const int32_t *__restrict__ offs_a, const int32_t *__restrict__ offs_b) {
+  __shared__ int32_t s_nf, s_limit;
+  if (threadIdx.x == 0) {
+    const int32_t nf = sizes[sel];
+    s_nf = c->done ? -1 : nf;
+    const bool check = !c->done && nf > 0 && nf <= c->threshold && nf <= kCtlBlock * kCtlItems &&
+                       c->iterations < c->iter_num && c->tier_max_degree > 0;
+    s_limit = check ? c->tier_max_degree : 0;
+  }
+  __syncthreads();
+  const int32_t nf = s_nf, limit = s_limit;
+  if (nf < 0) return;
+  int long_row = 0;
+  if (limit > 0) {
+    int32_t v[kCtlItems];
+#pragma unroll
+    for (int k = 0; k < kCtlItems; k++) {
+      const int32_t i = (int32_t)threadIdx.x + k * kCtlBlock;
+      v[k] = i < nf ? frontier[i] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < kCtlItems; k++) {
+      if (v[k] < 0) continue;
+      int32_t d = offs_a[v[k] + 1] - offs_a[v[k]];
+      if (offs_b) d = max(d, offs_b[v[k] + 1] - offs_b[v[k]]);
+      long_row |= d > limit;
+    }
+  }
+  long_row = __syncthreads_or(long_row);
+  if (threadIdx.x != 0) return;
   if (c->iterations > 0) c->sum_next += nf;  // the previous iteration's output
   if (nf == 0 || c->iterations >= c->iter_num) {
     c->done = 1;
@@ -105,7 +148,7 @@ __global__ void k_gas_ctl(GasCtl *c, int32_t *sizes, int sel) {
   c->iter = c->iterations;
   c->iterations++;
   c->sum_frontier += nf;
-  const bool lbs = nf > c->threshold;
+  const bool lbs = nf > c->threshold || long_row;
   c->lbs_iters += lbs ? 1 : 0;
   c->lbs_size = lbs ? nf : 0;
   c->tier_size = lbs ? 0 : nf;
@@ -224,7 +267,7 @@ class EnactorVertexCentric {
     GAS_RETURN_IF(hipGetLastError());
     const int32_t nf0 = Program::srcVertex() == SINGLE ? num_srcs : n;
     hipLaunchKernelGGL(k_gas_ctl_init<>, dim3(1), dim3(1), 0, s, ws.ctl, ws.sizes, nf0, threshold,
-                       iter_num < 0 ? 0 : iter_num);
+                       iter_num < 0 ? 0 : iter_num, config.tier_max_degree);
     GAS_RETURN_IF(fill_async<uint64_t>(ws.order_words, 0ull, 2 * kOrderGridMax, s));
     if (CsrProblemT::kMarkPredecessors) {
       GAS_RETURN_IF(ws.AllocatePreds());
@@ -269,8 +312,20 @@ class EnactorVertexCentric {
   // The kernels of iteration parity `sel`.  mode: 0 both expansions (gated on
   // the device), 1 LBS only, 2 tiered only.  nf_host sizes grids (n when
   // unknown).
+  void launch_gas_ctl(CsrProblemT &problem, int sel, hipStream_t s) {
+    Workspace &ws = problem.ws;
+    const DeviceGraph &g = problem.graph;
+    const ExpandEdges ex = Program::expandOverEdges();
+    const int32_t *offs_a = nullptr, *offs_b = nullptr;
+    if (ex == EXPAND_OUT_EDGES || ex == EXPAND_ALL_EDGES) offs_a = g.row_offsets;
+    if (ex == EXPAND_IN_EDGES || ex == EXPAND_ALL_EDGES) (offs_a ? offs_b : offs_a) = g.column_offsets;
+    if (!offs_a) offs_a = g.row_offsets;  // no expansion: the check is off anyway (no long-row case)
+    hipLaunchKernelGGL(k_gas_ctl<>, dim3(1), dim3(kCtlBlock), 0, s, ws.ctl, ws.sizes, sel, ws.frontier[sel], offs_a,
+                       offs_b);
+  }
+
   void issue_iteration(CsrProblemT &problem, GasState<Program> st, int sel, int mode, int64_t nf_host, int tgrid,
-                       hipStream_t s) {
+                       hipStream_t s, bool launch_ctl = true) {
     Workspace &ws = problem.ws;
     const DeviceGraph &g = problem.graph;
     const int32_t n = g.nodes;
@@ -279,7 +334,7 @@ class EnactorVertexCentric {
     int32_t *sz_out = ws.sizes + (sel ^ 1);
     st.dedup_next = ws.dedup[sel];
     st.dedup_prev = ws.dedup[sel ^ 1];
-    hipLaunchKernelGGL(k_gas_ctl<>, dim3(1), dim3(1), 0, s, c, ws.sizes, sel);
+    if (launch_ctl) launch_gas_ctl(problem, sel, s);
     const int64_t max_path = (int64_t)n + g.edges;
     const ExpandEdges ex = Program::expandOverEdges();
     const GatherEdges ga = Program::gatherOverEdges();
@@ -412,8 +467,11 @@ class EnactorVertexCentric {
     int32_t nf = snap.sizes[0];
     for (int it = 0; it < iter_num && nf > 0; it++) {
       const int sel = it & 1;
-      const bool lbs = nf > threshold;
-      issue_iteration(problem, st, sel, lbs ? 1 : 2, nf, 0, s);
+      launch_gas_ctl(problem, sel, s);  // its choice of expansion is read back
+      GAS_RETURN_IF(hipGetLastError());
+      GAS_RETURN_IF(snapshot(ws, s, &snap));
+      const bool lbs = snap.c.lbs_size > 0;
+      issue_iteration(problem, st, sel, lbs ? 1 : 2, nf, 0, s, false);
       GAS_RETURN_IF(hipGetLastError());
       GAS_RETURN_IF(snapshot(ws, s, &snap));
       const int32_t next = snap.sizes[sel ^ 1];

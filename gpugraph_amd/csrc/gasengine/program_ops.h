@@ -109,6 +109,8 @@ struct GasCtl {
   int32_t iterations;  // iterations run so far
   int32_t lbs_iters;   // of which expanded by LBS
   int32_t order_epoch; // k_gas_order runs so far (its parity picks the scan words)
+  int32_t tier_max_degree;  // > 0: a frontier within the threshold still goes to LBS when one
+                            // of its rows is longer than this (0: the threshold alone decides)
   long long sum_frontier, sum_next;
 };
 

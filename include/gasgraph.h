@@ -76,7 +76,9 @@ typedef struct gg_graph gg_graph;
  * enactor_vertex_centric.cuh:3949-3950) and takes the LBS/dynamic switch from
  * a per-algorithm "threshold" parameter (Algorithms/SSSP/register.h:39). */
 typedef struct gg_options {
-  int32_t threshold;     /* |frontier| above which the scan-based LBS expand is used */
+  int32_t threshold;     /* |frontier| above which the scan-based LBS expand is used; -1: the
+                            algorithm's default, and then a smaller frontier holding a row
+                            longer than 4 * block_degree is expanded by LBS as well */
   int32_t wave_degree;   /* degree >= this: one wave64 per vertex (tiered expand; also the
                             longest in-edge list a single lane scans in bottom-up BFS) */
   int32_t block_degree;  /* degree >= this: whole workgroup per vertex (tiered expand) */
