@@ -189,24 +189,16 @@ struct ProgramOp {
     const int it = s.iter();
     fe(true, it, v, l.nb, l.eid, s.vl, s.el, frontier, misc);
     if (frontier == -1) return false;
-    // The dedup word loads together with the contract's own loads.  Bits are
-    // only set while the kernel runs, so an early read can only miss a set
-    // bit, and then the atomic decides.
-    const int32_t f0 = frontier;
-    uint32_t word = s.dedup_next[f0 >> 5];
     detail::call_contract<Program>(it, frontier, s.vl, s.el, s.gather_tmp, misc,
                                    reinterpret_cast<unsigned char *>(s.dedup_next));
     if (frontier == -1) return false;
-    if (frontier != f0) word = s.dedup_next[frontier >> 5];
-    const uint32_t m = 1u << (frontier & 31);
-    if (word & m) return false;
     if (!s.append && !s.preds) {
       // Only marked (k_gas_order lists the next frontier from the bitmap):
       // the caller sets the bit (emit), a whole wave's marks together.
       out = frontier;
       return true;
     }
-    if (atomicOr(&s.dedup_next[frontier >> 5], m) & m) return false;
+    if (!bit_test_and_set(s.dedup_next, frontier)) return false;
     if (s.preds) s.preds[frontier] = v;  // the winner of the dedup bit discovered it
     if (!s.append) return false;
     out = frontier;
@@ -225,7 +217,7 @@ struct ProgramOp {
     int32_t o = 0;
     const bool keep = apply(v, load(v, val, slot), o);
     if (keep && !s.append && !s.preds) {
-      atomicOr(&s.dedup_next[o >> 5], 1u << (o & 31));
+      bit_test_and_set(s.dedup_next, o);
       return false;
     }
     out = o;

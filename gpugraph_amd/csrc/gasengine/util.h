@@ -37,11 +37,15 @@ __device__ __forceinline__ void wave_append(bool keep, int32_t value, int32_t *q
   if (keep) queue[base + rank_below(mask)] = value;
 }
 
-// Set bit v of bits for every lane with `want`.  Lanes whose bits share a
-// word combine into one atomicOr by the lowest of them (a word per round, at
-// most kMarkRounds rounds; lanes left after that set their own bits).  The
-// atomics' results are unused, so the wave does not wait for them.  Must be
-// reached by all lanes of the wave.
+// Set bit v of bits for every lane with `want`.  Rounds combine the lanes
+// whose bits share a word (the lowest lane's word each round) into one
+// request by that lane; they stop after kMarkRounds, or after two rounds
+// that found a lane alone, since scattered bits gain nothing from combining.
+// Each combined word, and each bit left over, is read first (one load per
+// lane, all issued together) and set with an atomicOr only where bits are
+// missing: bits already set, such as a hub's marked by an earlier wave, cost
+// no atomic.  The atomics' results are unused, so the wave does not wait for
+// them.  Must be reached by all lanes of the wave.
 constexpr int kMarkRounds = 8;
 __device__ __forceinline__ void wave_mark(bool want, int32_t v, uint32_t *bits) {
   uint64_t todo = ballot(want);
@@ -49,17 +53,32 @@ __device__ __forceinline__ void wave_mark(bool want, int32_t v, uint32_t *bits) 
   const int lane = lane_id();
   const int32_t w = v >> 5;
   const uint32_t m = 1u << (v & 31);
+  int32_t my_word = -1;  // the word this lane sets, and its bits
+  uint32_t my_bits = 0u;
+  int alone = 0;
   for (int r = 0; r < kMarkRounds && todo; r++) {
     const int leader = __builtin_ctzll(todo);
     const int32_t lw = __builtin_amdgcn_readlane(w, leader);
     const bool peer = ((todo >> lane) & 1ull) && w == lw;
+    const uint64_t peers = ballot(peer);
+    if (__popcll(peers) == 1 && ++alone == 2) break;  // the leader stays in todo and marks on its own
     uint32_t acc = peer ? m : 0u;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc |= __shfl_xor(acc, off);
-    if (lane == leader) atomicOr(&bits[lw], acc);
-    todo &= ~ballot(peer);
+    if (lane == leader) {
+      my_word = lw;
+      my_bits = acc;
+    }
+    todo &= ~peers;
   }
-  if ((todo >> lane) & 1ull) atomicOr(&bits[w], m);
+  if ((todo >> lane) & 1ull) {
+    my_word = w;
+    my_bits = m;
+  }
+  if (my_word >= 0) {
+    const uint32_t cur = bits[my_word];
+    if ((cur & my_bits) != my_bits) atomicOr(&bits[my_word], my_bits);
+  }
 }
 
 // Wave-wide reductions (all lanes participate).
