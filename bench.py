@@ -515,6 +515,19 @@ def copy_probe(torch):
     print(json.dumps({"copy_probe_bytes": 4 * n}))
 
 
+def lattice_gas_probe(gg, workloads, np):
+    """Only the generic GAS CC program on the 2048^2 lattice (3 runs), for counter passes."""
+    lat = workloads.config3_lattice(2048)
+    with gg.Graph(lat) as gl:
+        ref, _ = gl.cc()
+        runs = []
+        for _ in range(3):
+            lab, st = gl.cc(gas=True)
+            runs.append(st["gpu_ms"])
+        print(json.dumps({"lattice_gas_cc_ms": runs, "iterations": st["iterations"],
+                          "equal_to_labelprop": bool(np.array_equal(lab, ref))}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -528,6 +541,8 @@ def main():
                     help='fixed options as JSON, e.g. \'{"pull_alpha": 2}\' (no gg_autotune; for profiling runs)')
     ap.add_argument("--no-reorder", action="store_true", help="keep the R-MAT graphs in generated vertex order")
     ap.add_argument("--copy-probe", action="store_true", help="run only a 1 GiB device copy (PMC calibration)")
+    ap.add_argument("--lattice-gas-probe", action="store_true",
+                    help="run only the generic GAS CC on the 2048^2 lattice (for counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path.",
@@ -543,6 +558,10 @@ def main():
 
     import gpugraph_amd as gg
     from gpugraph_amd import _abi, workloads
+
+    if args.lattice_gas_probe:
+        lattice_gas_probe(gg, workloads, np)
+        return
 
     value, wall, roofline, kernels, per_alg, cpu, config = headline(args, gg, _abi, workloads, torch, np)
     cpu_all = cpu.pop("parallel", None) if cpu else None
