@@ -686,11 +686,15 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
     for (;;) {
       GG_HIP(hipGraphLaunch(g->bfs_loop.exec, s));
       GG_HIP(hipEventRecord(e1, s));  // the last replay's record times the run
+      // The caller's hand-off, on the chance that this replay ends the run
+      // (else it is enqueued again after the next one).
+      if (flags & GG_OUT_DEVICE) GG_TRY(enqueue_handoff(g, depth));
       GG_TRY(counters_to_host_async(g, 0, 0, ctl, sizeof(BfsCtl)));
       GG_HIP(hipStreamSynchronize(s));
       BfsCtl h;
       std::memcpy(&h, g->h64() + kCounterSlots, sizeof(h));
       if (h.done) {
+        if (flags & GG_OUT_DEVICE) g->handoff.done = g->handoff.armed;
         levels = h.level;
         pull_levels = h.pull_levels;
         reached = h.reached;

@@ -257,11 +257,19 @@ int cc_core(gg_graph *g, int32_t *label, uint32_t flags, gg_stats *stats) {
 }
 
 // A renumbered graph runs a core into re_out[0] and maps the result back.
+// The mapping is armed as the core's hand-off: a device-driven core enqueues
+// it from its own result buffer ahead of its last read-back (one
+// synchronisation for both); otherwise it runs here.
 template <class Core>
 int with_mapped_output(gg_graph *g, void *out, uint32_t flags, Core core) {
   GG_TRY(activate(g));
   GG_TRY(g->re_out[0].ensure(4 * (size_t)g->n));
-  GG_TRY(core(g->re_out[0].as<int32_t>(), flags | GG_OUT_DEVICE));
+  g->handoff = {out, flags, out != nullptr, false};
+  const int rc = core(g->re_out[0].as<int32_t>(), flags | GG_OUT_DEVICE);
+  const bool done = g->handoff.done;
+  g->handoff = {};
+  GG_TRY(rc);
+  if (done) return GG_OK;
   return reorder_out_vertices(g, g->re_out[0].p, out, flags);
 }
 

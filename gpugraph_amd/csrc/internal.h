@@ -232,6 +232,15 @@ struct gg_graph {
   bool reordered = false;
   gg::DevBuf re_order, re_new, re_emap, re_out[2], re_in[2], re_stage;
   std::vector<int32_t> h_new_id;
+  // Result hand-off to the caller, armed by a renumbered query: a
+  // device-driven core enqueues it (enqueue_handoff) ahead of its last
+  // read-back, so the mapping shares that synchronisation; `done` says the
+  // caller's buffer holds the result.
+  struct Handoff {
+    void *dst = nullptr;
+    uint32_t flags = 0;
+    bool armed = false, done = false;
+  } handoff;
   // Degree bins of the vertices for row reductions: [0] over CSC in-edges,
   // [1] over CSR out-edges; lists of small / mid / large rows.
   gg::DevBuf bins[2][3];
@@ -338,6 +347,9 @@ int csc_values(gg_graph *g, const void **vals);
 int build_reorder(gg_graph *g, const int32_t *ro0, const int32_t *col0, const void *val0);
 int32_t map_vertex(const gg_graph *g, int32_t v);
 int reorder_out_vertices(gg_graph *g, const void *in, void *dst, uint32_t flags);
+// Enqueues the armed hand-off of `result` (internal order) on g->stream, no
+// synchronisation; a no-op when none is armed.
+int enqueue_handoff(gg_graph *g, const void *result);
 int reorder_in_vertices(gg_graph *g, const void *src, uint32_t flags, DevBuf &buf, const void **out);
 int reorder_in_edges(gg_graph *g, const void *src, uint32_t flags, void *dst_dev);
 int reorder_out_edges(gg_graph *g, const void *in, void *dst, uint32_t flags);

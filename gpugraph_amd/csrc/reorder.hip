@@ -188,10 +188,9 @@ int build_reorder(gg_graph *g, const int32_t *ro0, const int32_t *col0, const vo
 
 int32_t map_vertex(const gg_graph *g, int32_t v) { return g->reordered ? g->h_new_id[(size_t)v] : v; }
 
-// 4-byte vertex results in internal order (device) -> caller order (host or
-// device per flags).
-int reorder_out_vertices(gg_graph *g, const void *in, void *dst, uint32_t flags) {
-  if (!dst) return GG_OK;
+namespace {
+// The mapping of reorder_out_vertices, enqueued only.
+int enqueue_out_vertices(gg_graph *g, const void *in, void *dst, uint32_t flags) {
   const int32_t n = g->n;
   hipStream_t s = g->stream;
   uint32_t *out = static_cast<uint32_t *>(dst);
@@ -203,8 +202,22 @@ int reorder_out_vertices(gg_graph *g, const void *in, void *dst, uint32_t flags)
                      static_cast<const uint32_t *>(in), g->re_new.as<int32_t>(), (int64_t)n, out);
   GG_HIP(hipGetLastError());
   if (!(flags & GG_OUT_DEVICE)) GG_HIP(hipMemcpyAsync(dst, out, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
-  GG_HIP(hipStreamSynchronize(s));
   return GG_OK;
+}
+}  // namespace
+
+// 4-byte vertex results in internal order (device) -> caller order (host or
+// device per flags).
+int reorder_out_vertices(gg_graph *g, const void *in, void *dst, uint32_t flags) {
+  if (!dst) return GG_OK;
+  GG_TRY(enqueue_out_vertices(g, in, dst, flags));
+  GG_HIP(hipStreamSynchronize(g->stream));
+  return GG_OK;
+}
+
+int enqueue_handoff(gg_graph *g, const void *result) {
+  if (!g->handoff.armed || !g->handoff.dst) return GG_OK;
+  return enqueue_out_vertices(g, result, g->handoff.dst, g->handoff.flags);
 }
 
 // 4-byte vertex inputs in caller order (host or device per flags) -> internal

@@ -805,11 +805,13 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       GG_HIP(hipGraphLaunch(exec, s));
       GG_HIP(hipEventRecord(e1, s));
       if (stats) GG_TRY(reach_launch(g, dist, kInf));
+      GG_TRY(enqueue_handoff(g, dist));  // the caller's result, from dist itself
       GG_TRY(counters_to_host_async(g, 34, 1ull << 17, ctl, sizeof(SsspCtl)));
       GG_HIP(hipStreamSynchronize(s));
       std::memcpy(&h, g->h64() + kCounterSlots, sizeof(h));
       if (h.done) {
         g->sssp_steps_hint = h.steps;
+        g->handoff.done = g->handoff.armed;
         break;
       }
       iters = kLoopIters;
@@ -879,6 +881,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     stats->gpu_ms = ms;
   }
   if (flags & GG_OUT_DEVICE) return GG_OK;
+  if (g->handoff.done) return GG_OK;  // the caller's buffer was written with the last read-back
   return copy_out(g, dist_out, dist, 4 * (size_t)n, flags);
 }
 
