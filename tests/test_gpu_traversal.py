@@ -518,3 +518,31 @@ def test_gas_path_device_and_host_loops(gpu_available, threshold):
         np.testing.assert_array_equal(res[0][0], res[1][0])
         assert res[0][1]["iterations"] == res[1][1]["iterations"] == 2
         assert res[0][0].max() == 2
+
+
+def test_gas_long_row_goes_to_lbs(gpu_available):
+    """Under the per-algorithm default threshold, a frontier within the
+    threshold that holds a row longer than 4 * block_degree (here the hub
+    source alone) is expanded by LBS instead of one tiered workgroup; an
+    explicit threshold keeps the reference's rule (frontier size alone).
+    Results equal the oracle's either way."""
+    c = gg.gen_rmat(14, 16, undirected=True, seed=28).with_weights("i32", 1, 64, symmetric=True, seed=29)
+    src = workloads.max_degree_vertex(c)
+    assert c.degrees()[src] > 4 * 256
+    exp_s = oracle.sssp(c.row_offsets, c.col_indices, c.values, src)
+    launches = {}
+    with gg.Graph(c) as g:
+        for th in (-1, 10000):
+            g.set_options(threshold=th, wave_degree=64, block_degree=256)
+            for fl in (0, _abi.GG_INSTRUMENT):
+                g.profile_reset()
+                dist, _ = g.sssp(src, flags=fl | _abi.GG_SSSP_GAS)
+                np.testing.assert_array_equal(dist, exp_s)
+            prof = g.profile()
+            launches[th] = {k: v["launches"] for k, v in prof.items()}
+    # Same iterations either way; at least the hub's iteration moves from the
+    # tiered kernel to the LBS one.
+    lbs = {th: launches[th].get("sssp.lbs_expand", 0) for th in launches}
+    tier = {th: launches[th].get("sssp.tiered_expand", 0) for th in launches}
+    assert lbs[-1] + tier[-1] == lbs[10000] + tier[10000]
+    assert lbs[-1] > lbs[10000] and tier[-1] < tier[10000]
