@@ -457,7 +457,13 @@ __device__ void sssp_decide(const CollectArgs &a, int32_t bound, int32_t nf, int
     return;
   }
   if (nf == 0) {
-    c->advance = 1;
+    // Nothing selected below the bound.  With the bound at kInf every dirty
+    // vertex was selectable (a dirty vertex has dist < kInf), so the run is
+    // over; otherwise the next collect moves the bound or finds it over.
+    if (bound == kInf)
+      c->done = 1;
+    else
+      c->advance = 1;
     return;
   }
   // Pull when the frontier's edges exceed m / pull_alpha, but never twice in
