@@ -61,6 +61,7 @@ class Options(C.Structure):
         ("pull_alpha", C.c_int32),
         ("expand_grid", C.c_int32),
         ("xcd_slices", C.c_int32),
+        ("gas_expand_grid", C.c_int32),
     ]
 
     def as_dict(self):

@@ -178,14 +178,15 @@ struct Knob {
 int32_t &field_ref(gg_options &o, int field) { return reinterpret_cast<int32_t *>(&o)[field]; }
 
 constexpr int kFieldThreshold = 0, kFieldWave = 1, kFieldBlock = 2, kFieldAlpha = 4, kFieldBeta = 5, kFieldTile = 6,
-              kFieldDelta = 7, kFieldPull = 8, kFieldGrid = 9, kFieldSlices = 10;
-static_assert(sizeof(gg_options) == 11 * sizeof(int32_t), "gg_options layout");
+              kFieldDelta = 7, kFieldPull = 8, kFieldGrid = 9, kFieldSlices = 10, kFieldGasGrid = 11;
+static_assert(sizeof(gg_options) == 12 * sizeof(int32_t), "gg_options layout");
 
 std::vector<Knob> knob_table() {
   const uint32_t lbs_users = GG_TUNE_BFS | GG_TUNE_SSSP | GG_TUNE_CC | GG_TUNE_GAS;
   return {
       {kFieldTile, lbs_users, {512, 1024, 2048}},
-      {kFieldGrid, lbs_users, {512, 1024, 2048, 4096, 8192}},
+      {kFieldGrid, GG_TUNE_BFS | GG_TUNE_SSSP | GG_TUNE_CC, {512, 1024, 2048, 4096, 8192}},
+      {kFieldGasGrid, GG_TUNE_GAS, {2048, 4096, 8192, 16384}},
       {kFieldAlpha, GG_TUNE_BFS, {4, 8, 14, 24, 48}},
       {kFieldBeta, GG_TUNE_BFS, {8, 16, 24, 48, 96}},
       {kFieldWave, GG_TUNE_BFS | GG_TUNE_CC | GG_TUNE_GAS, {32, 64, 128, 256, 512}},

@@ -130,7 +130,7 @@ int run_program(gg_graph *g, int num_srcs, int *srcs, typename P::DataType *out,
   GASengine::EnactorVertexCentric<Problem<P>, P> enactor;
   enactor.config.wave_degree = g->opt.wave_degree;
   enactor.config.block_degree = g->opt.block_degree;
-  enactor.config.expand_grid = g->opt.expand_grid;
+  enactor.config.expand_grid = g->opt.gas_expand_grid;
   enactor.config.expand_vt = g->opt.lbs_tile / GASengine::kExpandBlock;
   // Under the per-algorithm default threshold, a small frontier holding a
   // row longer than 4 * block_degree (a hub that one tiered workgroup would

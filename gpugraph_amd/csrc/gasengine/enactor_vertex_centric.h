@@ -57,7 +57,7 @@ struct KernelTimer {
 struct EnactorConfig {
   int32_t wave_degree = 64;
   int32_t block_degree = 4096;
-  int32_t expand_grid = 2048;  // persistent workgroups for k_lbs_expand
+  int32_t expand_grid = 8192;  // workgroups of k_lbs_expand / k_lbs_gather (strided over the tiles)
   int32_t expand_vt = kExpandVt;  // LBS tile = kExpandBlock * expand_vt positions (2, 4 or 8)
   // 0: LBS or tiered by frontier size against the threshold, as the
   // reference chooses.  > 0: a frontier within the threshold that holds a

@@ -419,6 +419,7 @@ int gg_default_options(gg_options *o) {
   o->sssp_delta = 0;
   o->pull_alpha = 0;
   o->expand_grid = 2048;
+  o->gas_expand_grid = 8192;
   o->xcd_slices = -1;
   return GG_OK;
 }
@@ -552,6 +553,7 @@ int gg_graph_set_options(gg_graph *g, const gg_options *o) {
   GG_CHECK_ARG(o->bfs_alpha >= 1 && o->bfs_beta >= 1, "bfs_alpha/bfs_beta must be >= 1");
   GG_CHECK_ARG(GASengine::lbs_tile_supported(o->lbs_tile), "lbs_tile must be 512, 1024 or 2048");
   GG_CHECK_ARG(o->expand_grid >= 64 && o->expand_grid <= 65536, "expand_grid must be in [64, 65536]");
+  GG_CHECK_ARG(o->gas_expand_grid >= 64 && o->gas_expand_grid <= 65536, "gas_expand_grid must be in [64, 65536]");
   GG_CHECK_ARG(o->xcd_slices >= -1 && o->xcd_slices <= 1, "xcd_slices must be -1, 0 or 1");
   GG_CHECK_ARG(o->sssp_delta >= 0, "sssp_delta must be >= 0");
   GG_CHECK_ARG(o->pull_alpha >= 0, "pull_alpha must be >= 0");

@@ -91,6 +91,9 @@ typedef struct gg_options {
   int32_t expand_grid;   /* persistent workgroups of the LBS expand kernel (64 .. 65536) */
   int32_t xcd_slices;    /* PageRank sweeps split by gathered-vertex slice, one slice per
                             XCD L2: -1 = auto (currently off), 0 = off, 1 = on */
+  int32_t gas_expand_grid; /* workgroups of the generic GAS path's LBS expand and gather
+                              kernels (64 .. 65536): their per-edge work is the program's
+                              functors, longer chains than the specialised kernels' */
 } gg_options;
 
 /* Per-run statistics (the reference prints these with INSTRUMENT/verbose). */

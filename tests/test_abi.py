@@ -34,6 +34,7 @@ def test_default_options_and_version():
     o = gg.default_options()
     assert (o.wave_degree, o.block_degree, o.bfs_alpha, o.bfs_beta) == (128, 1024, 14, 24)
     assert (o.lbs_tile, o.sssp_delta, o.pull_alpha, o.expand_grid, o.xcd_slices) == (1024, 0, 0, 2048, -1)
+    assert o.gas_expand_grid == 8192
     assert o.threshold == -1 and o.iter_num == 2**31 - 1
     assert gg.load_library().gg_version() == 10000
 
