@@ -125,3 +125,31 @@ def test_vertex_order_is_by_degree(gpu_available):
         a, b = C.c_void_p(1), C.c_void_p(1)
         _abi.check(g.lib.gg_graph_vertex_order(g.h, C.byref(a), C.byref(b)), "gg_graph_vertex_order")
         assert a.value is None and b.value is None
+
+
+def test_many_replays_host_and_device_outputs(gpu_available):
+    """Runs longer than one captured replay (a lattice: BFS and SSSP take a
+    few hundred levels / iterations) on a renumbered graph: the mapping to
+    caller ids, enqueued with each replay's read-back, ends with the last
+    replay's result, for host and device outputs, on first and repeated
+    queries."""
+    torch = pytest.importorskip("torch")
+    lat = workloads.config3_lattice(256).with_weights("i32", 1, 64, symmetric=True, seed=7)
+    ro, col, w = lat.row_offsets, lat.col_indices, lat.values
+    src = workloads.max_degree_vertex(lat)
+    exp_d = oracle.bfs(ro, col, src)
+    exp_s = oracle.sssp(ro, col, w, src)
+    assert exp_d.max() > 64  # many more levels than one replay holds
+    with gg.Graph(lat, reorder=True) as g:
+        dev = torch.device("cuda:0")
+        out_d = torch.full((lat.n,), -7, dtype=torch.int32, device=dev)
+        out_s = torch.full((lat.n,), -7, dtype=torch.int32, device=dev)
+        for _ in range(2):
+            d, _ = g.bfs(src)
+            np.testing.assert_array_equal(d, exp_d)
+            s, _ = g.sssp(src)
+            np.testing.assert_array_equal(s, exp_s)
+            g.bfs(src, flags=_abi.GG_OUT_DEVICE, out=out_d)
+            g.sssp(src, flags=_abi.GG_OUT_DEVICE, out=out_s)
+            np.testing.assert_array_equal(out_d.cpu().numpy(), exp_d)
+            np.testing.assert_array_equal(out_s.cpu().numpy(), exp_s)
