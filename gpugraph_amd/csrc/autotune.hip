@@ -192,8 +192,14 @@ std::vector<Knob> knob_table() {
       {kFieldWave, GG_TUNE_BFS | GG_TUNE_CC | GG_TUNE_GAS, {32, 64, 128, 256, 512}},
       {kFieldBlock, GG_TUNE_CC | GG_TUNE_GAS, {512, 1024, 2048, 4096, 16384}},
       {kFieldThreshold, GG_TUNE_CC | GG_TUNE_GAS, {0, 1000, 10000, 100000, INT_MAX}},
-      {kFieldDelta, GG_TUNE_SSSP, {12, 16, 24, 32, 64, 128, 256, 512, 1024}},
-      {kFieldPull, GG_TUNE_SSSP, {0, 2, 4, 8, 16}},
+      // The pull threshold before the bucket width: small widths pay only
+      // with a pull (without one, a run of narrow buckets never gathers a
+      // frontier large enough to trigger it later either).  3 is tried
+      // before 2: where both time alike (within the 3% a change must win
+      // by), the search keeps m / 3, whose trigger sits further from the
+      // frontier sizes of narrow buckets than m / 2 does.
+      {kFieldPull, GG_TUNE_SSSP, {0, 3, 2, 4, 8, 16}},
+      {kFieldDelta, GG_TUNE_SSSP, {3, 4, 6, 8, 12, 16, 24, 32, 64, 128, 256, 512, 1024}},
       {kFieldSlices, GG_TUNE_PAGERANK, {0, 1}},
   };
 }
