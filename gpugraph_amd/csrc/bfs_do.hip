@@ -54,13 +54,32 @@ struct BfsPushOp {
   __device__ __forceinline__ int32_t degree(int32_t v) { return ro[v + 1] - ro[v]; }
   __device__ __forceinline__ int32_t row_begin(int32_t v) { return ro[v]; }
   __device__ __forceinline__ int32_t item_value(int32_t) { return 0; }
-  __device__ __forceinline__ bool edge(int32_t, int32_t, int32_t slot, int32_t &out) {
+  // Staged form for k_lbs_expand: all of a thread's loads for its edges of a
+  // tile go out before any test-and-set: the column ids, then each target's
+  // visited word and row bounds (the bounds are used only if this edge
+  // discovers the target).  Bits are only set during the level, so an early
+  // read of the word can only miss a set bit, and then the atomic decides.
+  struct Loads {
+    int32_t nb;
+    uint32_t word;
+    int32_t r0, r1;
+  };
+  __device__ __forceinline__ Loads load(int32_t, int32_t, int32_t slot) {
     const int32_t nb = col[slot];
-    if (!bit_test_and_set(visited, nb)) return false;
+    return {nb, visited[nb >> 5], ro[nb], ro[nb + 1]};
+  }
+  __device__ __forceinline__ bool apply(int32_t, const Loads &l, int32_t &out) {
+    const int32_t nb = l.nb;
+    const uint32_t m = 1u << (nb & 31);
+    if (l.word & m) return false;
+    if (atomicOr(&visited[nb >> 5], m) & m) return false;
     depth[nb] = level ? *level + 1 : next_depth;
-    mf += (unsigned long long)(ro[nb + 1] - ro[nb]);
+    mf += (unsigned long long)(l.r1 - l.r0);
     out = nb;
     return true;
+  }
+  __device__ __forceinline__ bool edge(int32_t v, int32_t val, int32_t slot, int32_t &out) {
+    return apply(v, load(v, val, slot), out);
   }
   __device__ __forceinline__ void finish() { block_counter_add<kExpandBlock>(&counters[1], mf); }
 };
