@@ -562,6 +562,18 @@ k_pull_reduce(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
 //           whole: every in-edge of v lies in this tile, so no other
 //           workgroup writes v during the launch and `state` is exact
 //   void    finish()
+// Optional row filter (an op that knows which rows cannot change):
+//   int32_t row_limit()                     read once per launch; a row whose
+//                                           prefetched state is <= it has no
+//                                           gathers (its edges still stream)
+//   T       skipped()                       the value combined for them
+// With a filter the gathers go out after the slot map (they need their
+// row's state) instead of before it.
+template <class Op, class = void>
+struct PullRowFilter : std::false_type {};
+template <class Op>
+struct PullRowFilter<Op, std::void_t<decltype(std::declval<const Op &>().row_limit())>> : std::true_type {};
+
 template <class Op, class T>
 __global__ void __launch_bounds__(kExpandBlock)
 k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, const int32_t *__restrict__ parts,
@@ -579,6 +591,8 @@ k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
   const int64_t G = gridDim.x;
   typename Op::Edge ed[VT];
   int32_t ov[Q], sv[Q];
+  int32_t limit = 0;
+  if constexpr (PullRowFilter<Op>::value) limit = op.row_limit();
   // Stage 1 of tile tt with partition bounds p0, p1 into ed / ov / sv.
   auto stage1 = [&](int64_t tt, int32_t p0, int32_t p1) {
     const int64_t e0 = tt * TILE;
@@ -614,11 +628,14 @@ k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
     const int32_t j0 = (int32_t)(d0 - i0), j1 = (int32_t)(d1 - i1);
     const int32_t ni = i1 - i0, ne = j1 - j0;
     // Stage 2 first: the gathers depend only on the edges already in
-    // registers, so they are in flight while the slot map is built.
+    // registers, so they are in flight while the slot map is built (without
+    // a row filter).
     T val[VT];
-    if (ne > 0) {
+    if constexpr (!PullRowFilter<Op>::value) {
+      if (ne > 0) {
 #pragma unroll
-      for (int r = 0; r < VT; r++) val[r] = op.gather(ed[r]);
+        for (int r = 0; r < VT; r++) val[r] = op.gather(ed[r]);
+      }
     }
 #pragma unroll
     for (int q = 0; q < Q; q++) {
@@ -655,7 +672,17 @@ k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
       }
     }
     __syncthreads();
-    // Combine the gathered values (issued at the top of the tile).
+    if constexpr (PullRowFilter<Op>::value) {
+      // Gathers of the rows that can still change only.
+      if (ne > 0) {
+#pragma unroll
+        for (int r = 0; r < VT; r++) {
+          const int32_t k = tid + r * kExpandBlock;
+          val[r] = k < ne && s_state[s_slot[k]] > limit ? op.gather(ed[r]) : op.skipped();
+        }
+      }
+    }
+    // Combine the gathered values.
     if (ne > 0) {
 #pragma unroll
       for (int r = 0; r < VT; r++) {

@@ -66,6 +66,9 @@ static_assert(kLoopIters <= kSsspReplayMax, "replay lengths are cached up to kSs
 // Small pushes run as one tiered launch instead of the LBS partition and expand.
 constexpr int32_t kTierItems = 2048;
 constexpr int64_t kTierEdges = 65536;
+// Counter slot of the pull's distance gathers (slot 4: improvements).
+constexpr int kGatheredSlot = 5;
+
 struct Tally {
   unsigned long long improved = 0;
   __device__ __forceinline__ void add() { improved++; }
@@ -144,6 +147,8 @@ struct PullOp {
   uint8_t *dirty;
   unsigned long long *counters;
   Tally tally;
+  const int32_t *settled;  // SsspCtl::settled: rows of vertices with dist <= *settled are skipped
+  unsigned long long gathered = 0;
 
   struct Edge {
     int32_t u, w;
@@ -151,7 +156,10 @@ struct PullOp {
   __device__ __forceinline__ Edge load_edge(int32_t slot) {
     return {__builtin_nontemporal_load(&ri[slot]), __builtin_nontemporal_load(&wc[slot])};
   }
+  __device__ __forceinline__ int32_t row_limit() const { return *settled; }
+  __device__ __forceinline__ int32_t skipped() const { return INT_MAX; }
   __device__ __forceinline__ int32_t gather(const Edge &e) {
+    gathered++;
     const int64_t c = (int64_t)dist[e.u] + e.w;
     return c < kInf ? (int32_t)c : INT_MAX;
   }
@@ -175,7 +183,10 @@ struct PullOp {
     tally.add();
     dirty[v] = 1;
   }
-  __device__ __forceinline__ void finish() { tally.flush(counters); }
+  __device__ __forceinline__ void finish() {
+    tally.flush(counters);
+    block_counter_add<kExpandBlock>(&counters[kGatheredSlot], gathered);
+  }
 };
 
 // The pull over packed in-edges: one 32-bit word per edge holds the source
@@ -189,6 +200,8 @@ struct PackedPullOp {
   uint8_t *dirty;
   unsigned long long *counters;
   Tally tally;
+  const int32_t *settled;  // SsspCtl::settled: rows of vertices with dist <= *settled are skipped
+  unsigned long long gathered = 0;
 
   struct Edge {
     int32_t u, w;
@@ -199,7 +212,10 @@ struct PackedPullOp {
     const uint32_t e = __builtin_nontemporal_load(&pk[slot]);
     return {(int32_t)(e >> wbits), (int32_t)(e & wmask)};
   }
+  __device__ __forceinline__ int32_t row_limit() const { return *settled; }
+  __device__ __forceinline__ int32_t skipped() const { return INT_MAX; }
   __device__ __forceinline__ int32_t gather(const Edge &e) {
+    gathered++;
     const int64_t c = (int64_t)dist[e.u] + e.w;
     return c < kInf ? (int32_t)c : INT_MAX;
   }
@@ -221,7 +237,10 @@ struct PackedPullOp {
     tally.add();
     dirty[v] = 1;
   }
-  __device__ __forceinline__ void finish() { tally.flush(counters); }
+  __device__ __forceinline__ void finish() {
+    tally.flush(counters);
+    block_counter_add<kExpandBlock>(&counters[kGatheredSlot], gathered);
+  }
 };
 
 __global__ void k_pack_in_edges(const int32_t *__restrict__ src, const int32_t *__restrict__ w, int64_t m,
@@ -293,6 +312,8 @@ struct SsspCtl {
   int32_t steps;      // collects that ran (the last one ends the run)
   int32_t last_nf;    // frontier size of the last collect
   int32_t prev_pull;  // the last iteration pulled
+  int32_t settled;    // a pull skips the in-rows of vertices with dist <= settled (-1: none)
+  int32_t pad0;
   long long last_deg; // its out-degree sum
   long long edges;    // edges inspected
   double bytes;       // algorithmic bytes (model in the header comment)
@@ -473,6 +494,11 @@ __device__ void sssp_decide(const CollectArgs &a, int32_t bound, int32_t nf, int
   c->prev_pull = pull;
   if (pull) {
     c->pull_skip = 0;
+    // Every dirty vertex has dist >= bound - delta (the bucket's lower edge:
+    // relaxations from the bucket's vertices cannot go below it, weights
+    // being >= 0), so a vertex at or below that edge cannot improve: its
+    // in-row needs no gathers in this pull.
+    c->settled = bound >= kInf ? -1 : (int32_t)max<int64_t>((int64_t)bound - a.delta, -1);
     c->bound = kInf;  // after a pull, buckets end: every dirty vertex is collected
     c->bytes += a.pull_bytes;
     c->edges += a.m;
@@ -626,6 +652,7 @@ __global__ void k_sssp_start(int32_t *dist, int32_t n, uint4 *dirty16, int64_t n
     ctl->steps = 0;
     ctl->last_nf = 0;
     ctl->prev_pull = 0;
+    ctl->settled = -1;
     ctl->last_deg = 0;
     ctl->edges = 0;
     ctl->bytes = bytes0;
@@ -657,7 +684,10 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   const uint32_t *pk = nullptr;
   int32_t pk_bits = 0;
   if (pull_alpha > 0) GG_TRY(packed_in_edges(g, static_cast<const int32_t *>(wcv), &pk, &pk_bits));
-  const double pull_bytes = 4.0 * n + 12.0 * m;  // per inspected in-edge, as for the push
+  // A pull streams every in-edge (source and weight) and every vertex's
+  // distance; the distance gathers (4 bytes each) are counted as they run
+  // (counter slot kGatheredSlot): rows of settled vertices skip them.
+  const double pull_bytes = 4.0 * n + 8.0 * m;
   int32_t *dist;
   if (flags & GG_OUT_DEVICE) {
     GG_CHECK_ARG(dist_out, "dist is NULL");
@@ -704,8 +734,9 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   auto launch_collect = [&]() {
     hipLaunchKernelGGL(collect, dim3(collect_grid), dim3(kCollectBlock), 0, s, ca);
   };
-  PullOp pull_op{g->dg.row_indices, static_cast<const int32_t *>(wcv), dist, dirty, ctr, {}};
-  PackedPullOp pk_op{pk, pk_bits, pk_bits ? (uint32_t)((1ull << pk_bits) - 1) : 0u, dist, dirty, ctr, {}};
+  PullOp pull_op{g->dg.row_indices, static_cast<const int32_t *>(wcv), dist, dirty, ctr, {}, &ctl->settled};
+  PackedPullOp pk_op{pk,    pk_bits, pk_bits ? (uint32_t)((1ull << pk_bits) - 1) : 0u, dist, dirty, ctr, {},
+                     &ctl->settled};
   // One pull launch (packed in-edges when the graph has them).
   auto launch_pull = [&](const int32_t *skip) {
     if (pk)
@@ -755,6 +786,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
                          lb.edge_counter);
   };
   SsspCtl h{};
+  unsigned long long gathered_seen = 0;  // host-driven runs: pull gathers counted so far
   if (!g->prof.enabled) {
     // Device-driven: a replay of a captured launch sequence (per iteration:
     // the collect, which also takes the iteration's decisions; the LBS
@@ -848,7 +880,11 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
           launch_pull((const int32_t *)nullptr);
           GG_HIP(hipGetLastError());
         }
-        g->prof.add_launch_bytes("sssp.pull", pull_bytes);
+        // This launch's distance gathers (the counter accumulates over the run).
+        GG_TRY(read_counters(g, kGatheredSlot + 1));
+        const unsigned long long gsum = g->h64()[kGatheredSlot];
+        g->prof.add_launch_bytes("sssp.pull", pull_bytes + 4.0 * (double)(gsum - gathered_seen));
+        gathered_seen = gsum;
       } else if (h.tier_size > 0) {
         {
           TimedRegion t(g->prof, "sssp.push_small");
@@ -868,7 +904,9 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     GG_TRY(read_counters(g, 8));
   }
   const int64_t improved = g->h64()[4];
-  const double bytes = h.bytes + 8.0 * improved;  // dist + dirty writes of every improvement
+  const int64_t gathered = g->h64()[kGatheredSlot];
+  // dist + dirty writes of every improvement; the pulls' distance gathers
+  const double bytes = h.bytes + 8.0 * improved + 4.0 * gathered;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
   finish_profile(g);
