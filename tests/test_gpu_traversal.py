@@ -546,3 +546,34 @@ def test_gas_long_row_goes_to_lbs(gpu_available):
     tier = {th: launches[th].get("sssp.tiered_expand", 0) for th in launches}
     assert lbs[-1] + tier[-1] == lbs[10000] + tier[10000]
     assert lbs[-1] > lbs[10000] and tier[-1] < tier[10000]
+
+
+@pytest.mark.parametrize("delta", [1, 3, 16])
+def test_sssp_pull_skips_settled_rows(gpu_available, delta):
+    """A pull skips the gathers of rows whose vertex is at or below the
+    pulling bucket's lower edge (no dirty vertex can improve them, weights
+    being >= 0).  Zero-weight edges included; bit-exact against the oracle
+    in the device- and host-driven loops, and the pull's gathers (its byte
+    count) fewer than one per in-edge once some rows are settled."""
+    c = gg.gen_rmat(14, 16, undirected=True, seed=31)
+    rows = np.repeat(np.arange(c.n), c.degrees())
+    lo, hi = np.minimum(rows, c.col_indices), np.maximum(rows, c.col_indices)
+    w = ((lo * 5 + hi * 3) % 9).astype(np.int32)  # symmetric, 0..8, zeros included
+    src = workloads.max_degree_vertex(c)
+    exp = oracle.sssp(c.row_offsets, c.col_indices, w, src)
+    n, m = c.n, c.m
+    with gg.Graph(gg.CSR(c.row_offsets, c.col_indices, w), reorder=True) as g:
+        for pull_alpha in (3, 1 << 20):
+            g.set_options(sssp_delta=delta, pull_alpha=pull_alpha)
+            for fl in (0, _abi.GG_INSTRUMENT):
+                g.profile_reset()
+                dist, st = g.sssp(src, flags=fl)
+                np.testing.assert_array_equal(dist, exp)
+            pulls = [b for name, _, b in g.profile_launches() if name == "sssp.pull"]
+            assert st["pull_iterations"] == len(pulls)
+            # 4 n + 8 m streamed, plus 4 bytes per gather that ran
+            assert all(4.0 * n + 8.0 * m <= b <= 4.0 * n + 12.0 * m for b in pulls)
+            if pull_alpha == 1 << 20:  # pulls whenever the frontier has edges
+                assert len(pulls) >= 1
+                if delta <= 3:
+                    assert min(pulls) < 4.0 * n + 12.0 * m
