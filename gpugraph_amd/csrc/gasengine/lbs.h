@@ -569,10 +569,22 @@ k_pull_reduce(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
 //   T       skipped()                       the value combined for them
 // With a filter the gathers go out after the slot map (they need their
 // row's state) instead of before it.
+// Optional tile filter:
+//   const int32_t *row_prefix()             prefix count over the vertices of
+//                                           the rows that can change (n + 1
+//                                           entries); a tile whose rows
+//                                           [i0 - 1, i1) hold none is skipped
+//                                           whole, its edges never loaded
+//   void    tile_visited(int32_t edges, int32_t items)   per tile visited,
+//                                           called by one thread
 template <class Op, class = void>
 struct PullRowFilter : std::false_type {};
 template <class Op>
 struct PullRowFilter<Op, std::void_t<decltype(std::declval<const Op &>().row_limit())>> : std::true_type {};
+template <class Op, class = void>
+struct PullTileFilter : std::false_type {};
+template <class Op>
+struct PullTileFilter<Op, std::void_t<decltype(std::declval<const Op &>().row_prefix())>> : std::true_type {};
 
 template <class Op, class T>
 __global__ void __launch_bounds__(kExpandBlock)
@@ -593,8 +605,14 @@ k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
   int32_t ov[Q], sv[Q];
   int32_t limit = 0;
   if constexpr (PullRowFilter<Op>::value) limit = op.row_limit();
-  // Stage 1 of tile tt with partition bounds p0, p1 into ed / ov / sv.
-  auto stage1 = [&](int64_t tt, int32_t p0, int32_t p1) {
+  const int32_t *rows = nullptr;
+  if constexpr (PullTileFilter<Op>::value) rows = op.row_prefix();
+  // Stage 1 of tile tt with partition bounds p0, p1 into ed / ov / sv; false
+  // (nothing loaded) when the tile filter finds no row that can change.
+  auto stage1 = [&](int64_t tt, int32_t p0, int32_t p1) -> bool {
+    if constexpr (PullTileFilter<Op>::value) {
+      if (rows[p1] == rows[p0 > 0 ? p0 - 1 : 0]) return false;
+    }
     const int64_t e0 = tt * TILE;
     const int64_t e1 = e0 + TILE < path ? e0 + TILE : path;
     const int32_t jj0 = (int32_t)(e0 - p0), nne = (int32_t)(e1 - p1) - jj0, nni = p1 - p0;
@@ -609,19 +627,38 @@ k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
       ov[q] = i < 0 ? 0 : offs[i];
       sv[q] = (i < 0 || i >= n) ? 0 : op.vertex_state(i);
     }
+    return true;
   };
   int64_t t = blockIdx.x;
   int32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;  // partition bounds of tiles t and t + G
+  bool act = false;                        // tile t's stage 1 ran (the tile filter kept it)
   if (t * TILE < path) {
     a0 = parts[t];
     a1 = parts[t + 1];
-    stage1(t, a0, a1);
+    act = stage1(t, a0, a1);
   }
   if ((t + G) * TILE < path) {
     b0 = parts[t + G];
     b1 = parts[t + G + 1];
   }
   for (; t * TILE < path; t += G) {
+    if (!act) {  // (uniform) a tile of settled rows only: on to the next one
+      bool act_next = false;
+      int32_t c0 = 0, c1 = 0;
+      if ((t + G) * TILE < path) {
+        act_next = stage1(t + G, b0, b1);
+        if ((t + 2 * G) * TILE < path) {
+          c0 = parts[t + 2 * G];
+          c1 = parts[t + 2 * G + 1];
+        }
+      }
+      a0 = b0;
+      a1 = b1;
+      b0 = c0;
+      b1 = c1;
+      act = act_next;
+      continue;
+    }
     const int64_t d0 = t * TILE;
     const int64_t d1 = d0 + TILE < path ? d0 + TILE : path;
     const int32_t i0 = a0, i1 = a1;
@@ -691,11 +728,15 @@ k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
       }
     }
     __syncthreads();
+    if constexpr (PullTileFilter<Op>::value) {
+      if (tid == 0) op.tile_visited(ne, ni + 1);
+    }
     // Next tile's stage 1 and the bounds of the one after, ahead of this
     // tile's writes.
     int32_t c0 = 0, c1 = 0;
+    bool act_next = false;
     if ((t + G) * TILE < path) {
-      stage1(t + G, b0, b1);
+      act_next = stage1(t + G, b0, b1);
       if ((t + 2 * G) * TILE < path) {
         c0 = parts[t + 2 * G];
         c1 = parts[t + 2 * G + 1];
@@ -712,6 +753,7 @@ k_pull_staged(Op op, const int32_t *__restrict__ offs, int32_t n, int64_t m, con
     a1 = b1;
     b0 = c0;
     b1 = c1;
+    act = act_next;
   }
   op.finish();
 }

@@ -299,6 +299,7 @@ struct gg_graph {
   // Aggregate words of single-pass scans across workgroups (the SSSP
   // collect: two sets, by collect parity).
   gg::DevBuf scan_words;
+  gg::DevBuf pull_rows;  // SSSP pull: prefix count of the vertices whose in-rows can still change
 
   gg::EventProfiler prof;
 

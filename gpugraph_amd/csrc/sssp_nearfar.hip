@@ -66,8 +66,11 @@ static_assert(kLoopIters <= kSsspReplayMax, "replay lengths are cached up to kSs
 // Small pushes run as one tiered launch instead of the LBS partition and expand.
 constexpr int32_t kTierItems = 2048;
 constexpr int64_t kTierEdges = 65536;
-// Counter slot of the pull's distance gathers (slot 4: improvements).
+// Counter slots of the pull: distance gathers, then the in-edges and rows of
+// the tiles it visited (slot 4: improvements).
 constexpr int kGatheredSlot = 5;
+constexpr int kStreamedEdgesSlot = 6;
+constexpr int kStreamedRowsSlot = 7;
 
 struct Tally {
   unsigned long long improved = 0;
@@ -148,13 +151,19 @@ struct PullOp {
   unsigned long long *counters;
   Tally tally;
   const int32_t *settled;  // SsspCtl::settled: rows of vertices with dist <= *settled are skipped
-  unsigned long long gathered = 0;
+  const int32_t *rows;     // k_pull_rows' prefix count of the rows that can still change
+  unsigned long long gathered = 0, streamed_edges = 0, streamed_rows = 0;
 
   struct Edge {
     int32_t u, w;
   };
   __device__ __forceinline__ Edge load_edge(int32_t slot) {
     return {__builtin_nontemporal_load(&ri[slot]), __builtin_nontemporal_load(&wc[slot])};
+  }
+  __device__ __forceinline__ const int32_t *row_prefix() const { return rows; }
+  __device__ __forceinline__ void tile_visited(int32_t edges, int32_t items) {
+    streamed_edges += (unsigned long long)edges;
+    streamed_rows += (unsigned long long)items;
   }
   __device__ __forceinline__ int32_t row_limit() const { return *settled; }
   __device__ __forceinline__ int32_t skipped() const { return INT_MAX; }
@@ -186,6 +195,8 @@ struct PullOp {
   __device__ __forceinline__ void finish() {
     tally.flush(counters);
     block_counter_add<kExpandBlock>(&counters[kGatheredSlot], gathered);
+    block_counter_add<kExpandBlock>(&counters[kStreamedEdgesSlot], streamed_edges);
+    block_counter_add<kExpandBlock>(&counters[kStreamedRowsSlot], streamed_rows);
   }
 };
 
@@ -201,11 +212,17 @@ struct PackedPullOp {
   unsigned long long *counters;
   Tally tally;
   const int32_t *settled;  // SsspCtl::settled: rows of vertices with dist <= *settled are skipped
-  unsigned long long gathered = 0;
+  const int32_t *rows;     // k_pull_rows' prefix count of the rows that can still change
+  unsigned long long gathered = 0, streamed_edges = 0, streamed_rows = 0;
 
   struct Edge {
     int32_t u, w;
   };
+  __device__ __forceinline__ const int32_t *row_prefix() const { return rows; }
+  __device__ __forceinline__ void tile_visited(int32_t edges, int32_t items) {
+    streamed_edges += (unsigned long long)edges;
+    streamed_rows += (unsigned long long)items;
+  }
   // The edge stream is read once per pull: a non-temporal load keeps it from
   // displacing the gathered distances in L2.
   __device__ __forceinline__ Edge load_edge(int32_t slot) {
@@ -240,6 +257,8 @@ struct PackedPullOp {
   __device__ __forceinline__ void finish() {
     tally.flush(counters);
     block_counter_add<kExpandBlock>(&counters[kGatheredSlot], gathered);
+    block_counter_add<kExpandBlock>(&counters[kStreamedEdgesSlot], streamed_edges);
+    block_counter_add<kExpandBlock>(&counters[kStreamedRowsSlot], streamed_rows);
   }
 };
 
@@ -500,8 +519,7 @@ __device__ void sssp_decide(const CollectArgs &a, int32_t bound, int32_t nf, int
     // in-row needs no gathers in this pull.
     c->settled = bound >= kInf ? -1 : (int32_t)max<int64_t>((int64_t)bound - a.delta, -1);
     c->bound = kInf;  // after a pull, buckets end: every dirty vertex is collected
-    c->bytes += a.pull_bytes;
-    c->edges += a.m;
+    c->bytes += a.pull_bytes;  // k_pull_rows; the pull's own bytes and edges are counted as it runs
     c->pulls++;
   } else {
     if (nf <= kTierItems && degsum <= kTierEdges) {
@@ -614,6 +632,82 @@ __global__ void __launch_bounds__(kCollectBlock) k_sssp_collect(CollectArgs a) {
   }
 }
 
+// The rows a pull visits, in one launch before it (gated like the pull):
+// rows[v] = number of vertices u < v with dist[u] > ctl->settled, whose
+// in-rows can still improve; rows[n] = their total.  The collect's shape
+// (tiles of kCollectStep vertices, 16 per thread, at most kCollectGridMax
+// workgroups over runs of consecutive tiles) and a single-pass scan across
+// the workgroups on status words of the pull's parity (ctl->pulls, counted
+// when the pull was decided).  The pull skips a merge-path tile whose rows
+// are all settled: rows[i1] == rows[i0 - 1].
+// VEC: dist is 16-byte aligned, so a thread's 16 distances are four 16-byte
+// loads (and its 16 prefix entries four 16-byte stores) when all are in range.
+template <bool VEC>
+__device__ __forceinline__ uint32_t pull_row_bits(const int32_t *__restrict__ dist, int32_t n, int64_t v0,
+                                                  int32_t lim) {
+  uint32_t bits = 0;
+  if (VEC && v0 + kCollectPer <= n) {
+    const int4 *d4 = reinterpret_cast<const int4 *>(dist + v0);
+#pragma unroll
+    for (int q = 0; q < kCollectPer / 4; q++) {
+      const int4 d = d4[q];
+      bits |= (uint32_t)(d.x > lim) << (4 * q) | (uint32_t)(d.y > lim) << (4 * q + 1) |
+              (uint32_t)(d.z > lim) << (4 * q + 2) | (uint32_t)(d.w > lim) << (4 * q + 3);
+    }
+    return bits;
+  }
+#pragma unroll
+  for (int j = 0; j < kCollectPer; j++) {
+    const int64_t v = v0 + j;
+    if (v < n && dist[v] > lim) bits |= 1u << j;
+  }
+  return bits;
+}
+
+template <bool VEC>
+__global__ void __launch_bounds__(kCollectBlock)
+k_pull_rows(const SsspCtl *__restrict__ ctl, const int32_t *__restrict__ dist, int32_t n, int32_t tiles,
+            int32_t tiles_per_block, int32_t *__restrict__ rows, uint64_t *__restrict__ status) {
+  if (ctl->done || ctl->pull_skip) return;
+  const int32_t lim = ctl->settled;
+  const int par = ctl->pulls & 1;
+  const int G = gridDim.x, b = blockIdx.x;
+  if (threadIdx.x == 0) status[(par ^ 1) * G + b] = 0ull;  // this workgroup's word for the next pull
+  __shared__ int32_t s_scan[kCollectBlock / kWave + 1];
+  const int64_t t0 = (int64_t)b * tiles_per_block;
+  const int64_t t1 = min<int64_t>(t0 + tiles_per_block, tiles);
+  uint32_t bits = 0;
+  int32_t sum = 0;
+  for (int64_t t = t0; t < t1; t++) {
+    bits = pull_row_bits<VEC>(dist, n, t * kCollectStep + (int64_t)threadIdx.x * kCollectPer, lim);
+    sum += __popc(bits);
+  }
+  int32_t total;
+  (void)block_exclusive_scan<kCollectBlock>(sum, s_scan, &total);
+  int32_t carry = (int32_t)block_exclusive_across<kCollectBlock>(status + par * G, b, (uint64_t)total, true);
+  for (int64_t t = t0; t < t1; t++) {
+    const int64_t v0 = t * kCollectStep + (int64_t)threadIdx.x * kCollectPer;
+    if (t1 - t0 > 1) bits = pull_row_bits<VEC>(dist, n, v0, lim);
+    int32_t tile_total;
+    const int32_t p = carry + block_exclusive_scan<kCollectBlock>(__popc(bits), s_scan, &tile_total);
+    if (VEC && v0 + kCollectPer <= n) {
+      int4 *r4 = reinterpret_cast<int4 *>(rows + v0);
+#pragma unroll
+      for (int q = 0; q < kCollectPer / 4; q++) {
+        const int j = 4 * q;
+        r4[q] = make_int4(p + __popc(bits & ((1u << j) - 1u)), p + __popc(bits & ((1u << (j + 1)) - 1u)),
+                          p + __popc(bits & ((1u << (j + 2)) - 1u)), p + __popc(bits & ((1u << (j + 3)) - 1u)));
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < kCollectPer; j++)
+        if (v0 + j < n) rows[v0 + j] = p + __popc(bits & ((1u << j) - 1u));
+    }
+    carry += tile_total;
+  }
+  if (b == G - 1 && threadIdx.x == 0) rows[n] = carry;
+}
+
 // The start of a run in one launch: every element gets its final initial
 // value (no fill then patch, so no ordering between threads is needed):
 // dist = kInf except dist[src] = 0; the dirty bytes (16 per thread, the array
@@ -684,10 +778,12 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   const uint32_t *pk = nullptr;
   int32_t pk_bits = 0;
   if (pull_alpha > 0) GG_TRY(packed_in_edges(g, static_cast<const int32_t *>(wcv), &pk, &pk_bits));
-  // A pull streams every in-edge (source and weight) and every vertex's
-  // distance; the distance gathers (4 bytes each) are counted as they run
-  // (counter slot kGatheredSlot): rows of settled vertices skip them.
-  const double pull_bytes = 4.0 * n + 8.0 * m;
+  // A pull: k_pull_rows reads every distance and writes the row prefix
+  // (8 n); the pull itself is counted as it runs: 8 bytes per in-edge and
+  // per row of the tiles it visits (source and weight; offset and
+  // distance) and 4 per distance gather (counter slots kStreamedEdgesSlot,
+  // kStreamedRowsSlot, kGatheredSlot).
+  const double pull_bytes = 8.0 * n;
   int32_t *dist;
   if (flags & GG_OUT_DEVICE) {
     GG_CHECK_ARG(dist_out, "dist is NULL");
@@ -708,7 +804,9 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   const int32_t tiles = (int32_t)std::max<int64_t>(1, ((int64_t)n + kCollectStep - 1) / kCollectStep);
   const int32_t per_block = (tiles + kCollectGridMax - 1) / kCollectGridMax;
   const int32_t collect_grid = (tiles + per_block - 1) / per_block;
-  GG_TRY(g->scan_words.ensure(2 * sizeof(uint64_t) * (size_t)collect_grid));
+  // Scan words: two sets for the collect, two for k_pull_rows.
+  GG_TRY(g->scan_words.ensure(4 * sizeof(uint64_t) * (size_t)collect_grid));
+  GG_TRY(g->pull_rows.ensure(sizeof(int32_t) * ((size_t)n + 1)));
   GG_TRY(g->loop_ctl.ensure(sizeof(SsspCtl)));
   SsspCtl *ctl = g->loop_ctl.as<SsspCtl>();
 
@@ -720,7 +818,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     const int64_t n16 = n_dirty / kCollectPer;
     hipLaunchKernelGGL(k_sssp_start, dim3(grid_for(std::max<int64_t>(n, kCounterWords), 256, 8192)), dim3(256), 0, s,
                        dist, n, reinterpret_cast<uint4 *>(dirty), n16, ctr, g->scan_words.as<uint64_t>(),
-                       2 * (int64_t)collect_grid, src, ctl, (int32_t)std::min<int64_t>(delta, kInf), 5.0 * n);
+                       4 * (int64_t)collect_grid, src, ctl, (int32_t)std::min<int64_t>(delta, kInf), 5.0 * n);
     GG_HIP(hipGetLastError());
   }
   g->prof.add_bytes("sssp.init", 5.0 * n);
@@ -734,9 +832,18 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   auto launch_collect = [&]() {
     hipLaunchKernelGGL(collect, dim3(collect_grid), dim3(kCollectBlock), 0, s, ca);
   };
-  PullOp pull_op{g->dg.row_indices, static_cast<const int32_t *>(wcv), dist, dirty, ctr, {}, &ctl->settled};
+  int32_t *pull_rows = g->pull_rows.as<int32_t>();
+  PullOp pull_op{g->dg.row_indices, static_cast<const int32_t *>(wcv), dist, dirty, ctr, {}, &ctl->settled,
+                 pull_rows};
   PackedPullOp pk_op{pk,    pk_bits, pk_bits ? (uint32_t)((1ull << pk_bits) - 1) : 0u, dist, dirty, ctr, {},
-                     &ctl->settled};
+                     &ctl->settled, pull_rows};
+  // The rows a pull visits (gated like the pull).
+  const bool rows_vec = ((reinterpret_cast<uintptr_t>(dist) | reinterpret_cast<uintptr_t>(pull_rows)) & 15) == 0;
+  auto launch_pull_rows = [&]() {
+    hipLaunchKernelGGL(rows_vec ? k_pull_rows<true> : k_pull_rows<false>, dim3(collect_grid), dim3(kCollectBlock), 0,
+                       s, ctl, dist, n, tiles, per_block, pull_rows,
+                       g->scan_words.as<uint64_t>() + 2 * (int64_t)collect_grid);
+  };
   // One pull launch (packed in-edges when the graph has them).
   auto launch_pull = [&](const int32_t *skip) {
     if (pk)
@@ -787,6 +894,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   };
   SsspCtl h{};
   unsigned long long gathered_seen = 0;  // host-driven runs: pull gathers counted so far
+  unsigned long long streamed_seen[2] = {0, 0};  // and the pulls' visited edges and rows
   if (!g->prof.enabled) {
     // Device-driven: a replay of a captured launch sequence (per iteration:
     // the collect, which also takes the iteration's decisions; the LBS
@@ -800,7 +908,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
         (uint64_t)lb.edge_counter, (uint64_t)lb.expand_grid, (uint64_t)lb.vt, (uint64_t)delta,
         (uint64_t)pull_alpha, (uint64_t)g->opt.iter_num, (uint64_t)g->frontier[1].p, (uint64_t)pk,
         (uint64_t)pk_bits, (uint64_t)g->opt.wave_degree, (uint64_t)g->opt.block_degree,
-        (uint64_t)g->scan_words.p, (uint64_t)collect_grid};
+        (uint64_t)g->scan_words.p, (uint64_t)collect_grid, (uint64_t)pull_rows};
     // The replay of `iters` iterations, captured on first use for this graph
     // and these buffers / options.
     auto loop_exec = [&](int iters, hipGraphExec_t *out) -> int {
@@ -813,7 +921,10 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
           launch_collect();
           launch_push([](int) {});
           launch_tier();
-          if (pull_alpha > 0) launch_pull((const int32_t *)&ctl->pull_skip);
+          if (pull_alpha > 0) {
+            launch_pull_rows();
+            launch_pull((const int32_t *)&ctl->pull_skip);
+          }
         }
         const hipError_t le = hipGetLastError();
         const hipError_t ce = hipStreamEndCapture(s, &graph);
@@ -876,15 +987,26 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       const double pb = 16.0 * h.last_nf + 12.0 * (double)h.last_deg;
       if (!h.pull_skip) {
         {
+          TimedRegion t(g->prof, "sssp.pull_rows");
+          launch_pull_rows();
+          GG_HIP(hipGetLastError());
+        }
+        g->prof.add_launch_bytes("sssp.pull_rows", pull_bytes);
+        {
           TimedRegion t(g->prof, "sssp.pull");
           launch_pull((const int32_t *)nullptr);
           GG_HIP(hipGetLastError());
         }
-        // This launch's distance gathers (the counter accumulates over the run).
-        GG_TRY(read_counters(g, kGatheredSlot + 1));
-        const unsigned long long gsum = g->h64()[kGatheredSlot];
-        g->prof.add_launch_bytes("sssp.pull", pull_bytes + 4.0 * (double)(gsum - gathered_seen));
+        // This launch's share of the pull counters (they accumulate over the run).
+        GG_TRY(read_counters(g, kStreamedRowsSlot + 1));
+        const unsigned long long gsum = g->h64()[kGatheredSlot], esum = g->h64()[kStreamedEdgesSlot],
+                                 rsum = g->h64()[kStreamedRowsSlot];
+        g->prof.add_launch_bytes("sssp.pull", 4.0 * (double)(gsum - gathered_seen) +
+                                                  8.0 * (double)(esum - streamed_seen[0]) +
+                                                  8.0 * (double)(rsum - streamed_seen[1]));
         gathered_seen = gsum;
+        streamed_seen[0] = esum;
+        streamed_seen[1] = rsum;
       } else if (h.tier_size > 0) {
         {
           TimedRegion t(g->prof, "sssp.push_small");
@@ -905,8 +1027,9 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   }
   const int64_t improved = g->h64()[4];
   const int64_t gathered = g->h64()[kGatheredSlot];
-  // dist + dirty writes of every improvement; the pulls' distance gathers
-  const double bytes = h.bytes + 8.0 * improved + 4.0 * gathered;
+  const int64_t pulled_edges = g->h64()[kStreamedEdgesSlot], pulled_rows = g->h64()[kStreamedRowsSlot];
+  // dist + dirty writes of every improvement; the pulls' visited tiles and gathers
+  const double bytes = h.bytes + 8.0 * improved + 4.0 * gathered + 8.0 * pulled_edges + 8.0 * pulled_rows;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
   finish_profile(g);
@@ -920,7 +1043,7 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     }
     stats->iterations = h.iters;
     stats->pull_iterations = h.pulls;
-    stats->edges_inspected = h.edges;
+    stats->edges_inspected = h.edges + pulled_edges;
     stats->alg_bytes = bytes;
     stats->gpu_ms = ms;
   }
