@@ -577,3 +577,27 @@ def test_sssp_pull_skips_settled_rows(gpu_available, delta):
             assert all(0 < b <= 12.0 * m + 8.0 * (n + (n + m) / 512 + 1) for b in pulls)
             if pull_alpha == 1 << 20:  # pulls whenever the frontier has edges
                 assert len(pulls) >= 1
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_sssp_pull_rows_odd_n_and_wide_weights(gpu_available, directed):
+    """The pull's row prefix and tile skip on a vertex count that is not a
+    multiple of 16 (the prefix pass's scalar tail) and with weights too wide
+    to pack with the source id (the two-stream pull), pulls forced early and
+    late: bit-exact against the oracle."""
+    c = gg.gen_erdos_renyi(10007, 80000, seed=41)
+    if directed:  # keep an asymmetric subset of the entries
+        rows = np.repeat(np.arange(c.n), c.degrees())
+        keep = (rows < c.col_indices) | ((rows * 7 + c.col_indices) % 3 == 0)
+        ro = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=c.n))]).astype(np.int32)
+        c = gg.CSR(ro, c.col_indices[keep].astype(np.int32))
+    c = c.with_weights("i32", 0, 1 << 20, symmetric=not directed, seed=42)
+    assert c.n % 16 != 0
+    src = workloads.max_degree_vertex(c)
+    exp = oracle.sssp(c.row_offsets, c.col_indices, c.values, src)
+    with gg.Graph(c, directed=directed) as g:
+        for delta, pull_alpha in ((1 << 14, 1 << 20), (1 << 16, 3), (1 << 18, 2)):
+            g.set_options(sssp_delta=delta, pull_alpha=pull_alpha)
+            for fl in (0, _abi.GG_INSTRUMENT):
+                dist, st = g.sssp(src, flags=fl)
+                np.testing.assert_array_equal(dist, exp)
