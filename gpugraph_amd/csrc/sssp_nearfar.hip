@@ -516,8 +516,10 @@ __device__ void sssp_decide(const CollectArgs &a, int32_t bound, int32_t nf, int
     // Every dirty vertex has dist >= bound - delta (the bucket's lower edge:
     // relaxations from the bucket's vertices cannot go below it, weights
     // being >= 0), so a vertex at or below that edge cannot improve: its
-    // in-row needs no gathers in this pull.
-    c->settled = bound >= kInf ? -1 : (int32_t)max<int64_t>((int64_t)bound - a.delta, -1);
+    // in-row needs no gathers in this pull.  Once the buckets have ended
+    // (bound at kInf) the last such edge still holds: every later
+    // relaxation starts from a dirty vertex at or above it.
+    if (bound < kInf) c->settled = (int32_t)max<int64_t>((int64_t)bound - a.delta, -1);
     c->bound = kInf;  // after a pull, buckets end: every dirty vertex is collected
     c->bytes += a.pull_bytes;  // k_pull_rows; the pull's own bytes and edges are counted as it runs
     c->pulls++;
