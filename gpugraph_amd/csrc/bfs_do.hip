@@ -421,6 +421,18 @@ k_bfs_small(const int32_t *__restrict__ ro, const int32_t *__restrict__ col, int
   }
 }
 
+// Kinds of a level (what its slot in a scheduled replay launches); bit k of
+// a slot's mask allows kind k.
+enum BfsKind : int {
+  kBfsBottomUp = 1,      // bottom-up
+  kBfsBottomUpConv = 2,  // queue -> bitmap conversion, bottom-up
+  kBfsTier = 3,          // small top-down (tiered launch)
+  kBfsTierConv = 4,      // bitmap -> queue conversion, small top-down
+  kBfsLbs = 5,           // top-down by LBS
+  kBfsLbsConv = 6,       // bitmap -> queue conversion, top-down by LBS
+};
+constexpr int kBfsMaskAll = 0x7e;
+
 // State of the device-driven loop.  One controller launch per level closes
 // the previous level (folds its counters, the host loop's bookkeeping),
 // clears the counters for the new one and takes its direction decision; a
@@ -433,8 +445,10 @@ struct BfsCtl {
   int32_t push_size;  // frontier items of a top-down level by LBS, else 0
   int32_t tier_size;  // frontier items of a small top-down level (tiered launch), else 0
   int32_t done, pull_levels, pending;
+  int32_t stall;      // a level's kind was not in its slot of a scheduled replay: it has not started
   long long nf, prev_nf, mf, mu, reached, reached_edges, examined;
   double bytes;
+  uint8_t kinds[kSpecMax];  // kind of each level started (BfsKind), the first kSpecMax levels
 };
 static_assert(sizeof(BfsCtl) % 8 == 0, "read back in 8-byte words");
 
@@ -467,6 +481,7 @@ __global__ void k_bfs_start(int32_t *depth, int32_t n, uint32_t *visited, const 
       c->done = 0;
       c->pull_levels = 0;
       c->pending = 0;
+      c->stall = 0;
       c->nf = 1;
       c->prev_nf = 0;
       c->mf = mf;
@@ -480,40 +495,48 @@ __global__ void k_bfs_start(int32_t *depth, int32_t n, uint32_t *visited, const 
 }
 
 // qsize_cur: the queue the level in flight wrote (its next queue) and the
-// new level reads; qsize_nxt: the new level's output queue.
+// new level reads; qsize_nxt: the new level's output queue.  mask: the level
+// kinds this slot launches kernels for (kBfsMaskAll in the general loop).  A
+// level whose kind is not in the mask is not started: the controller sets
+// `stall`, the rest of the replay does nothing, and the general loop's first
+// controller (which clears `stall`) takes the same decision again.
 __global__ void __launch_bounds__(64)
 k_bfs_ctrl(BfsCtl *c, unsigned long long *__restrict__ ctr, int32_t *__restrict__ qsize_cur,
            int32_t *__restrict__ qsize_nxt, int32_t n, int32_t alpha, int32_t beta, int32_t max_levels,
-           int close_only) {
+           int close_only, int mask) {
   __shared__ int s_go;
   if (threadIdx.x == 0) {
-    if (c->pending && !c->done) {  // close the level in flight
-      const long long E = (long long)stripe_sum(ctr + 2);
-      long long N;
-      if (c->bottom_up) {
-        N = (long long)stripe_sum(ctr + 0);
-        c->bytes += 8.0 * (double)stripe_sum(ctr + 3) + 4.0 * E + n / 8.0 + 8.0 * N;
-        c->pull_levels++;
-      } else {
-        N = *qsize_cur;
-        c->bytes += 12.0 * c->nf + 8.0 * E + 8.0 * N;
+    const bool stalled = c->stall && mask != kBfsMaskAll;
+    if (!stalled) {
+      c->stall = 0;
+      if (c->pending && !c->done) {  // close the level in flight
+        const long long E = (long long)stripe_sum(ctr + 2);
+        long long N;
+        if (c->bottom_up) {
+          N = (long long)stripe_sum(ctr + 0);
+          c->bytes += 8.0 * (double)stripe_sum(ctr + 3) + 4.0 * E + n / 8.0 + 8.0 * N;
+          c->pull_levels++;
+        } else {
+          N = *qsize_cur;
+          c->bytes += 12.0 * c->nf + 8.0 * E + 8.0 * N;
+        }
+        c->examined += E;
+        c->prev_nf = c->nf;
+        c->nf = N;
+        c->mf = (long long)stripe_sum(ctr + 1);
+        c->mu -= c->mf;
+        c->reached += N;
+        c->reached_edges += c->mf;
+        c->level++;
+        if (N == 0 || c->level >= max_levels) c->done = 1;
       }
-      c->examined += E;
-      c->prev_nf = c->nf;
-      c->nf = N;
-      c->mf = (long long)stripe_sum(ctr + 1);
-      c->mu -= c->mf;
-      c->reached += N;
-      c->reached_edges += c->mf;
-      c->level++;
-      if (N == 0 || c->level >= max_levels) c->done = 1;
+      c->pending = 0;
+      c->q2b = c->b2q = 0;
+      c->bu_skip = 1;
+      c->push_size = 0;
+      c->tier_size = 0;
     }
-    c->pending = 0;
-    c->q2b = c->b2q = 0;
-    c->bu_skip = 1;
-    c->push_size = 0;
-    c->tier_size = 0;
-    s_go = !close_only && !c->done;
+    s_go = !stalled && !close_only && !c->done;
   }
   __syncthreads();
   if (!s_go) return;
@@ -524,17 +547,29 @@ k_bfs_ctrl(BfsCtl *c, unsigned long long *__restrict__ ctr, int32_t *__restrict_
   }
   if (threadIdx.x == 0) {
     *qsize_nxt = 0;
-    if (!c->bottom_up && c->mf > c->mu / alpha && c->nf > c->prev_nf) {
-      c->q2b = 1;
-      c->bottom_up = 1;
-    } else if (c->bottom_up && c->nf < n / beta && c->nf < c->prev_nf) {
-      c->b2q = 1;
-      c->bottom_up = 0;
-      *qsize_cur = 0;  // k_bfs_convert appends
+    int bu = c->bottom_up, q2b = 0, b2q = 0;
+    if (!bu && c->mf > c->mu / alpha && c->nf > c->prev_nf) {
+      q2b = 1;
+      bu = 1;
+    } else if (bu && c->nf < n / beta && c->nf < c->prev_nf) {
+      b2q = 1;
+      bu = 0;
     }
-    if (c->bottom_up)
+    const bool tier = !bu && c->nf <= kTierItems && c->mf <= kTierEdges;
+    const int kind = bu ? (q2b ? kBfsBottomUpConv : kBfsBottomUp)
+                        : tier ? (b2q ? kBfsTierConv : kBfsTier) : (b2q ? kBfsLbsConv : kBfsLbs);
+    if (!((mask >> kind) & 1)) {
+      c->stall = 1;  // the flags stay cleared: every kernel of the replay returns
+      return;
+    }
+    if (c->level < kSpecMax) c->kinds[c->level] = (uint8_t)kind;
+    c->bottom_up = bu;
+    c->q2b = q2b;
+    c->b2q = b2q;
+    if (b2q) *qsize_cur = 0;  // k_bfs_convert appends
+    if (bu)
       c->bu_skip = 0;
-    else if (c->nf <= kTierItems && c->mf <= kTierEdges)
+    else if (tier)
       c->tier_size = (int32_t)c->nf;
     else
       c->push_size = (int32_t)c->nf;
@@ -654,56 +689,104 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
   int levels = 0, pull_levels = 0;
   bool timed = false;  // the device-driven loop recorded e1 and synchronised past it
   if (!g->prof.enabled) {
-    // Device-driven: kLoopLevels levels per replay of a captured launch
-    // sequence (an even count, so level parity and the ping-pong buffers
-    // agree across replays).  Per level: k_bfs_ctrl (closes the previous
-    // level, picks the direction), k_bfs_convert, the bottom-up kernel and
-    // the LBS top-down (over ctl->push_size), each running or returning on
-    // the controller's flags; a last k_bfs_ctrl closes the replay's final
-    // level.  Per-kernel events (GG_INSTRUMENT) need the host loop below.
+    // Device-driven: replays of captured launch sequences.  The general loop
+    // has kLoopLevels level slots (an even count, so a replay keeps the
+    // parity of levels and ping-pong buffers; one capture per parity of its
+    // first level).  Per slot: k_bfs_ctrl (closes the previous level, picks
+    // the direction), k_bfs_convert, the bottom-up kernel, the LBS top-down
+    // (over ctl->push_size) and the tiered top-down, each running or
+    // returning on the controller's flags; a last k_bfs_ctrl closes the
+    // replay's final level.  When the previous run on this graph took at
+    // most kSpecMax levels, the first replay is that run's schedule: one slot
+    // per level with only the kernels of the level's kind (a controller that
+    // needs another kind stalls the replay, and the general loop goes on from
+    // that level).  Per-kernel events (GG_INSTRUMENT) need the host loop
+    // below.
     const std::vector<uint64_t> key = {
         (uint64_t)depth, (uint64_t)visited, (uint64_t)fbits[0], (uint64_t)fbits[1], (uint64_t)queue[0],
         (uint64_t)queue[1], (uint64_t)ctr, (uint64_t)ctl, (uint64_t)lb.scan_local, (uint64_t)lb.partials,
         (uint64_t)lb.info, (uint64_t)lb.parts, (uint64_t)lb.edge_counter, (uint64_t)lb.expand_grid, (uint64_t)lb.vt,
         (uint64_t)alpha, (uint64_t)beta, (uint64_t)g->opt.wave_degree, (uint64_t)g->opt.iter_num,
         (uint64_t)g->opt.block_degree};
-    if (!g->bfs_loop.exec || g->bfs_loop.key != key) {
-      g->bfs_loop.reset();
-      hipGraph_t graph = nullptr;
-      GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      const int conv_grid = grid_for((int64_t)n, 256, 2048);  // strides by waves; idle at most levels
-      for (int lv = 0; lv < kLoopLevels; lv++) {
-        const int c = lv & 1, x = c ^ 1;
-        hipLaunchKernelGGL(k_bfs_ctrl, dim3(1), dim3(64), 0, s, ctl, ctr, qsz[c], qsz[x], n, alpha, beta,
-                           g->opt.iter_num, 0);
+    const int conv_grid = grid_for((int64_t)n, 256, 2048);  // strides by waves; idle at most levels
+    // One level slot: slot parity c (its input queue / bitmap), the kinds
+    // it launches for.
+    auto launch_level = [&](int c, int mask) {
+      const int x = c ^ 1;
+      const auto has = [&](int k) { return (mask >> k) & 1; };
+      hipLaunchKernelGGL(k_bfs_ctrl, dim3(1), dim3(64), 0, s, ctl, ctr, qsz[c], qsz[x], n, alpha, beta,
+                         g->opt.iter_num, 0, mask);
+      if (has(kBfsBottomUpConv) || has(kBfsTierConv) || has(kBfsLbsConv))
         hipLaunchKernelGGL(k_bfs_convert, dim3(conv_grid), dim3(256), 0, s, depth, (const int32_t *)&ctl->level,
                            (const int32_t *)&ctl->q2b, (const int32_t *)&ctl->b2q, n, fbits[c], queue[c], qsz[c]);
+      if (has(kBfsBottomUp) || has(kBfsBottomUpConv))
         hipLaunchKernelGGL(k_bfs_bottom_up, dim3(bottom_up_grid(n)), dim3(kBuBlock), 0, s, g->dg.column_offsets,
                            g->dg.row_indices, first4, g->directed ? g->dg.row_offsets : nullptr, depth, visited,
                            fbits[c], fbits[x], n, 0, ctr, (const int32_t *)&ctl->bu_skip, (const int32_t *)&ctl->level);
-        BfsPushOp op{g->dg.row_offsets, g->dg.column_indices, depth, visited, ctr, 0, &ctl->level, 0ull};
+      BfsPushOp op{g->dg.row_offsets, g->dg.column_indices, depth, visited, ctr, 0, &ctl->level, 0ull};
+      if (has(kBfsLbs) || has(kBfsLbsConv))
         lbs_launch(op, false, queue[c], &ctl->push_size, n, (int64_t)n + g->m, lb, queue[x], qsz[x], s, [](int) {});
+      if (has(kBfsTier) || has(kBfsTierConv))
         hipLaunchKernelGGL((k_tiered_expand<BfsPushOp, false>), dim3(grid_for(kTierItems, kExpandBlock)),
                            dim3(kExpandBlock), 0, s, op, queue[c], (const int32_t *)&ctl->tier_size, queue[x], qsz[x],
                            g->opt.wave_degree, g->opt.block_degree, lb.edge_counter);
-      }
-      // close the last level of the replay (kLoopLevels is even: its output queue is qsz[0])
-      hipLaunchKernelGGL(k_bfs_ctrl, dim3(1), dim3(64), 0, s, ctl, ctr, qsz[0], qsz[1], n, alpha, beta,
-                         g->opt.iter_num, 1);
+    };
+    // Captures `masks.size()` level slots starting at parity p, then the
+    // closing controller (close_mask: kBfsMaskAll in the general loop, 0 in
+    // a scheduled one, whose closing controller leaves a stall in place).
+    auto capture = [&](CapturedLoop &L, int p, const std::vector<int> &masks, int close_mask) -> int {
+      L.reset();
+      hipGraph_t graph = nullptr;
+      GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      for (size_t lv = 0; lv < masks.size(); lv++) launch_level((int)((lv + p) & 1), masks[lv]);
+      const int e = (int)((masks.size() + p) & 1);  // parity of the queue the last slot wrote
+      hipLaunchKernelGGL(k_bfs_ctrl, dim3(1), dim3(64), 0, s, ctl, ctr, qsz[e], qsz[e ^ 1], n, alpha, beta,
+                         g->opt.iter_num, 1, close_mask);
       const hipError_t le = hipGetLastError();
       const hipError_t ce = hipStreamEndCapture(s, &graph);
       hipError_t ie = hipErrorUnknown;
-      if (le == hipSuccess && ce == hipSuccess) ie = hipGraphInstantiate(&g->bfs_loop.exec, graph, nullptr, nullptr, 0);
+      if (le == hipSuccess && ce == hipSuccess) ie = hipGraphInstantiate(&L.exec, graph, nullptr, nullptr, 0);
       if (graph) (void)hipGraphDestroy(graph);
       if (ie != hipSuccess) {
-        g->bfs_loop.exec = nullptr;
+        L.exec = nullptr;
         return set_error(GG_ERR_HIP, "BFS loop capture failed: %s / %s / %s", hipGetErrorString(le),
                          hipGetErrorString(ce), hipGetErrorString(ie));
       }
-      g->bfs_loop.key = key;
-    }
+      L.key = key;
+      return GG_OK;
+    };
+    auto general = [&](int p, hipGraphExec_t *out) -> int {
+      CapturedLoop &L = g->bfs_loop[p];
+      if (!L.exec || L.key != key) GG_TRY(capture(L, p, std::vector<int>(kLoopLevels, kBfsMaskAll), kBfsMaskAll));
+      *out = L.exec;
+      return GG_OK;
+    };
+    auto scheduled = [&](hipGraphExec_t *out) -> int {
+      for (SpecLoop &sp : g->bfs_spec)
+        if (sp.loop.exec && sp.loop.key == key && sp.sched == g->bfs_sched) {
+          *out = sp.loop.exec;
+          return GG_OK;
+        }
+      SpecLoop &sp = g->bfs_spec[g->bfs_spec_next];
+      g->bfs_spec_next = (g->bfs_spec_next + 1) % kSpecCache;
+      std::vector<int> masks;
+      for (uint8_t k : g->bfs_sched) masks.push_back(1 << k);
+      sp.sched.clear();
+      GG_TRY(capture(sp.loop, 0, masks, 0));
+      sp.sched = g->bfs_sched;
+      *out = sp.loop.exec;
+      return GG_OK;
+    };
+    bool first = true;
+    int parity = 0;  // parity of the next level to run
     for (;;) {
-      GG_HIP(hipGraphLaunch(g->bfs_loop.exec, s));
+      hipGraphExec_t exec = nullptr;
+      if (first && !g->bfs_sched.empty())
+        GG_TRY(scheduled(&exec));
+      else
+        GG_TRY(general(parity, &exec));
+      first = false;
+      GG_HIP(hipGraphLaunch(exec, s));
       GG_HIP(hipEventRecord(e1, s));  // the last replay's record times the run
       // The caller's hand-off, on the chance that this replay ends the run
       // (else it is enqueued again after the next one).
@@ -722,8 +805,14 @@ int bfs_direction_optimizing(gg_graph *g, int32_t src, int32_t *depth_out, uint3
         bytes = h.bytes;
         nf = 0;
         timed = true;
+        // Levels started: all that ran, the last of which found nothing.
+        if (levels <= kSpecMax)
+          g->bfs_sched.assign(h.kinds, h.kinds + levels);
+        else
+          g->bfs_sched.clear();
         break;
       }
+      parity = h.level & 1;
     }
   } else {
     GG_TRY(read_counters(g, 16));

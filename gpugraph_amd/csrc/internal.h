@@ -127,6 +127,19 @@ struct CapturedLoop {
   }
 };
 
+// A device-driven loop captured for one schedule: the kinds of the
+// iterations (levels) a previous run on the same graph took, each slot
+// holding only the kernels its kind launches.  The loop's controller
+// kernels know each slot's kind; an iteration that needs another kind
+// stalls the replay (nothing more runs in it) and the host continues with
+// the general loop, whose slots hold every kernel.
+constexpr int kSpecMax = 32;    // longest schedule recorded (iterations)
+constexpr int kSpecCache = 4;   // schedules kept captured per graph and algorithm
+struct SpecLoop {
+  CapturedLoop loop;
+  std::vector<uint8_t> sched;
+};
+
 // Two events timing a run, created on first use and kept with the graph
 // (creating and destroying them on every call costs host time per query).
 struct RunEvents {
@@ -292,9 +305,14 @@ struct gg_graph {
   // Device-driven BFS / SSSP loops: captured launch sequences and their
   // device-side loop state.
   gg::RunEvents bfs_ev, sssp_ev;
-  gg::CapturedLoop bfs_loop;
+  gg::CapturedLoop bfs_loop[2];  // general BFS loop, by the parity of its first level
   gg::CapturedLoop sssp_loops[gg::kSsspReplayMax + 1];  // by iterations per replay
+  gg::CapturedLoop sssp_resume;     // the iteration kernels after a stalled SSSP collect
   int sssp_steps_hint = 0;          // loop iterations the last device-driven SSSP took
+  // Schedules of the last device-driven runs and their captured loops.
+  std::vector<uint8_t> bfs_sched, sssp_sched;
+  gg::SpecLoop bfs_spec[gg::kSpecCache], sssp_spec[gg::kSpecCache];
+  int bfs_spec_next = 0, sssp_spec_next = 0;  // slot the next capture replaces
   gg::DevBuf loop_ctl;
   // Aggregate words of single-pass scans across workgroups (the SSSP
   // collect: two sets, by collect parity).

@@ -332,11 +332,24 @@ struct SsspCtl {
   int32_t last_nf;    // frontier size of the last collect
   int32_t prev_pull;  // the last iteration pulled
   int32_t settled;    // a pull skips the in-rows of vertices with dist <= settled (-1: none)
-  int32_t pad0;
+  int32_t stall;      // a scheduled replay's collect decided a kind its slot lacks (SsspKind, else 0)
   long long last_deg; // its out-degree sum
   long long edges;    // edges inspected
   double bytes;       // algorithmic bytes (model in the header comment)
+  uint8_t kinds[kSpecMax];  // kind of each collect's iteration (SsspKind), the first kSpecMax collects
 };
+
+// Kinds of an iteration, as its collect decides (what its slot in a
+// scheduled replay launches); bit k of a slot's mask allows kind k.  A
+// collect that only moves the bound or ends the run is kind 0, which every
+// slot allows.
+enum SsspKind : int {
+  kSsspCollectOnly = 0,
+  kSsspLbs = 1,   // LBS partition + expand
+  kSsspTier = 2,  // tiered push
+  kSsspPull = 3,  // row prefix + pull
+};
+constexpr int32_t kSsspMaskAll = 0xf;
 static_assert(sizeof(SsspCtl) % 8 == 0, "read back in 8-byte words");
 
 // A workgroup takes kCollectStep consecutive vertices per tile, 16 per
@@ -392,6 +405,7 @@ struct CollectArgs {
   SsspCtl *ctl;
   int64_t m, delta, pull_alpha;
   double pull_bytes;
+  int32_t mask;             // kinds this collect's slot launches kernels for (kSsspMaskAll: all)
 };
 
 // The thread's 16 vertices from v0: bit j of the result selects v0 + j
@@ -480,6 +494,7 @@ __device__ __forceinline__ void collect_write(const CollectArgs &a, int64_t v0, 
 // advance, push (LBS or tiered) or pull, end.
 __device__ void sssp_decide(const CollectArgs &a, int32_t bound, int32_t nf, int64_t degsum) {
   SsspCtl *c = a.ctl;
+  if (c->steps < kSpecMax) c->kinds[c->steps] = kSsspCollectOnly;
   c->epoch++;
   c->steps++;
   c->bound = bound;
@@ -536,6 +551,37 @@ __device__ void sssp_decide(const CollectArgs &a, int32_t bound, int32_t nf, int
     c->edges += degsum;
   }
   c->iters++;
+  const int kind = pull ? kSsspPull : c->tier_size > 0 ? kSsspTier : kSsspLbs;
+  if (c->steps <= kSpecMax) c->kinds[c->steps - 1] = (uint8_t)kind;
+  if (!((a.mask >> kind) & 1)) {
+    // The slot lacks this kind's kernels: the iteration is decided and its
+    // frontier collected, but its kernels are held back (every gate closed:
+    // the LBS expand runs over info->path) until k_sssp_resume opens them
+    // outside this replay.
+    c->stall = kind;
+    c->push_size = 0;
+    c->tier_size = 0;
+    c->pull_skip = 1;
+    a.info->edges = 0;
+    a.info->path = 0;
+  }
+}
+
+// After a stalled replay: opens the held-back iteration's gates (the
+// collect's decision, from the frontier size and degree sum it recorded).
+__global__ void k_sssp_resume(SsspCtl *c, LbsInfo *info) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int k = c->stall;
+  c->stall = 0;
+  if (k == kSsspLbs) {
+    c->push_size = c->last_nf;
+    info->edges = (int32_t)c->last_deg;
+    info->path = (int64_t)c->last_nf + c->last_deg;
+  } else if (k == kSsspTier) {
+    c->tier_size = c->last_nf;
+  } else if (k == kSsspPull) {
+    c->pull_skip = 0;
+  }
 }
 
 // The collect, fused with the LBS count and scan and with the controller:
@@ -550,7 +596,7 @@ __device__ void sssp_decide(const CollectArgs &a, int32_t bound, int32_t nf, int
 template <bool VEC>
 __global__ void __launch_bounds__(kCollectBlock) k_sssp_collect(CollectArgs a) {
   SsspCtl *c = a.ctl;
-  if (c->done) return;
+  if (c->done || c->stall) return;
   const int par = c->epoch & 1;
   int32_t bound = c->bound;
   const int64_t G = gridDim.x, b = blockIdx.x;
@@ -560,6 +606,7 @@ __global__ void __launch_bounds__(kCollectBlock) k_sssp_collect(CollectArgs a) {
     const unsigned long long left = stripe_sum(prev);
     if (left == 0) {  // nothing dirty anywhere: the run is over
       if (last && threadIdx.x == 0) {
+        if (c->steps < kSpecMax) c->kinds[c->steps] = kSsspCollectOnly;
         c->steps++;
         c->done = 1;
       }
@@ -749,6 +796,7 @@ __global__ void k_sssp_start(int32_t *dist, int32_t n, uint4 *dirty16, int64_t n
     ctl->last_nf = 0;
     ctl->prev_pull = 0;
     ctl->settled = -1;
+    ctl->stall = 0;
     ctl->last_deg = 0;
     ctl->edges = 0;
     ctl->bytes = bytes0;
@@ -830,9 +878,12 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
   CollectArgs ca{dirty,       dist,        g->dg.row_offsets, n,        tiles,       per_block,
                  g->opt.iter_num, queue,   lb.scan_local,     lb.partials, lb.info, ctr,
                  lb.edge_counter, g->scan_words.as<uint64_t>(), ctl, m,      delta,       pull_alpha,
-                 pull_bytes};
-  auto launch_collect = [&]() {
-    hipLaunchKernelGGL(collect, dim3(collect_grid), dim3(kCollectBlock), 0, s, ca);
+                 pull_bytes,  kSsspMaskAll};
+  // mask: the iteration kinds the launches after this collect cover.
+  auto launch_collect = [&](int32_t mask = kSsspMaskAll) {
+    CollectArgs a = ca;
+    a.mask = mask;
+    hipLaunchKernelGGL(collect, dim3(collect_grid), dim3(kCollectBlock), 0, s, a);
   };
   int32_t *pull_rows = g->pull_rows.as<int32_t>();
   PullOp pull_op{g->dg.row_indices, static_cast<const int32_t *>(wcv), dist, dirty, ctr, {}, &ctl->settled,
@@ -902,8 +953,13 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
     // the collect, which also takes the iteration's decisions; the LBS
     // partition and expand over ctl->push_size; the tiered push over
     // ctl->tier_size; the pull unless ctl->pull_skip); the host reads the
-    // run state once per replay.  Per-kernel events (GG_INSTRUMENT) need the
-    // host loop below.
+    // run state once per replay.  When the previous run on this graph took
+    // at most kSpecMax collects, the first replay is that run's schedule:
+    // each collect followed only by the kernels of the kind it took then.  A
+    // collect that decides another kind stalls the replay (its kernels held
+    // back, later collects idle); k_sssp_resume and that iteration's kernels
+    // then run ahead of the general loop.  Per-kernel events (GG_INSTRUMENT)
+    // need the host loop below.
     const std::vector<uint64_t> key = {
         (uint64_t)dist, (uint64_t)dirty, (uint64_t)queue, (uint64_t)ctr, (uint64_t)ctl, (uint64_t)wcv,
         (uint64_t)pparts, (uint64_t)lb.scan_local, (uint64_t)lb.partials, (uint64_t)lb.info, (uint64_t)lb.parts,
@@ -911,49 +967,104 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
         (uint64_t)pull_alpha, (uint64_t)g->opt.iter_num, (uint64_t)g->frontier[1].p, (uint64_t)pk,
         (uint64_t)pk_bits, (uint64_t)g->opt.wave_degree, (uint64_t)g->opt.block_degree,
         (uint64_t)g->scan_words.p, (uint64_t)collect_grid, (uint64_t)pull_rows};
-    // The replay of `iters` iterations, captured on first use for this graph
-    // and these buffers / options.
+    // The kernels of one iteration of the given kinds (mask), after its collect.
+    auto launch_kinds = [&](int32_t mask) {
+      if ((mask >> kSsspLbs) & 1) launch_push([](int) {});
+      if ((mask >> kSsspTier) & 1) launch_tier();
+      if (pull_alpha > 0 && ((mask >> kSsspPull) & 1)) {
+        launch_pull_rows();
+        launch_pull((const int32_t *)&ctl->pull_skip);
+      }
+    };
+    // Captures what `body` launches into L (keyed by the kernel arguments).
+    auto capture = [&](CapturedLoop &L, auto &&body) -> int {
+      L.reset();
+      hipGraph_t graph = nullptr;
+      GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      body();
+      const hipError_t le = hipGetLastError();
+      const hipError_t ce = hipStreamEndCapture(s, &graph);
+      hipError_t ie = hipErrorUnknown;
+      if (le == hipSuccess && ce == hipSuccess) ie = hipGraphInstantiate(&L.exec, graph, nullptr, nullptr, 0);
+      if (graph) (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        L.exec = nullptr;
+        return set_error(GG_ERR_HIP, "SSSP loop capture failed: %s / %s / %s", hipGetErrorString(le),
+                         hipGetErrorString(ce), hipGetErrorString(ie));
+      }
+      L.key = key;
+      return GG_OK;
+    };
+    // The general replay of `iters` iterations, captured on first use for
+    // this graph and these buffers / options.
     auto loop_exec = [&](int iters, hipGraphExec_t *out) -> int {
       CapturedLoop &L = g->sssp_loops[iters];
-      if (!L.exec || L.key != key) {
-        L.reset();
-        hipGraph_t graph = nullptr;
-        GG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        for (int it = 0; it < iters; it++) {
-          launch_collect();
-          launch_push([](int) {});
-          launch_tier();
-          if (pull_alpha > 0) {
-            launch_pull_rows();
-            launch_pull((const int32_t *)&ctl->pull_skip);
+      if (!L.exec || L.key != key)
+        GG_TRY(capture(L, [&]() {
+          for (int it = 0; it < iters; it++) {
+            launch_collect();
+            launch_kinds(kSsspMaskAll);
           }
-        }
-        const hipError_t le = hipGetLastError();
-        const hipError_t ce = hipStreamEndCapture(s, &graph);
-        hipError_t ie = hipErrorUnknown;
-        if (le == hipSuccess && ce == hipSuccess) ie = hipGraphInstantiate(&L.exec, graph, nullptr, nullptr, 0);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (ie != hipSuccess) {
-          L.exec = nullptr;
-          return set_error(GG_ERR_HIP, "SSSP loop capture failed: %s / %s / %s", hipGetErrorString(le),
-                           hipGetErrorString(ce), hipGetErrorString(ie));
-        }
-        L.key = key;
-      }
+        }));
       *out = L.exec;
       return GG_OK;
     };
-    // The first replay runs as many iterations as the previous run on this
-    // graph took (a repeated query then needs one replay and no idle
-    // iterations); later replays run kLoopIters.
+    // The replay of the last run's schedule.
+    auto scheduled = [&](hipGraphExec_t *out) -> int {
+      for (SpecLoop &sp : g->sssp_spec)
+        if (sp.loop.exec && sp.loop.key == key && sp.sched == g->sssp_sched) {
+          *out = sp.loop.exec;
+          return GG_OK;
+        }
+      SpecLoop &sp = g->sssp_spec[g->sssp_spec_next];
+      g->sssp_spec_next = (g->sssp_spec_next + 1) % kSpecCache;
+      sp.sched.clear();
+      GG_TRY(capture(sp.loop, [&]() {
+        for (uint8_t k : g->sssp_sched) {
+          const int32_t mask = k == kSsspCollectOnly ? 0 : 1 << k;
+          launch_collect(mask);
+          launch_kinds(mask);
+        }
+      }));
+      sp.sched = g->sssp_sched;
+      *out = sp.loop.exec;
+      return GG_OK;
+    };
+    // The held-back iteration of a stalled replay.
+    auto resume_exec = [&](hipGraphExec_t *out) -> int {
+      CapturedLoop &L = g->sssp_resume;
+      if (!L.exec || L.key != key)
+        GG_TRY(capture(L, [&]() {
+          hipLaunchKernelGGL(k_sssp_resume, dim3(1), dim3(64), 0, s, ctl, lb.info);
+          launch_kinds(kSsspMaskAll);
+        }));
+      *out = L.exec;
+      return GG_OK;
+    };
+    // Without a schedule, the first replay runs as many iterations as the
+    // previous run on this graph took (a run longer than kSpecMax collects);
+    // later replays run kLoopIters.
+    const bool use_sched = !g->sssp_sched.empty();
     int iters = g->sssp_steps_hint > 0 ? std::min(g->sssp_steps_hint, kSsspReplayMax) : kLoopIters;
     // After each replay, one synchronisation reads the run state and (when
     // asked for) the reach statistics, enqueued on the chance that the replay
     // ends the run (otherwise they are dropped).
-    for (;;) {
-      hipGraphExec_t exec = nullptr;
-      GG_TRY(loop_exec(iters, &exec));
-      GG_HIP(hipGraphLaunch(exec, s));
+    for (int replay = 0;; replay++) {
+      if (replay == 0 && use_sched) {
+        hipGraphExec_t exec = nullptr;
+        GG_TRY(scheduled(&exec));
+        GG_HIP(hipGraphLaunch(exec, s));
+      } else {
+        if (h.stall) {
+          hipGraphExec_t rexec = nullptr;
+          GG_TRY(resume_exec(&rexec));
+          GG_HIP(hipGraphLaunch(rexec, s));
+        }
+        hipGraphExec_t exec = nullptr;
+        GG_TRY(loop_exec(iters, &exec));
+        GG_HIP(hipGraphLaunch(exec, s));
+        iters = kLoopIters;
+      }
       GG_HIP(hipEventRecord(e1, s));
       if (stats) GG_TRY(reach_launch(g, dist, kInf));
       GG_TRY(enqueue_handoff(g, dist));  // the caller's result, from dist itself
@@ -962,10 +1073,13 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       std::memcpy(&h, g->h64() + kCounterSlots, sizeof(h));
       if (h.done) {
         g->sssp_steps_hint = h.steps;
+        if (h.steps <= kSpecMax)
+          g->sssp_sched.assign(h.kinds, h.kinds + h.steps);
+        else
+          g->sssp_sched.clear();
         g->handoff.done = g->handoff.armed;
         break;
       }
-      iters = kLoopIters;
     }
   } else {
     // Host-driven, one kernel event each: the collect (with the same

@@ -601,3 +601,52 @@ def test_sssp_pull_rows_odd_n_and_wide_weights(gpu_available, directed):
             for fl in (0, _abi.GG_INSTRUMENT):
                 dist, st = g.sssp(src, flags=fl)
                 np.testing.assert_array_equal(dist, exp)
+
+
+@pytest.mark.parametrize("undirected", [True, False])
+def test_scheduled_replays_follow_and_leave_the_last_schedule(gpu_available, undirected):
+    """A device-driven BFS / SSSP first replays the schedule (the kinds of the
+    levels / iterations) of the previous run on the same graph, each slot
+    holding only its kind's kernels, and goes on with the general loop from
+    the first level or iteration whose kind differs (the replay stalls there).
+    Sources with different schedules in turn -- the hub, a vertex of a short
+    path component, an isolated vertex (one level), a low-degree vertex, the
+    hub twice -- must all match the oracle, and BFS statistics must equal the
+    host-driven loop's, under options that change which kinds occur."""
+    r = gg.gen_rmat(14, 16, undirected=undirected, seed=61)
+    L, iso = 12, 3
+    n = r.n + L + iso
+    rows = [np.repeat(np.arange(r.n), np.diff(r.row_offsets))]
+    cols = [r.col_indices.astype(np.int64)]
+    a = np.arange(r.n, r.n + L - 1)
+    rows += [a, a + 1]
+    cols += [a + 1, a]
+    row, col = np.concatenate(rows), np.concatenate(cols)
+    order = np.lexsort((col, row))
+    row, col = row[order], col[order]
+    ro = np.zeros(n + 1, np.int64)
+    np.add.at(ro, row + 1, 1)
+    c = csr_of(np.cumsum(ro), col).with_weights("i32", 1, 64, symmetric=undirected, seed=62)
+    deg = np.diff(c.row_offsets)
+    hub = workloads.max_degree_vertex(c)
+    nz = np.flatnonzero(deg[: r.n] > 0)
+    low = int(nz[np.argmin(deg[nz])])
+    sources = (hub, hub, r.n + 3, r.n + L, low, hub, hub, r.n + L + iso - 1, low)
+    with gg.Graph(c, directed=not undirected) as g:
+        for opts in ({}, {"pull_alpha": 2, "sssp_delta": 8}, {"bfs_alpha": 1, "bfs_beta": 1000000, "pull_alpha": 0},
+                     {"pull_alpha": 1, "sssp_delta": 64}):
+            g.set_options(**gg.default_options().as_dict())
+            g.set_options(**opts)
+            for src in sources:
+                src = int(src)
+                exp_b = oracle.bfs(c.row_offsets, c.col_indices, src)
+                exp_s = oracle.sssp(c.row_offsets, c.col_indices, c.values, src)
+                d, st = g.bfs(src)
+                np.testing.assert_array_equal(d, exp_b, err_msg=str(("bfs", opts, src)))
+                d2, st2 = g.bfs(src, flags=_abi.GG_INSTRUMENT)
+                np.testing.assert_array_equal(d2, exp_b)
+                for k in ("iterations", "pull_iterations", "reached_vertices", "reached_edges", "edges_inspected"):
+                    assert st[k] == st2[k], (opts, src, k, st[k], st2[k])
+                ds, sts = g.sssp(src)
+                np.testing.assert_array_equal(ds, exp_s, err_msg=str(("sssp", opts, src)))
+                assert sts["reached_vertices"] == int(np.sum(exp_s != INF))
