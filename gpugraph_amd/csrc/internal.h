@@ -313,6 +313,7 @@ struct gg_graph {
   std::vector<uint8_t> bfs_sched, sssp_sched;
   gg::SpecLoop bfs_spec[gg::kSpecCache], sssp_spec[gg::kSpecCache];
   int bfs_spec_next = 0, sssp_spec_next = 0;  // slot the next capture replaces
+  bool sssp_sched_repeated = false;  // the last two device-driven SSSP runs took sssp_sched
   gg::DevBuf loop_ctl;
   // Aggregate words of single-pass scans across workgroups (the SSSP
   // collect: two sets, by collect parity).

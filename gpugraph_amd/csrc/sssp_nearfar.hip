@@ -1041,10 +1041,12 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       *out = L.exec;
       return GG_OK;
     };
-    // Without a schedule, the first replay runs as many iterations as the
-    // previous run on this graph took (a run longer than kSpecMax collects);
-    // later replays run kLoopIters.
-    const bool use_sched = !g->sssp_sched.empty();
+    // The schedule is replayed once two runs in a row took it (repeated
+    // queries); while schedules vary from query to query (sources in turn),
+    // a stall would cost a second synchronisation, so the first replay is
+    // then the general loop, as long as the previous run.  Later replays run
+    // kLoopIters.
+    const bool use_sched = !g->sssp_sched.empty() && g->sssp_sched_repeated;
     int iters = g->sssp_steps_hint > 0 ? std::min(g->sssp_steps_hint, kSsspReplayMax) : kLoopIters;
     // After each replay, one synchronisation reads the run state and (when
     // asked for) the reach statistics, enqueued on the chance that the replay
@@ -1071,10 +1073,18 @@ int sssp_near_far(gg_graph *g, int32_t src, int32_t *dist_out, uint32_t flags, g
       GG_TRY(counters_to_host_async(g, 34, 1ull << 17, ctl, sizeof(SsspCtl)));
       GG_HIP(hipStreamSynchronize(s));
       std::memcpy(&h, g->h64() + kCounterSlots, sizeof(h));
+      if (replay == 0 && use_sched) {
+        // The scheduled replay ended early (a stall) or ran short: the next
+        // one runs the collects the previous run still had left.
+        const int left = g->sssp_steps_hint - h.steps;
+        iters = left > 0 ? std::min(left, kSsspReplayMax) : kLoopIters;
+      }
       if (h.done) {
         g->sssp_steps_hint = h.steps;
+        const std::vector<uint8_t> ran(h.kinds, h.kinds + std::min(h.steps, kSpecMax));
+        g->sssp_sched_repeated = h.steps <= kSpecMax && ran == g->sssp_sched;
         if (h.steps <= kSpecMax)
-          g->sssp_sched.assign(h.kinds, h.kinds + h.steps);
+          g->sssp_sched = ran;
         else
           g->sssp_sched.clear();
         g->handoff.done = g->handoff.armed;
