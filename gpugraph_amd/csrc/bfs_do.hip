@@ -465,7 +465,7 @@ __global__ void k_bfs_start(int32_t *depth, int32_t n, uint32_t *visited, const 
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const long long mf = (long long)(ro[src + 1] - ro[src]);
-  for (int64_t i = t0; i < n; i += stride) depth[i] = i == src ? 0 : -1;
+  fill_except_one(depth, n, -1, src, 0, t0, stride);
   for (int64_t i = t0; i < words; i += stride) visited[i] = noin[i] | (i == (src >> 5) ? 1u << (src & 31) : 0u);
   for (int64_t i = t0; i < kCounterWords; i += stride)
     ctr[i] = i == 1 ? (unsigned long long)mf : i == 8 ? 1ull : 0ull;  // slot 8: int32 queue sizes {1, 0}

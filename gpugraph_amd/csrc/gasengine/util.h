@@ -225,6 +225,32 @@ __device__ __forceinline__ void block_append(bool keep, int32_t value, int32_t *
   __syncthreads();
 }
 
+// a[i] = v for every i < n except a[at] = av, by the threads t0, t0 +
+// stride, ...; 16-byte stores when a is 16-byte aligned (the tail and an
+// unaligned array one element per store).
+__device__ __forceinline__ void fill_except_one(int32_t *a, int64_t n, int32_t v, int64_t at, int32_t av,
+                                                int64_t t0, int64_t stride) {
+  int64_t done = 0;
+  if ((reinterpret_cast<uintptr_t>(a) & 15) == 0) {
+    const int64_t n4 = n >> 2;
+    int4 *a4 = reinterpret_cast<int4 *>(a);
+    const int64_t at4 = at >> 2;
+    const int k = (int)(at & 3);
+    for (int64_t i = t0; i < n4; i += stride) {
+      int4 x = make_int4(v, v, v, v);
+      if (i == at4) {
+        x.x = k == 0 ? av : v;
+        x.y = k == 1 ? av : v;
+        x.z = k == 2 ? av : v;
+        x.w = k == 3 ? av : v;
+      }
+      a4[i] = x;
+    }
+    done = n4 << 2;
+  }
+  for (int64_t i = done + t0; i < n; i += stride) a[i] = i == at ? av : v;
+}
+
 // ---- bitmaps (bit v of word v>>5) --------------------------------------
 
 __device__ __forceinline__ bool bit_test(const uint32_t *bits, int32_t v) {

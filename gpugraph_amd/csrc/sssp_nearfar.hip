@@ -768,7 +768,7 @@ __global__ void k_sssp_start(int32_t *dist, int32_t n, uint4 *dirty16, int64_t n
                              double bytes0) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (int64_t i = t0; i < n; i += stride) dist[i] = i == src ? 0 : kInf;
+  fill_except_one(dist, n, kInf, src, 0, t0, stride);
   const int64_t src16 = src / kCollectPer;
   const int sb = src % kCollectPer;  // byte of src within its 16
   const uint32_t bit = 1u << (8 * (sb & 3));
