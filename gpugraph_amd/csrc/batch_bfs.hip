@@ -12,10 +12,15 @@
 //   top-down   LBS over the union frontier queue (vertices with F != 0):
 //              per out-edge (u, w): bits = F[u] & ~seen[w]; atomicOr into
 //              next[w] when it adds a bit (plain pre-check first);
-//   bottom-up  one lane per vertex not yet reached by every instance, or
-//              the whole wave for in-rows longer than 32: OR of F over the
-//              in-neighbours, stopping once every missing instance is found;
-//              the owner writes next[v] with a plain store;
+//   bottom-up  one lane per vertex not yet reached by every instance: OR
+//              of F over the in-neighbours, stopping once every missing
+//              instance is found; the owner writes next[v] with a plain
+//              store.  In-rows longer than 32 are cut into chunks of at most
+//              1024 in-edges (listed once per graph) that the waves share
+//              out evenly, 64 in-edges per step, each chunk stopping early the
+//              same way and folding its bits into next[v] with an atomicOr:
+//              a wave per whole row left the waves holding the hubs (the
+//              lowest ids after renumbering) with most of the level's work;
 //   update     per vertex: next -> seen, F_next = next, the vertex joins the
 //              next union queue, and depth[i][v] = level + 1 for every
 //              instance bit i of next (a wave's lanes write instance i
@@ -74,10 +79,14 @@ __device__ __forceinline__ unsigned long long wave_or(unsigned long long x) {
   return x;
 }
 
+constexpr int32_t kMsLongRow = 32;  // in-rows longer than this are scanned in chunks by whole waves
+constexpr int32_t kMsChunk = 1024;  // in-edges per chunk
+
 __global__ void __launch_bounds__(256)
 k_ms_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, const unsigned long long *__restrict__ F,
                const unsigned long long *__restrict__ seen, unsigned long long *__restrict__ next, int32_t n,
-               unsigned long long full, unsigned long long *__restrict__ counters) {
+               unsigned long long full, const int4 *__restrict__ chunks, int64_t nchunks,
+               unsigned long long *__restrict__ counters) {
   const int lane = lane_id();
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   unsigned long long examined = 0;
@@ -90,9 +99,8 @@ k_ms_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, c
       b = co[v];
       e = co[v + 1];
     }
-    const bool todo = v < n && sv != full && e > b;
-    const bool coop = todo && e - b > 32;
-    if (todo && !coop) {
+    const bool todo = v < n && sv != full && e > b && e - b <= kMsLongRow;
+    if (todo) {
       unsigned long long acc = 0;
       int32_t k = b;
       for (; k + 4 <= e && ((acc | sv) != full); k += 4) {
@@ -107,26 +115,48 @@ k_ms_bottom_up(const int32_t *__restrict__ co, const int32_t *__restrict__ ri, c
       const unsigned long long nx = acc & ~sv;
       if (nx) next[v] = nx;
     }
-    // Long in-rows: the whole wave, 64 in-edges per step.
-    uint64_t cm = ballot(coop);
-    while (cm) {
-      const int l = __builtin_ctzll(cm);
-      cm &= cm - 1;
-      const int32_t wb = __shfl(b, l), we = __shfl(e, l);
-      const unsigned long long wsv = __shfl(sv, l);
-      unsigned long long acc = 0;
-      for (int32_t k0 = wb; k0 < we && ((acc | wsv) != full); k0 += kWave) {
-        const int32_t k = k0 + lane;
-        acc |= wave_or(k < we ? F[ri[k]] : 0ull);
-        if (lane == 0) examined += (unsigned long long)min(kWave, we - k0);
-      }
-      if (lane == l) {
-        const unsigned long long nx = acc & ~wsv;
-        if (nx) next[v] = nx;
-      }
+  }
+  // Chunks of the long in-rows (vertex, first in-edge, end): the whole wave,
+  // 64 in-edges per step; chunks of one row fold into next[v] atomically.
+  for (int64_t q = (((int64_t)blockIdx.x * blockDim.x) + threadIdx.x) >> 6; q < nchunks; q += nwaves) {
+    const int4 ch = chunks[q];
+    const unsigned long long wsv = seen[ch.x];
+    if (wsv == full) continue;
+    unsigned long long acc = 0;
+    for (int32_t k0 = ch.y; k0 < ch.z && ((acc | wsv) != full); k0 += kWave) {
+      const int32_t k = k0 + lane;
+      acc |= wave_or(k < ch.z ? F[ri[k]] : 0ull);
+      if (lane == 0) examined += (unsigned long long)min(kWave, ch.z - k0);
     }
+    const unsigned long long nx = acc & ~wsv;
+    if (lane == 0 && nx) atomicOr(&next[ch.x], nx);
   }
   block_counter_add<256>(&counters[kMsExaminedSlot], examined);
+}
+
+// The chunk list of the long in-rows, built once per graph on the host from
+// the in-edge offsets: (v, b, e, 0) per chunk of at most kMsChunk in-edges.
+int ms_long_chunks(gg_graph *g, const int4 **out, int64_t *count) {
+  if (g->ms_chunk_count < 0) {
+    const int32_t n = g->n;
+    std::vector<int32_t> co((size_t)n + 1);
+    GG_HIP(hipMemcpyAsync(co.data(), g->dg.column_offsets, 4 * ((size_t)n + 1), hipMemcpyDeviceToHost, g->stream));
+    GG_HIP(hipStreamSynchronize(g->stream));
+    std::vector<int4> ch;
+    for (int32_t v = 0; v < n; v++) {
+      const int32_t b = co[(size_t)v], e = co[(size_t)v + 1];
+      if (e - b <= kMsLongRow) continue;
+      for (int32_t k = b; k < e; k += kMsChunk) ch.push_back(make_int4(v, k, std::min(k + kMsChunk, e), 0));
+    }
+    GG_TRY(g->ms_chunks.ensure(sizeof(int4) * std::max<size_t>(ch.size(), 1)));
+    if (!ch.empty())
+      GG_HIP(hipMemcpyAsync(g->ms_chunks.p, ch.data(), sizeof(int4) * ch.size(), hipMemcpyHostToDevice, g->stream));
+    GG_HIP(hipStreamSynchronize(g->stream));
+    g->ms_chunk_count = (int64_t)ch.size();
+  }
+  *out = g->ms_chunks.as<int4>();
+  *count = g->ms_chunk_count;
+  return GG_OK;
 }
 
 // next -> seen / F_next / union queue / depths.  perm: internal -> caller id
@@ -197,6 +227,9 @@ int ms_bfs_pass(gg_graph *g, const int32_t *src_int, int32_t k, int32_t *depth, 
   GG_TRY(g->msbfs[1].ensure(16 * (size_t)n));  // F[2]
   GG_TRY(g->msbfs[2].ensure(8 * (size_t)n));  // next
   GG_TRY(g->msbfs[3].ensure(4 * (size_t)(k + 4)));  // sources, queue sizes
+  const int4 *chunks = nullptr;
+  int64_t nchunks = 0;
+  GG_TRY(ms_long_chunks(g, &chunks, &nchunks));
   unsigned long long *seen = g->msbfs[0].as<unsigned long long>();
   unsigned long long *F[2] = {g->msbfs[1].as<unsigned long long>(), g->msbfs[1].as<unsigned long long>() + n};
   unsigned long long *next = g->msbfs[2].as<unsigned long long>();
@@ -237,7 +270,7 @@ int ms_bfs_pass(gg_graph *g, const int32_t *src_int, int32_t k, int32_t *depth, 
     if (bottom_up) {
       TimedRegion t(g->prof, "bfs_batch.bottom_up");
       hipLaunchKernelGGL(k_ms_bottom_up, dim3(grid_for(((int64_t)n + 63) / 64 * 64, 256, 8192)), dim3(256), 0, s,
-                         g->dg.column_offsets, g->dg.row_indices, F[cur], seen, next, n, full, ctr);
+                         g->dg.column_offsets, g->dg.row_indices, F[cur], seen, next, n, full, chunks, nchunks, ctr);
       pulls++;
     } else {
       op.F = F[cur];

@@ -239,6 +239,8 @@ struct gg_graph {
   gg::DevBuf batch[8]; // batched-instance state
   gg::DevBuf tree[2];  // gg_bfs_tree: device depth and parent arrays
   gg::DevBuf msbfs[4]; // batched BFS: seen, frontier words (2 n), next, sizes + sources
+  gg::DevBuf ms_chunks;          // batched BFS: the long in-rows cut into chunks (built once)
+  int64_t ms_chunk_count = -1;   // -1: not built yet
   // Degree reordering (GG_GRAPH_REORDER): order[new] = old, new_id[old] =
   // new, emap[new slot] = old slot; results in internal order before mapping;
   // staging for host outputs; host copy of new_id for source ids.
