@@ -180,6 +180,41 @@ def test_bfs_batch_matches_oracle(gpu_available, undirected, reorder):
     assert st["iterations"] >= 2
 
 
+def test_bfs_batch_long_in_rows(gpu_available):
+    """Batched BFS on a directed graph whose three hubs have in-rows of 3000
+    to 5000 edges (several 1024-edge chunks each, scanned by different waves
+    and folded with atomicOr), plus a chain and an isolated vertex: 64
+    sources, every instance bit-exact against the oracle, with bottom-up
+    levels taken."""
+    rng = np.random.default_rng(71)
+    n = 20000
+    rows = [rng.integers(0, n - 20, 60000), np.repeat([5, 6, 7], 2000)]
+    cols = [rng.integers(0, n - 20, 60000), rng.integers(0, n - 20, 6000)]
+    for h, cnt in ((5, 5000), (6, 4000), (7, 3000)):  # long in-rows
+        rows.append(rng.integers(0, n - 20, cnt))
+        cols.append(np.full(cnt, h))
+    chain = np.arange(n - 20, n - 2)
+    rows += [chain, np.array([3])]
+    cols += [chain + 1, np.array([n - 20])]
+    row, col = np.concatenate(rows).astype(np.int64), np.concatenate(cols).astype(np.int64)
+    keep = row != col
+    key = np.unique(row[keep] * n + col[keep])
+    row, col = key // n, key % n
+    ro = np.zeros(n + 1, np.int64)
+    np.add.at(ro, row + 1, 1)
+    c = gg.CSR(np.cumsum(ro).astype(np.int32), col.astype(np.int32), None)
+    indeg = np.bincount(col, minlength=n)
+    assert indeg[5] > 3 * 1024 and indeg[7] > 2 * 1024
+    src = rng.choice(n - 20, 64, replace=False).astype(np.int32)
+    src[0] = n - 1  # isolated: no edge touches it
+    with gg.Graph(c, directed=True) as g:
+        _, _, depth, st = g.batch(bfs_sources=src)
+        for i in range(64):
+            np.testing.assert_array_equal(depth[i], oracle.bfs(c.row_offsets, c.col_indices, int(src[i])),
+                                          err_msg=f"instance {i}")
+    assert st["pull_iterations"] >= 1
+
+
 def _abi_flags():
     from gpugraph_amd import _abi
 
