@@ -528,6 +528,23 @@ def lattice_gas_probe(gg, workloads, np):
                           "equal_to_labelprop": bool(np.array_equal(lab, ref))}))
 
 
+def batch_bfs_probe(gg, workloads, torch, np):
+    """Only the batched 128-source BFS on the R-MAT s20 directed graph (renumbered), 3 runs, for kernel traces."""
+    c = workloads.config4_rmat(20)
+    src, _ = workloads.config4_instances(c, 128, 64)
+    with gg.Graph(c, directed=True, reorder=True) as g:
+        depth = torch.empty((len(src), c.n), dtype=torch.int32, device="cuda")
+        g.batch(bfs_sources=src[:64], flags=gg._abi.GG_OUT_DEVICE, depth_out=depth)  # warm-up
+        runs = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, _, _, st = g.batch(bfs_sources=src, flags=gg._abi.GG_OUT_DEVICE, depth_out=depth)
+            runs.append((time.perf_counter() - t0) * 1e3)
+        print(json.dumps({"batched_bfs128_ms": runs, "levels": st["iterations"],
+                          "bottom_up_levels": st["pull_iterations"], "edges_inspected": st["edges_inspected"]}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -543,6 +560,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="run only a 1 GiB device copy (PMC calibration)")
     ap.add_argument("--lattice-gas-probe", action="store_true",
                     help="run only the generic GAS CC on the 2048^2 lattice (for counter passes)")
+    ap.add_argument("--batch-bfs-probe", action="store_true",
+                    help="run only the batched 128-source BFS on the R-MAT s20 graph (for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path.",
@@ -559,6 +578,9 @@ def main():
     import gpugraph_amd as gg
     from gpugraph_amd import _abi, workloads
 
+    if args.batch_bfs_probe:
+        batch_bfs_probe(gg, workloads, torch, np)
+        return
     if args.lattice_gas_probe:
         lattice_gas_probe(gg, workloads, np)
         return

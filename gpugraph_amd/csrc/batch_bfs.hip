@@ -194,26 +194,42 @@ k_ms_update(const int32_t *__restrict__ ro, unsigned long long *__restrict__ nex
 }
 
 // Sources: seen, F and depth 0 for each instance; the union queue of the
-// distinct sources and its out-degree sum.  One thread.
-__global__ void k_ms_sources(const int32_t *__restrict__ ro, const int32_t *__restrict__ src, int32_t k,
-                             unsigned long long *__restrict__ seen, unsigned long long *__restrict__ F, int32_t n,
-                             const int32_t *__restrict__ perm, int32_t *__restrict__ depth, int32_t *__restrict__ queue,
-                             int32_t *__restrict__ qsize, unsigned long long *__restrict__ counters) {
-  int32_t q = 0;
-  unsigned long long mf = 0;
-  for (int i = 0; i < k; i++) {
-    const int32_t s = src[i];
-    if (seen[s] == 0ull) {
-      queue[q++] = s;
-      mf += (unsigned long long)(ro[s + 1] - ro[s]);
-    }
-    seen[s] |= 1ull << i;
-    F[s] |= 1ull << i;
-    depth[(int64_t)i * n + (perm ? perm[s] : s)] = 0;
+// distinct sources (in order of first appearance) and its out-degree sum.
+// One wave, lane i for instance i (k <= 64): a source repeated by several
+// instances joins the queue from its first lane, and its bits are combined
+// over those lanes before one store per word.
+__global__ void __launch_bounds__(64)
+k_ms_sources(const int32_t *__restrict__ ro, const int32_t *__restrict__ src, int32_t k,
+             unsigned long long *__restrict__ seen, unsigned long long *__restrict__ F, int32_t n,
+             const int32_t *__restrict__ perm, int32_t *__restrict__ depth, int32_t *__restrict__ queue,
+             int32_t *__restrict__ qsize, unsigned long long *__restrict__ counters) {
+  const int i = threadIdx.x;
+  const bool on = i < k;
+  const int32_t s = on ? src[i] : -1;
+  // The lanes holding the same source as this lane, and whether this lane is the first of them.
+  unsigned long long same = 0;
+  for (int j = 0; j < k; j++) {
+    const int32_t sj = __shfl(s, j);
+    if (sj == s) same |= 1ull << j;
   }
-  qsize[0] = q;
-  qsize[1] = 0;
-  counters[kMsMfSlot] = mf;
+  const bool first = on && (same & ((1ull << i) - 1ull)) == 0;
+  const unsigned long long lead = ballot(first);
+  const int32_t pos = __popcll(lead & ((1ull << i) - 1ull));
+  unsigned long long deg = 0;
+  if (first) {
+    queue[pos] = s;
+    seen[s] = same;  // seen and F start at zero
+    F[s] = same;
+    deg = (unsigned long long)(ro[s + 1] - ro[s]);
+  }
+  if (on) depth[(int64_t)i * n + (perm ? perm[s] : s)] = 0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) deg += __shfl_xor(deg, off);
+  if (i == 0) {
+    qsize[0] = __popcll(lead);
+    qsize[1] = 0;
+    counters[kMsMfSlot] = deg;
+  }
 }
 
 // One pass of up to 64 instances; depth: instance-major k x n on the device.
@@ -246,7 +262,7 @@ int ms_bfs_pass(gg_graph *g, const int32_t *src_int, int32_t k, int32_t *depth, 
   GG_HIP(fill_async<unsigned long long>(next, 0ull, n, s));
   GG_HIP(fill_async<int32_t>(depth, -1, (int64_t)k * n, s));
   GG_HIP(fill_async<unsigned long long>(ctr, 0ull, kCounterWords, s));
-  hipLaunchKernelGGL(k_ms_sources, dim3(1), dim3(1), 0, s, g->dg.row_offsets, d_src, k, seen, F[0], n, perm, depth,
+  hipLaunchKernelGGL(k_ms_sources, dim3(1), dim3(64), 0, s, g->dg.row_offsets, d_src, k, seen, F[0], n, perm, depth,
                      queue[0], qsz, ctr);
   GG_HIP(hipGetLastError());
   GG_TRY(counters_to_host_async(g, kCounterSlots, 0, qsz, 8));
