@@ -16,7 +16,10 @@
 //            scan across workgroups) and the controller (its
 //            last workgroup decides push, tiered push, pull or end).  Empty
 //            frontier with dirty vertices left: the next collect moves the
-//            bound to the bucket of the smallest distance still dirty.
+//            bound to the bucket of the smallest distance still dirty; when
+//            the last push set no distance below the bound (its smallest is
+//            kept in a counter) the collect moves the bound first and never
+//            finds that empty frontier.
 //   push     frontier edges <= m / pull_alpha: LBS partition + expand over
 //            the queue
 //            (one tiered launch instead when the frontier is small: at most
@@ -72,12 +75,31 @@ constexpr int kGatheredSlot = 5;
 constexpr int kStreamedEdgesSlot = 6;
 constexpr int kStreamedRowsSlot = 7;
 
+// Counter slot of the smallest distance an iteration's relaxations set
+// (striped minimum; the collect that follows reads it, its last workgroup
+// resets it).
+constexpr int kRelaxMinSlot = 20;
+
 struct Tally {
   unsigned long long improved = 0;
-  __device__ __forceinline__ void add() { improved++; }
+  int32_t set_min = INT_MAX;  // smallest distance this thread's improvements set
+  __device__ __forceinline__ void add(int32_t d) {
+    improved++;
+    set_min = min(set_min, d);
+  }
   // All threads of the workgroup must call it.
   __device__ __forceinline__ void flush(unsigned long long *counters) {
     block_counter_add<kExpandBlock>(&counters[4], improved);
+    __shared__ int32_t s_min[kExpandBlock / kWave];
+    int32_t mn = set_min;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mn = min(mn, __shfl_xor(mn, off));
+    if (lane_id() == 0) s_min[threadIdx.x / kWave] = mn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < kExpandBlock / kWave; w++) mn = min(mn, s_min[w]);
+      if (mn != INT_MAX) atomicMin(stripe_of(&counters[kRelaxMinSlot]), (unsigned long long)mn);
+    }
   }
 };
 
@@ -125,7 +147,7 @@ struct PushOpT {
     const int32_t nd = (int32_t)l.nd;
     const int32_t old = atomicMin(&dist[l.nb], nd);
     if (nd >= old) return false;
-    tally.add();
+    tally.add(nd);
     dirty[l.nb] = 1;
     return false;
   }
@@ -189,7 +211,7 @@ struct PullOp {
       old = atomicMin(&dist[v], acc);
       if (acc >= old) return;
     }
-    tally.add();
+    tally.add(acc);
     dirty[v] = 1;
   }
   __device__ __forceinline__ void finish() {
@@ -251,7 +273,7 @@ struct PackedPullOp {
       old = atomicMin(&dist[v], acc);
       if (acc >= old) return;
     }
-    tally.add();
+    tally.add(acc);
     dirty[v] = 1;
   }
   __device__ __forceinline__ void finish() {
@@ -601,10 +623,20 @@ __global__ void __launch_bounds__(kCollectBlock) k_sssp_collect(CollectArgs a) {
   int32_t bound = c->bound;
   const int64_t G = gridDim.x, b = blockIdx.x;
   const bool last = b == G - 1;
-  if (c->advance) {
+  // The smallest distance the last push set (INT_MAX: it improved nothing).
+  // Every dirty vertex is either left from the previous collect (at or
+  // above the bound then) or was improved since, to at least that distance;
+  // so when it is at or above the bound, nothing below the bound is dirty,
+  // and the bound moves now (the collect that would find nothing is
+  // skipped), to the bucket of the smallest dirty distance.  Not after a
+  // pull, which is not followed by buckets.
+  unsigned long long rmin = a.ctr[kRelaxMinSlot];
+  for (int q = 1; q < kStripes; q++) rmin = min(rmin, a.ctr[kRelaxMinSlot + q * kCounterSlots]);
+  const bool skip_empty = !c->advance && c->iters > 0 && !c->prev_pull && rmin >= (unsigned long long)bound;
+  if (c->advance || skip_empty) {
     const unsigned long long *prev = a.ctr + kLeftSlot + 2 * (par ^ 1);
     const unsigned long long left = stripe_sum(prev);
-    if (left == 0) {  // nothing dirty anywhere: the run is over
+    if (left == 0 && (c->advance || rmin == (unsigned long long)INT_MAX)) {  // nothing dirty anywhere: the run is over
       if (last && threadIdx.x == 0) {
         if (c->steps < kSpecMax) c->kinds[c->steps] = kSsspCollectOnly;
         c->steps++;
@@ -614,6 +646,7 @@ __global__ void __launch_bounds__(kCollectBlock) k_sssp_collect(CollectArgs a) {
     }
     unsigned long long dm = prev[1];
     for (int q = 1; q < kStripes; q++) dm = min(dm, prev[1 + q * kCounterSlots]);
+    if (skip_empty) dm = min(dm, rmin);
     bound = (int32_t)min<int64_t>(((int64_t)dm / a.delta + 1) * a.delta, kInf);
   }
   // This workgroup's word of the next collect's scan (the previous collect,
@@ -673,6 +706,8 @@ __global__ void __launch_bounds__(kCollectBlock) k_sssp_collect(CollectArgs a) {
     if (threadIdx.x < kStripes * 2) {
       const int q = threadIdx.x >> 1, k = threadIdx.x & 1;
       a.ctr[q * kCounterSlots + kLeftSlot + 2 * (par ^ 1) + k] = k ? (unsigned long long)INT_MAX : 0ull;
+    } else if (threadIdx.x < kStripes * 3) {  // the next push's smallest distance set
+      a.ctr[(threadIdx.x - kStripes * 2) * kCounterSlots + kRelaxMinSlot] = (unsigned long long)INT_MAX;
     }
     if (threadIdx.x == 0) {
       const uint64_t incl = excl + agg;
@@ -779,7 +814,8 @@ __global__ void k_sssp_start(int32_t *dist, int32_t n, uint4 *dirty16, int64_t n
   }
   for (int64_t i = t0; i < kCounterWords; i += stride) {
     const int slot = (int)(i % kCounterSlots);
-    ctr[i] = slot == kLeftSlot + 1 || slot == kLeftSlot + 3 ? (unsigned long long)INT_MAX : 0ull;
+    ctr[i] = slot == kLeftSlot + 1 || slot == kLeftSlot + 3 || slot == kRelaxMinSlot ? (unsigned long long)INT_MAX
+                                                                                       : 0ull;
   }
   for (int64_t i = t0; i < status_words; i += stride) status[i] = 0ull;
   if (t0 == 0) {
